@@ -1,0 +1,581 @@
+// C ABI (include/cgfdm.h) over the HIP kernels: context lifetime, the
+// stage/step driver in the order of the reference time loop
+// (srcd/main.cu:420-535), field access and output files.
+#include "../../include/cgfdm.h"
+#include "cgfdm_internal.h"
+#include "netcdf_classic.h"
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <sys/stat.h>
+
+using namespace cgfdm;
+
+struct cgfdm_ctx {
+  Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_err = "";
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+// FD operator direction table, main.cu:86-94; negated after every stage.
+const int FD_FLAGS[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};
+
+void stage_flags(int it, int irk, int& fx, int& fy, int& fz) {
+  int s = (irk % 2) ? -1 : 1;
+  fx = FD_FLAGS[it % 8][0] * s;
+  fy = FD_FLAGS[it % 8][1] * s;
+  fz = FD_FLAGS[it % 8][2] * s;
+}
+
+// Stage buffers: S = step start (RK mW), A/B = stage values, T = RK
+// accumulator (tW).  Stage 0: S->A, 1: A->B, 2: B->A, 3: A->S.
+StageBufs bufs_for(Ctx& c, int irk) {
+  StageBufs b;
+  float* in[4] = {c.S, c.A, c.B, c.A};
+  float* out[4] = {c.A, c.B, c.A, c.S};
+  b.Win = in[irk];
+  b.Wout = out[irk];
+  b.S = c.S;
+  b.T = c.Tacc;
+  return b;
+}
+
+int check_hip(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CGFDM_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+struct FieldRef {
+  void* ptr = nullptr;
+  size_t count = 0;
+  int dtype = 0;  // 0 f32, 1 f64, 2 i32
+};
+
+FieldRef field(Ctx& c, const std::string& n) {
+  FieldRef r;
+  const long nxyz = c.d.nxyz, nyz = c.d.nyz, n2 = (long)c.d.nj * c.d.nk;
+  FaultDev& F = c.f;
+  auto f32 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 0; };
+  auto f64 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 1; };
+  auto i32 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 2; };
+  if (n == "W") f32(c.cur, 9 * nxyz);
+  else if (n == "hW") f32(c.hW, 9 * nxyz);
+  else if (n == "M") f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz);
+  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; }
+  else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);
+  else if (n == "matVy2Vz") f32(c.matVy2Vz, (size_t)c.d.nx * c.d.ny * 9);
+  else if (n == "Bx") f32(c.pml.B[0], c.d.ni);
+  else if (n == "By") f32(c.pml.B[1], c.d.nj);
+  else if (n == "Bz") f32(c.pml.B[2], c.d.nk);
+  else if (n == "Ax") f32(c.pml.A[0], c.d.ni);
+  else if (n == "Ay") f32(c.pml.A[1], c.d.nj);
+  else if (n == "Az") f32(c.pml.A[2], c.d.nk);
+  else if (n == "Dx") f32(c.pml.D[0], c.d.ni);
+  else if (n == "Dy") f32(c.pml.D[1], c.d.nj);
+  else if (n == "Dz") f32(c.pml.D[2], c.d.nk);
+  else if (n == "cerjan_x") f32(c.cj[0], c.d.nx);
+  else if (n == "cerjan_y") f32(c.cj[1], c.d.ny);
+  else if (n == "cerjan_z") f32(c.cj[2], c.d.nz);
+  else if (n.size() == 5 && n.compare(0, 3, "PML") == 0) {
+    int f = (n[3] - 'x') * 2 + (n[4] - '1');
+    if (f >= 0 && f < 6 && c.pml.Y[f]) {
+      int a = f / 2;
+      long cnt = 9L * c.pml.N * (a == 0 ? (long)c.d.nj * c.d.nk : a == 1 ? (long)c.d.nk * c.d.ni : (long)c.d.ni * c.d.nj);
+      f32(c.pml.Y[f], cnt);
+    }
+  } else if (c.p.Rupture && c.setup_done) {
+    if (n == "FW") f32(F.W, 18 * nyz);
+    else if (n == "FhW") f32(F.hW, 18 * nyz);
+    else if (n == "FmW") f32(F.mW, 18 * nyz);
+    else if (n == "T11") f32(F.T11, 7 * nyz);
+    else if (n == "T12") f32(F.T12, 7 * nyz);
+    else if (n == "T13") f32(F.T13, 7 * nyz);
+    else if (n == "hT11") f32(F.hT11, nyz);
+    else if (n == "hT12") f32(F.hT12, nyz);
+    else if (n == "hT13") f32(F.hT13, nyz);
+    else if (n == "slabH") f32(c.slabH, 7 * 9 * nyz);
+    else if (n == "Tn") f32(F.Tn, n2);
+    else if (n == "Ts1") f32(F.Ts1, n2);
+    else if (n == "Ts2") f32(F.Ts2, n2);
+    else if (n == "tTn") f32(F.tTn, n2);
+    else if (n == "tTs1") f32(F.tTs1, n2);
+    else if (n == "tTs2") f32(F.tTs2, n2);
+    else if (n == "Vs1") f32(F.Vs1, n2);
+    else if (n == "Vs2") f32(F.Vs2, n2);
+    else if (n == "hslip") f32(F.hslip, n2);
+    else if (n == "slip") f64(F.slip, n2);
+    else if (n == "State") f64(F.State, n2);
+    else if (n == "init_t0") f32(F.init_t0, n2);
+    else if (n == "friction") f32(F.friction, n2);
+    else if (n == "rup_sensor") f32(F.rup_sensor, n2);
+    else if (n == "str_init_x") f32(F.str_init_x, n2);
+    else if (n == "str_init_y") f32(F.str_init_y, n2);
+    else if (n == "str_init_z") f32(F.str_init_z, n2);
+    else if (n == "str_peak") f32(F.str_peak, n2);
+    else if (n == "C0") f32(F.C0, n2);
+    else if (n == "a") f32(F.a, n2);
+    else if (n == "b") f32(F.b, n2);
+    else if (n == "Vw") f32(F.Vw, n2);
+    else if (n == "vec_n") f32(F.vec_n, 3 * nyz);
+    else if (n == "vec_s1") f32(F.vec_s1, 3 * nyz);
+    else if (n == "vec_s2") f32(F.vec_s2, 3 * nyz);
+    else if (n == "rho_f") f32(F.rho_f, 2 * nyz);
+    else if (n == "matT1toVxm") f32(F.matT1toVx, 9 * nyz);
+    else if (n == "matT1toVxp") f32(F.matT1toVx + 9 * nyz, 9 * nyz);
+    else if (n == "matVytoVxm") f32(F.matVytoVx, 9 * nyz);
+    else if (n == "matVytoVxp") f32(F.matVytoVx + 9 * nyz, 9 * nyz);
+    else if (n == "matVztoVxm") f32(F.matVztoVx, 9 * nyz);
+    else if (n == "matVztoVxp") f32(F.matVztoVx + 9 * nyz, 9 * nyz);
+    else if (n == "matVx2Vz1") f32(F.matVx2Vz_f, 9 * c.d.ny);
+    else if (n == "matVx2Vz2") f32(F.matVx2Vz_f + 9 * c.d.ny, 9 * c.d.ny);
+    else if (n == "matVy2Vz1") f32(F.matVy2Vz_f, 9 * c.d.ny);
+    else if (n == "matVy2Vz2") f32(F.matVy2Vz_f + 9 * c.d.ny, 9 * c.d.ny);
+    else if (n.size() == 13 && n.compare(0, 11, "matPlus2Min") == 0) f32(F.matP2Mf + (n[11] - '1') * 9 * c.d.ny, 9 * c.d.ny);
+    else if (n.size() == 13 && n.compare(0, 11, "matMin2Plus") == 0) f32(F.matM2Pf + (n[11] - '1') * 9 * c.d.ny, 9 * c.d.ny);
+    else if (n.size() == 5 && n[0] == 'D' && n[3] == '_') {
+      int a = n[1] - '1', b = n[2] - '1', m = n[4] - '1';
+      if (a >= 0 && a < 3 && b >= 0 && b < 3 && m >= 0 && m < 2) f32(F.Dmat + (long)(m * 9 + a * 3 + b) * nyz * 9, 9 * nyz);
+    }
+    else if (n == "united") i32(F.united, n2);
+    else if (n == "faultgrid") i32(F.faultgrid, n2);
+    else if (n == "rup_index_y") i32(F.rup_index_y, n2);
+    else if (n == "rup_index_z") i32(F.rup_index_z, n2);
+    else if (n == "init_t0_flag") i32(F.init_t0_flag, n2);
+    else if (n == "flag_rup") i32(F.flag_rup, n2);
+  }
+  return r;
+}
+
+size_t dsize(int dt) { return dt == 1 ? 8 : dt == 3 ? 2 : 4; }
+
+// ---------------------------------------------------------------------------
+// Output files (io_fault.cu:16-275, io_wave_xy.cu, io_wave_xz.cu, io_wave_yz.cu)
+struct OutFiles {
+  NcFile fault, xy, xz, yz;
+  std::vector<int> fv;  // record var ids of the fault file
+  int f_init_t0 = -1;
+  int wxy[3], wxz[3], wyz[3];
+};
+
+}  // namespace
+
+namespace cgfdm {
+struct NcOut : OutFiles {};
+}  // namespace cgfdm
+
+extern "C" {
+
+const char* cgfdm_last_error(void) { return g_err.c_str(); }
+
+int cgfdm_create_from_string(const char* json_text, cgfdm_ctx** out) {
+  if (!json_text || !out) return fail(CGFDM_EPARAM, "null argument");
+  cgfdm_ctx* h = new cgfdm_ctx();
+  std::string err;
+  if (parse_params_json(json_text, h->c.p, err)) {
+    delete h;
+    return fail(CGFDM_EJSON, err);
+  }
+  *out = h;
+  return 0;
+}
+
+int cgfdm_create(const char* path, cgfdm_ctx** out) {
+  if (!path || !out) return fail(CGFDM_EPARAM, "null argument");
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return fail(CGFDM_EFILE, std::string("Error at opening the configuration file ") + path);
+  std::stringstream ss;
+  ss << in.rdbuf();
+  return cgfdm_create_from_string(ss.str().c_str(), out);
+}
+
+int cgfdm_set_param(cgfdm_ctx* h, const char* key, double v) {
+  if (!h || !key) return fail(CGFDM_EPARAM, "null argument");
+  if (h->c.setup_done) return fail(CGFDM_ESTATE, "parameters are fixed after setup");
+  if (h->c.p.set(key, v)) return fail(CGFDM_EPARAM, std::string("unknown parameter ") + key);
+  h->c.p.derive();
+  return 0;
+}
+
+int cgfdm_get_param(cgfdm_ctx* h, const char* key, double* v) {
+  if (!h || !key || !v) return fail(CGFDM_EPARAM, "null argument");
+  h->c.p.derive();
+  if (h->c.p.get(key, v)) return fail(CGFDM_EPARAM, std::string("unknown parameter ") + key);
+  return 0;
+}
+
+int cgfdm_destroy(cgfdm_ctx* h) {
+  if (!h) return 0;
+  Ctx& c = h->c;
+  if (c.out) { cgfdm_close_outputs(h); }
+  if (c.setup_done) hipSetDevice(c.device);
+  if (c.stream) { hipStreamSynchronize(c.stream); hipStreamDestroy(c.stream); }
+  for (void* p : c.allocations) hipFree(p);
+  if (c.hW) hipFree(c.hW);
+  delete h;
+  return 0;
+}
+
+int cgfdm_set_fault_geometry(cgfdm_ctx* h, const float* x, const float* y, const float* z, size_t n) {
+  if (!h || !x || !y || !z) return fail(CGFDM_EPARAM, "null argument");
+  h->c.gx.assign(x, x + n); h->c.gy.assign(y, y + n); h->c.gz.assign(z, z + n);
+  return 0;
+}
+
+int cgfdm_set_coord3d(cgfdm_ctx* h, const double* xyz, size_t n) {
+  if (!h || !xyz) return fail(CGFDM_EPARAM, "null argument");
+  h->c.coord3d.assign(xyz, xyz + n);
+  return 0;
+}
+
+int cgfdm_set_media1d(cgfdm_ctx* h, int n, const float* depth, const float* rho, const float* vp, const float* vs) {
+  if (!h || n <= 0 || !depth || !rho || !vp || !vs) return fail(CGFDM_EPARAM, "bad media1d arguments");
+  Ctx& c = h->c;
+  c.m1d_depth.assign(depth, depth + n); c.m1d_rho.assign(rho, rho + n);
+  c.m1d_vp.assign(vp, vp + n); c.m1d_vs.assign(vs, vs + n);
+  return 0;
+}
+
+int cgfdm_set_init_stress(cgfdm_ctx* h, const char* name, const float* data, size_t n) {
+  if (!h || !name || !data) return fail(CGFDM_EPARAM, "null argument");
+  static const char* ok[] = {"Tx", "Ty", "Tz", "dTx", "dTy", "dTz", "mu_s", "mu_d", "Dc", "C0", "a", "b", "L", "Vw", "State"};
+  for (const char* s : ok)
+    if (std::strcmp(s, name) == 0) {
+      h->c.stress_in[name] = std::vector<float>(data, data + n);
+      return 0;
+    }
+  return fail(CGFDM_EPARAM, std::string("unknown stress variable ") + name);
+}
+
+int cgfdm_setup(cgfdm_ctx* h) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (c.setup_done) return fail(CGFDM_ESTATE, "setup already done");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CGFDM_EHIP, "no HIP device");
+  c.device = ndev > 1 ? c.p.igpu % ndev : 0;
+  hipSetDevice(c.device);
+  std::string err;
+  int rc = host_setup(c, err);
+  if (rc) {
+    int code = rc == 5 ? CGFDM_ECFL : rc == 4 ? CGFDM_ENETCDF : rc == 2 ? CGFDM_EFILE : rc == 6 ? CGFDM_EHIP : CGFDM_EPARAM;
+    return fail(code, err);
+  }
+  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return fail(CGFDM_EHIP, "stream creation failed");
+  c.setup_done = true;
+  return 0;
+}
+
+int cgfdm_stage(cgfdm_ctx* h, int it, int irk) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "stage before setup");
+  if (irk < 0 || irk > 3 || it < 0) return fail(CGFDM_EPARAM, "bad stage index");
+  int fx, fy, fz;
+  stage_flags(it, irk, fx, fy, fz);
+  StageBufs b = bufs_for(c, irk);
+  if (c.p.Rupture) launch_fault_pre(c, b.Win, it, irk, fx, fy, fz);
+  launch_wave_stage(c, b, irk, fx, fy, fz, false, nullptr);
+  if (c.p.Rupture) {
+    launch_fault_deriv(c, b.Win, fx, fy, fz);
+    launch_fault_post(c, b, irk);
+  }
+  c.cur = b.Wout;
+  return check_hip("stage launch");
+}
+
+int cgfdm_step(cgfdm_ctx* h, int it) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "step before setup");
+  if (c.p.Rupture) launch_rup_sensor(c);
+  for (int irk = 0; irk < 4; irk++) {
+    int rc = cgfdm_stage(h, it, irk);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int cgfdm_synchronize(cgfdm_ctx* h) {
+  if (!h || !h->c.stream) return fail(CGFDM_ESTATE, "no stream");
+  hipError_t e = hipStreamSynchronize(h->c.stream);
+  if (e != hipSuccess) return fail(CGFDM_EHIP, hipGetErrorString(e));
+  return 0;
+}
+
+int cgfdm_run(cgfdm_ctx* h, int it0, int nt, int write_outputs) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  for (int it = it0; it < it0 + nt; it++) {
+    if (write_outputs) {
+      int rc = cgfdm_write_outputs(h, it);
+      if (rc) return rc;
+    }
+    int rc = cgfdm_step(h, it);
+    if (rc) return rc;
+  }
+  return cgfdm_synchronize(h);
+}
+
+int cgfdm_wave_stage(cgfdm_ctx* h, int irk, int fx, int fy, int fz) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  if (irk < 0 || irk > 3) return fail(CGFDM_EPARAM, "bad irk");
+  StageBufs b = bufs_for(c, irk);
+  launch_wave_stage(c, b, irk, fx, fy, fz, false, nullptr);
+  c.cur = b.Wout;
+  return check_hip("wave_stage launch");
+}
+
+int cgfdm_wave_rhs(cgfdm_ctx* h, int fx, int fy, int fz) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  if (!c.hW) {
+    if (hipMalloc(&c.hW, 9 * c.d.nxyz * sizeof(float)) != hipSuccess) return fail(CGFDM_EHIP, "hipMalloc hW");
+    launch_fill(c, c.hW, 9 * c.d.nxyz, 0.0f);
+  }
+  StageBufs b{c.cur, nullptr, c.S, c.Tacc};
+  launch_wave_stage(c, b, 0, fx, fy, fz, true, c.hW);
+  return check_hip("wave_rhs launch");
+}
+
+int cgfdm_field_info(cgfdm_ctx* h, const char* name, size_t* count, int* dtype) {
+  if (!h || !name) return fail(CGFDM_EPARAM, "null argument");
+  FieldRef r = field(h->c, name);
+  if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
+  if (count) *count = r.count;
+  if (dtype) *dtype = r.dtype;
+  return 0;
+}
+
+int cgfdm_get_field(cgfdm_ctx* h, const char* name, void* dst, size_t bytes) {
+  if (!h || !name || !dst) return fail(CGFDM_EPARAM, "null argument");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  FieldRef r = field(c, name);
+  if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
+  if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
+  hipStreamSynchronize(c.stream);
+  if (hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  return 0;
+}
+
+int cgfdm_put_field(cgfdm_ctx* h, const char* name, const void* src, size_t bytes) {
+  if (!h || !name || !src) return fail(CGFDM_EPARAM, "null argument");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  std::string n(name);
+  if (n == "W") {  // a new wavefield starts a step: it lands in the step-start buffer
+    c.cur = c.S;
+  }
+  FieldRef r = field(c, name);
+  if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
+  if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
+  hipStreamSynchronize(c.stream);
+  if (hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  return 0;
+}
+
+int cgfdm_get_dims(cgfdm_ctx* h, int dims[8]) {
+  if (!h || !dims) return fail(CGFDM_EPARAM, "null argument");
+  Params& p = h->c.p;
+  p.derive();
+  dims[0] = p.ni; dims[1] = p.nj; dims[2] = p.nk; dims[3] = p.nx; dims[4] = p.ny; dims[5] = p.nz;
+  dims[6] = p.nx / 2; dims[7] = p.NT;
+  return 0;
+}
+
+int cgfdm_newton_stats(cgfdm_ctx* h, double* solves, double* iterations, double* fallbacks) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  unsigned long long s[4] = {0, 0, 0, 0};
+  if (c.setup_done && c.p.Rupture && c.f.newton_stats) {
+    hipStreamSynchronize(c.stream);
+    hipMemcpy(s, c.f.newton_stats, sizeof s, hipMemcpyDeviceToHost);
+  }
+  if (solves) *solves = (double)s[0];
+  if (iterations) *iterations = (double)s[1];
+  if (fallbacks) *fallbacks = (double)s[2];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+int cgfdm_open_outputs(cgfdm_ctx* h) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  if (c.out) return 0;
+  mkdir(c.p.OUT.c_str(), 0755);
+  NcOut* o = new NcOut();
+  std::string err;
+  const int nj = c.d.nj, nk = c.d.nk, ni = c.d.ni, nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
+  const long nxyz = c.d.nxyz, nyz = c.d.nyz;
+  auto cpos = [&](int i, int j, int k) { return (long)j + (long)k * ny + (long)i * nyz; };
+  if (c.p.Rupture) {
+    NcFile& f = o->fault;
+    if (!f.create(c.p.OUT + "/fault_mpi000000.nc", err)) { delete o; return fail(CGFDM_ENETCDF, err); }
+    int dt = f.def_dim("nt", 0), dz = f.def_dim("nz", nk), dy = f.def_dim("ny", nj);
+    std::vector<int> d2 = {dz, dy}, d3 = {dt, dz, dy};
+    int vx = f.def_var("x", NC_FLOAT_, d2), vy = f.def_var("y", NC_FLOAT_, d2), vz = f.def_var("z", NC_FLOAT_, d2);
+    o->f_init_t0 = f.def_var("init_t0", NC_FLOAT_, d2);
+    int vc0 = f.def_var("C0", NC_FLOAT_, d2), vpk = f.def_var("str_peak", NC_FLOAT_, d2), vun = f.def_var("united", NC_INT_, d2);
+    int va = f.def_var("a", NC_FLOAT_, d2), vb = f.def_var("b", NC_FLOAT_, d2), vw = f.def_var("Vw", NC_FLOAT_, d2);
+    for (const char* nm : {"Vs1", "Vs2", "tn", "ts1", "ts2", "Us0", "State"}) o->fv.push_back(f.def_var(nm, NC_FLOAT_, d3));
+    o->fv.push_back(f.def_var("rup_index_y", NC_INT_, d3));
+    o->fv.push_back(f.def_var("rup_index_z", NC_INT_, d3));
+    for (const char* nm : {"rup_sensor", "TP_T", "TP_P"}) o->fv.push_back(f.def_var(nm, NC_FLOAT_, d3));
+    if (!f.end_def(err)) { delete o; return fail(CGFDM_ENETCDF, err); }
+    std::vector<float> buf((size_t)nj * nk);
+    std::vector<int> ib((size_t)nj * nk);
+    const double* C = c.hostC.data();
+    for (int comp = 0; comp < 3; comp++) {
+      for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++) buf[j + (long)k * nj] = (float)C[comp * nxyz + cpos(c.d.i0, j + 3, k + 3)];
+      f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, buf.data());
+    }
+    auto put2 = [&](const char* fld, int vid) {
+      FieldRef r = field(c, fld);
+      hipMemcpy(buf.data(), r.ptr, buf.size() * 4, hipMemcpyDeviceToHost);
+      f.put_var(vid, buf.data());
+    };
+    put2("C0", vc0); put2("str_peak", vpk); put2("a", va); put2("b", vb); put2("Vw", vw);
+    hipMemcpy(ib.data(), c.f.united, ib.size() * 4, hipMemcpyDeviceToHost);
+    f.put_var(vun, ib.data());
+  }
+  // wave slices: xy at global k, xz at global j, yz at global i
+  auto slice_file = [&](NcFile& f, const std::string& name, int d1n, int d2n, const char* n1, const char* n2, int* vids,
+                        int axis, int idx) -> bool {
+    if (!f.create(c.p.OUT + "/" + name, err)) return false;
+    int dt = f.def_dim("nt", 0), da = f.def_dim(n1, d1n), db = f.def_dim(n2, d2n);
+    std::vector<int> d2 = {da, db}, d3 = {dt, da, db};
+    int vx = f.def_var("x", NC_FLOAT_, d2), vy = f.def_var("y", NC_FLOAT_, d2), vz = f.def_var("z", NC_FLOAT_, d2);
+    vids[0] = f.def_var("Vx", NC_FLOAT_, d3); vids[1] = f.def_var("Vy", NC_FLOAT_, d3); vids[2] = f.def_var("Vz", NC_FLOAT_, d3);
+    if (!f.end_def(err)) return false;
+    std::vector<float> buf((size_t)d1n * d2n);
+    const double* C = c.hostC.data();
+    for (int comp = 0; comp < 3; comp++) {
+      for (int a = 0; a < d1n; a++)
+        for (int b = 0; b < d2n; b++) {
+          long p;
+          if (axis == 2) p = cpos(a + 3, b + 3, idx + 3);       // (nx, ny) at k
+          else if (axis == 1) p = cpos(a + 3, idx + 3, b + 3);  // (nx, nz) at j
+          else p = cpos(idx + 3, b + 3, a + 3);                 // (nz, ny) at i
+          buf[(long)a * d2n + b] = (float)C[comp * nxyz + p];
+        }
+      f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, buf.data());
+    }
+    return true;
+  };
+  int kz = std::min(std::max(c.p.EXPORT_WAVE_SLICE_Z, 0), nk - 1);
+  int jy = std::min(std::max(c.p.EXPORT_WAVE_SLICE_Y, 0), nj - 1);
+  int ix = std::min(std::max(c.p.EXPORT_WAVE_SLICE_X, 0), ni - 1);
+  if (!slice_file(o->xy, "wave_xy_mpi000000.nc", ni, nj, "nx", "ny", o->wxy, 2, kz) ||
+      !slice_file(o->xz, "wave_xz_mpi000000.nc", ni, nk, "nx", "nz", o->wxz, 1, jy) ||
+      !slice_file(o->yz, "wave_yz_mpi000000.nc", nk, nj, "nz", "ny", o->wyz, 0, ix)) {
+    delete o;
+    return fail(CGFDM_ENETCDF, err);
+  }
+  (void)nx; (void)nz;
+  c.out = o;
+  return 0;
+}
+
+int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.out) {
+    int rc = cgfdm_open_outputs(h);
+    if (rc) return rc;
+  }
+  int skip = std::max(1, c.p.EXPORT_TIME_SKIP);
+  if (it % skip) return 0;
+  long rec = it / skip;
+  NcOut* o = c.out;
+  hipStreamSynchronize(c.stream);
+  const int nj = c.d.nj, nk = c.d.nk, ni = c.d.ni, ny = c.d.ny;
+  const long nxyz = c.d.nxyz, nyz = c.d.nyz;
+  if (c.p.Rupture) {
+    long n2 = (long)nj * nk;
+    std::vector<float> buf(n2);
+    std::vector<double> db(n2);
+    std::vector<int> ib(n2);
+    const char* names[7] = {"Vs1", "Vs2", "Tn", "Ts1", "Ts2", "slip", "State"};
+    for (int v = 0; v < 7; v++) {
+      FieldRef r = field(c, names[v]);
+      if (r.dtype == 1) {
+        hipMemcpy(db.data(), r.ptr, n2 * 8, hipMemcpyDeviceToHost);
+        for (long q = 0; q < n2; q++) buf[q] = (float)db[q];
+      } else {
+        hipMemcpy(buf.data(), r.ptr, n2 * 4, hipMemcpyDeviceToHost);
+      }
+      o->fault.put_record(o->fv[v], rec, buf.data());
+    }
+    hipMemcpy(buf.data(), c.f.init_t0, n2 * 4, hipMemcpyDeviceToHost);
+    o->fault.put_var(o->f_init_t0, buf.data());
+    hipMemcpy(ib.data(), c.f.rup_index_y, n2 * 4, hipMemcpyDeviceToHost);
+    o->fault.put_record(o->fv[7], rec, ib.data());
+    hipMemcpy(ib.data(), c.f.rup_index_z, n2 * 4, hipMemcpyDeviceToHost);
+    o->fault.put_record(o->fv[8], rec, ib.data());
+    hipMemcpy(buf.data(), c.f.rup_sensor, n2 * 4, hipMemcpyDeviceToHost);
+    o->fault.put_record(o->fv[9], rec, buf.data());
+    std::fill(buf.begin(), buf.end(), 0.0f);  // thermal pressurization is not modelled
+    o->fault.put_record(o->fv[10], rec, buf.data());
+    o->fault.put_record(o->fv[11], rec, buf.data());
+    o->fault.sync();
+  }
+  // slices from the current wavefield
+  auto put_slice = [&](NcFile& f, int* vids, int axis, int idx, int d1n, int d2n) {
+    std::vector<float> buf((size_t)d1n * d2n);
+    std::vector<float> line;
+    for (int comp = 0; comp < 3; comp++) {
+      if (axis == 2) {  // xy: rows of ny for each i
+        line.resize(ny);
+        for (int a = 0; a < d1n; a++) {
+          hipMemcpy(line.data(), c.cur + comp * nxyz + (long)(idx + 3) * ny + (long)(a + 3) * nyz, ny * 4, hipMemcpyDeviceToHost);
+          std::memcpy(&buf[(long)a * d2n], &line[3], d2n * 4);
+        }
+      } else if (axis == 1) {  // xz
+        std::vector<float> plane(nyz);
+        for (int a = 0; a < d1n; a++) {
+          hipMemcpy(plane.data(), c.cur + comp * nxyz + (long)(a + 3) * nyz, nyz * 4, hipMemcpyDeviceToHost);
+          for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[(idx + 3) + (long)(b + 3) * ny];
+        }
+      } else {  // yz plane at i
+        std::vector<float> plane(nyz);
+        hipMemcpy(plane.data(), c.cur + comp * nxyz + (long)(idx + 3) * nyz, nyz * 4, hipMemcpyDeviceToHost);
+        for (int a = 0; a < d1n; a++)
+          for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[(b + 3) + (long)(a + 3) * ny];
+      }
+      f.put_record(vids[comp], rec, buf.data());
+    }
+    f.sync();
+  };
+  int kz = std::min(std::max(c.p.EXPORT_WAVE_SLICE_Z, 0), nk - 1);
+  int jy = std::min(std::max(c.p.EXPORT_WAVE_SLICE_Y, 0), nj - 1);
+  int ix = std::min(std::max(c.p.EXPORT_WAVE_SLICE_X, 0), ni - 1);
+  put_slice(o->xy, o->wxy, 2, kz, ni, nj);
+  put_slice(o->xz, o->wxz, 1, jy, ni, nk);
+  put_slice(o->yz, o->wyz, 0, ix, nk, nj);
+  return 0;
+}
+
+int cgfdm_close_outputs(cgfdm_ctx* h) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.out) return 0;
+  c.out->fault.close(); c.out->xy.close(); c.out->xz.close(); c.out->yz.close();
+  delete c.out;
+  c.out = nullptr;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// Internal declarations of the MI355X CGFDM solver (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <map>
+
+namespace cgfdm {
+
+// ---------------------------------------------------------------------------
+// Parameters: the key set and defaults of get_params (set_params.cu:56-298)
+// plus runtime forms of the reference's compile-time feature macros.
+struct Params {
+  int NX = 100, NY = 90, NZ = 80;
+  int PX = 1, PY = 1, PZ = 1, igpu = 0;
+  double TMAX = 1.0, DT = 0.01, DH = 100.0;
+  int INPORT_GRID_TYPE = 0, INPORT_STRESS_TYPE = 0, Friction_type = 0;
+  int EXPORT_TIME_SKIP = 1, EXPORT_WAVE_SLICE_X = 0, EXPORT_WAVE_SLICE_Y = 0, EXPORT_WAVE_SLICE_Z = 0;
+  int PML_N = 20;  // PML defaults are re-set after the struct defaults (set_params.cu:264-267)
+  double PML_velocity = 3000.0, PML_bmax = 3.0, PML_fc = 1.0;
+  int DAMP_N = 20;
+  double mu_s = 0.677, mu_d = 0.525, mu_0 = 0.0, Dc = 0.4, C0 = 0.0;
+  double f0 = 0, V0 = 0, Vini = 0, fw = 0, L = 0, smooth_load_T = 0;
+  int Fault_grid[4] = {0, 0, 0, 0};
+  int Asp_grid[4] = {0, 0, 0, 0};
+  double vp1 = 6000, vs1 = 3464, rho1 = 2670;
+  double bi_vp1 = 6000, bi_vs1 = 3464, bi_rho1 = 2670;
+  double bi_vp2 = 6000, bi_vs2 = 3464, bi_rho2 = 2670;
+  double viscosity = 0, RupThres = 0;
+  int TP_n = 1;
+  std::string OUT = "output", Fault_geometry = "Fault_geometry.nc", Fault_init_stress = "Fault_init_stress.nc";
+  std::string Media1D, Media3D;
+  // Makefile feature macros as runtime flags (Makefile:36-121 defaults)
+  int usePML = 1, useCerjan = 0, FreeSurface = 1, NormalRefine = 1, Rupture = 1;
+  // extensions
+  int MetricBF16 = 0;   // store metric + media in bfloat16 (FP32 compute)
+  int NewtonMode = 0;   // 0: reference NRsearch; 1: residual tolerance + plain fallback
+  double NewtonTol = 1e-10;
+  int NewtonFallbackIter = 20;
+  std::string NewtonMLP; // path of trained MLP weights for the Newton guess
+
+  // derived (set_params.cu:286-295)
+  int ni = 0, nj = 0, nk = 0, nx = 0, ny = 0, nz = 0, NT = 0;
+  void derive();
+  int set(const std::string& key, double v);   // numeric keys
+  int get(const std::string& key, double* v) const;
+};
+
+int parse_params_json(const std::string& text, Params& p, std::string& err);
+
+// ---------------------------------------------------------------------------
+// Device-side views.  Volume arrays use the reference index
+// pos = j + k*ny + i*ny*nz over padded dims (3 ghost layers per side).
+struct Dims {
+  int ni, nj, nk, nx, ny, nz, i0;
+  long nxyz, nyz;
+};
+
+// Fault state on the device (2-D arrays [nk][nj] unless noted).
+struct FaultDev {
+  float *W, *mW, *hW, *tW;         // split nodes [9][2][nz][ny]
+  float *T11, *T12, *T13;          // [7][nz][ny]
+  float *mT11, *mT12, *mT13;       // plane 3 used
+  float *hT11, *hT12, *hT13;       // [nz][ny]
+  float *rho_f;                    // [2][nz][ny]
+  float *Dmat;                     // [2][3][3][nz][ny][9]  D_ab for both sides (x J)
+  float *matT1toVx, *matVytoVx, *matVztoVx;   // [2][nz][ny][9]
+  float *matVx2Vz_f, *matVy2Vz_f;              // [2][ny][9]
+  float *matP2Mf, *matM2Pf;                    // [3][ny][9]
+  float *vec_n, *vec_s1, *vec_s2;              // [nz][ny][3]
+  float *jacvec, *jac_i0;                      // [nz][ny]
+  float *Tn, *Ts1, *Ts2, *tTn, *tTs1, *tTs2;
+  float *str_init_x, *str_init_y, *str_init_z;
+  float *T0x, *T0y, *T0z, *dT0x, *dT0y, *dT0z;
+  float *str_peak, *Dc, *C0, *a, *b, *L, *Vw;
+  double *slip, *mslip, *tslip, *State, *mState, *tState;
+  float *hslip, *Vs1, *Vs2, *init_t0, *friction, *rsD_y, *rsD_z, *rup_sensor;
+  int *init_t0_flag, *rup_index_y, *rup_index_z, *united, *faultgrid, *flag_rup;
+  unsigned long long* newton_stats;  // [3]: solves, iterations, fallbacks
+};
+
+struct PmlDev {
+  float *A[3], *B[3], *D[3];  // 1-D profiles along x (ni), y (nj), z (nk); D stored as d/beta
+  float *Y[6], *S[6], *T[6];  // aux per face x1 x2 y1 y2 z1 z2: [9][pni][pnk][pnj]
+  int on[6];
+  int N;
+};
+
+// ---------------------------------------------------------------------------
+struct Ctx {
+  Params p;
+  Dims d;
+  bool setup_done = false;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // host inputs
+  std::vector<float> gx, gy, gz;
+  std::vector<double> coord3d;
+  std::vector<float> m1d_depth, m1d_rho, m1d_vp, m1d_vs;
+  std::map<std::string, std::vector<float>> stress_in;
+  // device volume buffers
+  float *S = nullptr, *A = nullptr, *B = nullptr, *Tacc = nullptr;
+  float *cur = nullptr;         // buffer holding W for the last completed stage
+  float *hW = nullptr;          // RHS buffer for cgfdm_wave_rhs (tests)
+  void* met = nullptr;          // 13 fields, float or bf16
+  float *slabH = nullptr;       // [7][9][nz][ny] fault-slab RHS
+  float *matVx2Vz = nullptr, *matVy2Vz = nullptr;  // [nx][ny][9]
+  float *cj[3] = {nullptr, nullptr, nullptr};      // Cerjan 1-D profiles (nx, ny, nz incl. ghosts)
+  PmlDev pml{};
+  FaultDev f{};
+  std::vector<void*> allocations;
+  // host copies kept from setup
+  std::vector<double> hostC;    // coordinates [3][nxyz] (used for outputs)
+  double hmin = 0, vpmax = 0;
+  // outputs
+  struct NcOut* out = nullptr;
+};
+
+// setup (host, double precision)
+int host_setup(Ctx& c, std::string& err);
+
+// kernel launchers (kernels.hip)
+struct StageBufs {
+  const float* Win; float* Wout; const float* S; float* T;
+};
+void launch_wave_stage(Ctx& c, const StageBufs& b, int irk, int fx, int fy, int fz, bool rhs_only, float* rhs_out);
+void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz);
+void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz);
+void launch_fault_post(Ctx& c, const StageBufs& b, int irk);
+void launch_rup_sensor(Ctx& c);
+void launch_fill(Ctx& c, float* p, long n, float v);
+
+}  // namespace cgfdm

@@ -1,0 +1,587 @@
+// Host-side setup in double precision: grid, metric, media, absorbing-layer
+// profiles, fault coefficients and fault initial state, then upload to HBM.
+// Replaces construct_coord (coord.cu:153-328), extend_crew_array /
+// cal_metric / extend_Symm_array (metric.cu), init_media1d/3d (media.cu),
+// abs_init (init_pml.cu), init_cerjan_host (cerjan.cu:31-164),
+// init_fault_coef_cu (fault_coef.cu:26-1302), init_fault_cu (init.cu:160-849),
+// nc_read_init_stress (init.cu:904-1022) and init_wave_free_cu
+// (fault_coef.cu:1320-1486).  Setup runs once and is not on the timed path.
+#include "cgfdm_internal.h"
+#include "netcdf_classic.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace cgfdm {
+
+namespace {
+
+const double PI_ = 3.141592653589793238463;
+
+void inv3(double m[3][3]) {
+  double v[3][3];
+  v[0][0] = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  v[0][1] = m[2][1] * m[0][2] - m[0][1] * m[2][2];
+  v[0][2] = m[0][1] * m[1][2] - m[0][2] * m[1][1];
+  v[1][0] = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  v[1][1] = m[0][0] * m[2][2] - m[2][0] * m[0][2];
+  v[1][2] = m[1][0] * m[0][2] - m[0][0] * m[1][2];
+  v[2][0] = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  v[2][1] = m[2][0] * m[0][1] - m[0][0] * m[2][1];
+  v[2][2] = m[0][0] * m[1][1] - m[0][1] * m[1][0];
+  double det = v[0][0] * m[0][0] + v[0][1] * m[1][0] + v[0][2] * m[2][0];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) m[a][b] = v[a][b] / det;
+}
+void mul3(const double A[3][3], const double B[3][3], double C[3][3]) {
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) C[a][b] = A[a][0] * B[0][b] + A[a][1] * B[1][b] + A[a][2] * B[2][b];
+}
+inline void cross(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+inline double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline double nrm(const double* a) { return std::sqrt(dot(a, a)); }
+
+// Distance from x0 to the plane through x1 x2 x3 (mathFuncs.cu:73-98).
+double dist_point_plane(const double* x0, const double* x1, const double* x2, const double* x3) {
+  double u[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
+  double v[3] = {x3[0] - x1[0], x3[1] - x1[1], x3[2] - x1[2]};
+  double p[3];
+  cross(u, v, p);
+  return std::fabs(dot(p, x0) - dot(p, x1)) / nrm(p);
+}
+
+uint16_t to_bf16(float f) {  // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  uint32_t r = u + 0x7fff + ((u >> 16) & 1);
+  return (uint16_t)(r >> 16);
+}
+
+template <class T>
+T* dmalloc(Ctx& c, size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+  hipMemset(p, 0, n * sizeof(T));
+  c.allocations.push_back(p);
+  return (T*)p;
+}
+template <class T>
+bool upload(T* dst, const std::vector<T>& src) {
+  return hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
+}
+
+bool read_media1d(const std::string& path, Ctx& c, std::string& err) {
+  std::ifstream in(path);
+  if (!in) { err = "can not open input media dat " + path; return false; }
+  std::string line;
+  std::getline(in, line);  // comment
+  int n = 0;
+  in >> n;
+  std::getline(in, line);
+  std::getline(in, line);  // header
+  for (int l = 0; l < n; l++) {
+    float d, r, a, b;
+    if (!(in >> d >> r >> a >> b)) { err = "bad Media1D file " + path; return false; }
+    c.m1d_depth.push_back(d); c.m1d_rho.push_back(r); c.m1d_vp.push_back(a); c.m1d_vs.push_back(b);
+  }
+  return true;
+}
+
+}  // namespace
+
+int host_setup(Ctx& c, std::string& err) {
+  Params& p = c.p;
+  p.derive();
+  if (p.PX * p.PY * p.PZ != 1) { err = "only one rank (PX=PY=PZ=1) is supported: the whole domain lives on one GPU"; return 1; }
+  Dims& d = c.d;
+  d.ni = p.ni; d.nj = p.nj; d.nk = p.nk; d.nx = p.nx; d.ny = p.ny; d.nz = p.nz;
+  d.nxyz = (long)d.nx * d.ny * d.nz;
+  d.nyz = (long)d.ny * d.nz;
+  d.i0 = d.nx / 2;
+  const int nx = d.nx, ny = d.ny, nz = d.nz, ni = d.ni, nj = d.nj, nk = d.nk;
+  const long nxyz = d.nxyz, nyz = d.nyz;
+  auto P = [&](int i, int j, int k) { return j + (long)k * ny + (long)i * nyz; };
+  const int freenode = p.FreeSurface ? 1 : 0;
+  int on[6] = {1, 1, 1, 1, 1, freenode ? 0 : 1};  // single rank: every face absorbs except a free top
+
+  // ---- inputs from files when not given as arrays
+  if (p.INPORT_GRID_TYPE == 1 && c.gx.empty()) {
+    NcFile nf;
+    if (!nf.open_read(p.Fault_geometry, err)) return 4;
+    if (!nf.read_float("x", c.gx, err) || !nf.read_float("y", c.gy, err) || !nf.read_float("z", c.gz, err)) return 4;
+    if ((long)c.gx.size() < (long)nj * nk) { err = "Fault_geometry smaller than NY*NZ"; return 4; }
+    // hyperslab start {0,0}, count {nk,nj} of arrays shaped (NZ,NY)
+    c.gx.resize((size_t)nj * nk); c.gy.resize((size_t)nj * nk); c.gz.resize((size_t)nj * nk);
+  }
+  if (!p.Media1D.empty() && c.m1d_depth.empty())
+    if (!read_media1d(p.Media1D, c, err)) return 2;
+  if (p.Rupture && p.INPORT_STRESS_TYPE == 1 && c.stress_in.find("Tx") == c.stress_in.end()) {
+    NcFile nf;
+    if (!nf.open_read(p.Fault_init_stress, err)) return 4;
+    std::vector<std::string> names = {"Tx", "Ty", "Tz", "dTx", "dTy", "dTz"};
+    if (p.Friction_type == 0) for (auto s : {"mu_s", "mu_d", "Dc", "C0"}) names.push_back(s);
+    else for (auto s : {"a", "b", "L", "Vw", "State"}) names.push_back(s);
+    for (auto& s : names) {
+      std::vector<float> v;
+      if (!nf.read_float(s, v, err)) return 4;
+      v.resize((size_t)nj * nk);
+      c.stress_in[s] = v;
+    }
+  }
+
+  // ---- coordinates (coord.cu:153-328) and ghost extrapolation (metric.cu:248-313)
+  c.hostC.assign(3 * nxyz, 0.0);
+  double* X = &c.hostC[0];
+  double* Y = &c.hostC[nxyz];
+  double* Z = &c.hostC[2 * nxyz];
+  std::vector<double> xline(p.NX + 6);
+  {
+    int ic = p.NX / 2 + 3;
+    xline[ic] = 0;
+    auto spacing = [&](int i, int ref) {
+      int dist = std::abs(i - ic);
+      double cp = dist < 10 ? 0.0 : (dist < 55 ? 1.0 - std::cos(PI_ * (i - ref) / 45.0) : 2.0);
+      return p.DH * (0.5 + 0.25 * cp);
+    };
+    for (int i = ic + 1; i < p.NX + 6; i++) xline[i] = xline[i - 1] + spacing(i, ic + 10);
+    for (int i = ic - 1; i >= 0; i--) xline[i] = xline[i + 1] - spacing(i, ic - 10);
+  }
+#pragma omp parallel for
+  for (int i = 3; i < nx - 3; i++)
+    for (int k = 3; k < nz - 3; k++)
+      for (int j = 3; j < ny - 3; j++) {
+        long q = P(i, j, k);
+        if (!c.coord3d.empty()) { X[q] = c.coord3d[q]; Y[q] = c.coord3d[nxyz + q]; Z[q] = c.coord3d[2 * nxyz + q]; continue; }
+        int gi = i - 3, gj = j - 3, gk = k - 3;
+        double x0 = (-p.NX / 2) * p.DH, y0 = (-p.NY / 2) * p.DH, z0 = (1 - p.NZ) * p.DH;
+        double x = gi * p.DH + x0, y = gj * p.DH + y0, z = gk * p.DH + z0;
+        if (p.NormalRefine) x = xline[i];
+        if (p.INPORT_GRID_TYPE == 1) {
+          long f = (j - 3) + (long)(k - 3) * nj;
+          x = (p.NormalRefine ? xline[i] : gi * p.DH + x0) + c.gx[f];
+          y = c.gy[f];
+          z = c.gz[f];
+        }
+        X[q] = x; Y[q] = y; Z[q] = z;
+      }
+  for (int l = 0; l < 3; l++) {
+    double* A = &c.hostC[l * nxyz];
+    for (int n = 1; n <= 3; n++)
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+          A[P(3 - n, j, k)] = 2.0 * A[P(3, j, k)] - A[P(3 + n, j, k)];
+          A[P(nx - 4 + n, j, k)] = 2.0 * A[P(nx - 4, j, k)] - A[P(nx - 4 - n, j, k)];
+        }
+    for (int i = 0; i < nx; i++)
+      for (int k = 0; k < nz; k++)
+        for (int n = 1; n <= 3; n++) {
+          A[P(i, 3 - n, k)] = 2.0 * A[P(i, 3, k)] - A[P(i, 3 + n, k)];
+          A[P(i, ny - 4 + n, k)] = 2.0 * A[P(i, ny - 4, k)] - A[P(i, ny - 4 - n, k)];
+        }
+    for (int i = 0; i < nx; i++)
+      for (int n = 1; n <= 3; n++)
+        for (int j = 0; j < ny; j++) {
+          A[P(i, j, 3 - n)] = 2.0 * A[P(i, j, 3)] - A[P(i, j, 3 + n)];
+          A[P(i, j, nz - 4 + n)] = 2.0 * A[P(i, j, nz - 4)] - A[P(i, j, nz - 4 - n)];
+        }
+  }
+
+  // ---- grid-step range for the CFL check (cal_range_steph, coord.cu:330-391)
+  double hmin = 1e38;
+#pragma omp parallel for reduction(min : hmin)
+  for (int i = 3; i < nx - 3; i++)
+    for (int j = 3; j < ny - 3; j++)
+      for (int k = 3; k < nz - 3; k++) {
+        long q = P(i, j, k);
+        double x0[3] = {X[q], Y[q], Z[q]};
+        for (int ii = -1; ii <= 1; ii += 2)
+          for (int jj = -1; jj <= 1; jj += 2)
+            for (int kk = -1; kk <= 1; kk += 2) {
+              long q1 = P(i + ii, j, k), q2 = P(i, j + jj, k), q3 = P(i, j, k + kk);
+              double x1[3] = {X[q1], Y[q1], Z[q1]}, x2[3] = {X[q2], Y[q2], Z[q2]}, x3[3] = {X[q3], Y[q3], Z[q3]};
+              hmin = std::min(hmin, dist_point_plane(x0, x1, x2, x3));
+            }
+      }
+  c.hmin = hmin;
+
+  // ---- metric + media (metric.cu:15-171, media.cu:63-204), kept in double here
+  std::vector<double> Md(13 * nxyz, 0.0);
+  {
+    double rDH = 1.0 / p.DH;
+    double lam1 = p.bi_rho1 * (p.bi_vp1 * p.bi_vp1 - 2 * p.bi_vs1 * p.bi_vs1), miu1 = p.bi_rho1 * p.bi_vs1 * p.bi_vs1;
+    double lam2 = p.bi_rho2 * (p.bi_vp2 * p.bi_vp2 - 2 * p.bi_vs2 * p.bi_vs2), miu2 = p.bi_rho2 * p.bi_vs2 * p.bi_vs2;
+    int nl = (int)c.m1d_depth.size();
+#pragma omp parallel for
+    for (int i = 3; i < nx - 3; i++)
+      for (int k = 3; k < nz - 3; k++)
+        for (int j = 3; j < ny - 3; j++) {
+          long q = P(i, j, k);
+          const long s[3] = {nyz, 1, (long)ny};
+          double t[3][3];  // t[axis][component] = d(x,y,z)/d(xi,eta,zeta)
+          for (int a = 0; a < 3; a++) {
+            t[a][0] = 0.5 * rDH * (X[q + s[a]] - X[q - s[a]]);
+            t[a][1] = 0.5 * rDH * (Y[q + s[a]] - Y[q - s[a]]);
+            t[a][2] = 0.5 * rDH * (Z[q + s[a]] - Z[q - s[a]]);
+          }
+          double g[3];
+          cross(t[0], t[1], g);
+          double jac = dot(g, t[2]);
+          Md[9 * nxyz + q] = jac;
+          cross(t[1], t[2], g);
+          for (int a = 0; a < 3; a++) Md[a * nxyz + q] = g[a] / jac;
+          cross(t[2], t[0], g);
+          for (int a = 0; a < 3; a++) Md[(3 + a) * nxyz + q] = g[a] / jac;
+          cross(t[0], t[1], g);
+          for (int a = 0; a < 3; a++) Md[(6 + a) * nxyz + q] = g[a] / jac;
+          double lam = i < p.NX / 2 ? lam1 : lam2, miu = i < p.NX / 2 ? miu1 : miu2, rho = i < p.NX / 2 ? p.bi_rho1 : p.bi_rho2;
+          if (nl > 0) {
+            double depth1 = -Z[q];
+            double r = c.m1d_rho[nl - 1], va = c.m1d_vp[nl - 1], vb = c.m1d_vs[nl - 1];
+            for (int l = nl - 1; l >= 0; l--)
+              if (depth1 > c.m1d_depth[l]) { r = c.m1d_rho[l]; va = c.m1d_vp[l]; vb = c.m1d_vs[l]; break; }
+            lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
+          }
+          Md[10 * nxyz + q] = lam; Md[11 * nxyz + q] = miu; Md[12 * nxyz + q] = rho;
+        }
+    // mirror into ghosts (extend_Symm_array, metric.cu:173-246)
+#pragma omp parallel for
+    for (int l = 0; l < 13; l++) {
+      double* A = &Md[l * nxyz];
+      for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+          for (int n = 1; n <= 3; n++) { A[P(3 - n, j, k)] = A[P(3 + n, j, k)]; A[P(nx - 4 + n, j, k)] = A[P(nx - 4 - n, j, k)]; }
+      for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nx; i++)
+          for (int n = 1; n <= 3; n++) { A[P(i, 3 - n, k)] = A[P(i, 3 + n, k)]; A[P(i, ny - 4 + n, k)] = A[P(i, ny - 4 - n, k)]; }
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++)
+          for (int n = 1; n <= 3; n++) { A[P(i, j, 3 - n)] = A[P(i, j, 3 + n)]; A[P(i, j, nz - 4 + n)] = A[P(i, j, nz - 4 - n)]; }
+    }
+  }
+  // CFL (main.cu:302-345)
+  double vpmax = 0;
+#pragma omp parallel for reduction(max : vpmax)
+  for (int i = 3; i < nx - 3; i++)
+    for (int k = 3; k < nz - 3; k++)
+      for (int j = 3; j < ny - 3; j++) {
+        long q = P(i, j, k);
+        vpmax = std::max(vpmax, std::sqrt((Md[10 * nxyz + q] + 2 * Md[11 * nxyz + q]) / Md[12 * nxyz + q]));
+      }
+  c.vpmax = vpmax;
+  float dtmax = (float)(1.3 * hmin / vpmax);
+  if (!((float)p.DT < dtmax)) {
+    char b[256];
+    std::snprintf(b, sizeof b, "Serious Error: DT = %10.2e > dtmax = %10.2e (sec) do not satisfy stability condition", p.DT, (double)dtmax);
+    err = b;
+    return 5;
+  }
+
+  // ---- PML profiles (init_pml.cu:18-155)
+  std::vector<double> Ap[3], Bp[3], Dp[3];
+  {
+    int n3[3] = {ni, nj, nk};
+    int N = p.PML_N;
+    double Rpp = std::pow(10.0, -((std::log10((double)N) - 1.0) / std::log10(2.0) + 3.0));
+    double amax = PI_ * p.PML_fc, L0 = p.DH * (N - 1);
+    double dmax = -p.PML_velocity / (2.0 * L0) * std::log(Rpp) * 3.0;
+    for (int a = 0; a < 3; a++) {
+      int n = n3[a];
+      Ap[a].assign(n, 0.0); Bp[a].assign(n, 1.0); Dp[a].assign(n, 0.0);
+      if (!p.usePML) continue;
+      std::vector<double> Lx(n, -1.0);
+      if (on[2 * a]) for (int i = 0; i < N && i < n; i++) Lx[i] = (N - 1 - i) * p.DH;
+      if (on[2 * a + 1]) for (int i = 0; i < N && n - N + i >= 0; i++) Lx[n - N + i] = i * p.DH;
+      for (int i = 0; i < n; i++) {
+        double x = Lx[i];
+        if (x < 0) continue;
+        double r = x / L0;
+        Ap[a][i] = amax * (1.0 - r);
+        Bp[a][i] = 1.0 + (p.PML_bmax - 1.0) * r * r;
+        Dp[a][i] = dmax * r * r / Bp[a][i];
+      }
+    }
+  }
+
+  // ---- free-surface velocity maps (init_wave_free_cu, fault_coef.cu:1320-1486)
+  std::vector<float> mVx(nx * (long)ny * 9, 0.f), mVy(nx * (long)ny * 9, 0.f);
+  if (freenode) {
+    int k = nz - 4;
+#pragma omp parallel for
+    for (int i = 3; i < nx - 3; i++)
+      for (int j = 3; j < ny - 3; j++) {
+        double rb[3] = {1.0 / Bp[0][i - 3], 1.0 / Bp[1][j - 3], 1.0 / Bp[2][k - 3]};
+        long q = P(i, j, k);
+        double e[3][3];
+        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = Md[a * nxyz + q];
+        double lam = Md[10 * nxyz + q], mu = Md[11 * nxyz + q];
+        // traction on the zeta surface produced by derivatives along axis b:
+        // K_b[r][c] = lam e3[r] eb[c] + mu e3[c] eb[r] + mu (e3.eb) delta_rc
+        double K[3][3][3];
+        for (int b = 0; b < 3; b++) {
+          double eb3 = dot(e[2], e[b]);
+          for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++)
+              K[b][r][cc] = rb[b] * (lam * e[2][r] * e[b][cc] + mu * e[2][cc] * e[b][r] + (r == cc ? mu * eb3 : 0.0));
+        }
+        double Ainv[3][3], AB[3][3], AC[3][3], Bm[3][3], Cm[3][3];
+        for (int r = 0; r < 3; r++)
+          for (int cc = 0; cc < 3; cc++) { Ainv[r][cc] = K[2][r][cc]; Bm[r][cc] = -K[0][r][cc]; Cm[r][cc] = -K[1][r][cc]; }
+        inv3(Ainv);
+        mul3(Ainv, Bm, AB);
+        mul3(Ainv, Cm, AC);
+        for (int a = 0; a < 9; a++) {
+          mVx[(j + (long)i * ny) * 9 + a] = (float)AB[a / 3][a % 3];
+          mVy[(j + (long)i * ny) * 9 + a] = (float)AC[a / 3][a % 3];
+        }
+      }
+  }
+
+  // ---- device allocations: 4 wavefield buffers (step start S, stage A/B, accumulator T)
+  c.S = dmalloc<float>(c, 9 * nxyz);
+  c.A = dmalloc<float>(c, 9 * nxyz);
+  c.B = dmalloc<float>(c, 9 * nxyz);
+  c.Tacc = dmalloc<float>(c, 9 * nxyz);
+  if (!c.S || !c.A || !c.B || !c.Tacc) { err = "hipMalloc failed for the wavefield"; return 6; }
+  c.cur = c.S;
+  if (p.MetricBF16) {
+    std::vector<uint16_t> h(13 * nxyz);
+#pragma omp parallel for
+    for (long q = 0; q < 13 * nxyz; q++) h[q] = to_bf16((float)Md[q]);
+    uint16_t* dm = dmalloc<uint16_t>(c, 13 * nxyz);
+    if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
+    c.met = dm;
+  } else {
+    std::vector<float> h(13 * nxyz);
+#pragma omp parallel for
+    for (long q = 0; q < 13 * nxyz; q++) h[q] = (float)Md[q];
+    float* dm = dmalloc<float>(c, 13 * nxyz);
+    if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
+    c.met = dm;
+  }
+  c.matVx2Vz = dmalloc<float>(c, mVx.size());
+  c.matVy2Vz = dmalloc<float>(c, mVy.size());
+  upload(c.matVx2Vz, mVx);
+  upload(c.matVy2Vz, mVy);
+
+  // PML
+  c.pml.N = p.PML_N;
+  for (int a = 0; a < 3; a++) {
+    std::vector<float> fa(Ap[a].begin(), Ap[a].end()), fb(Bp[a].begin(), Bp[a].end()), fd(Dp[a].begin(), Dp[a].end());
+    c.pml.A[a] = dmalloc<float>(c, fa.size()); upload(c.pml.A[a], fa);
+    c.pml.B[a] = dmalloc<float>(c, fb.size()); upload(c.pml.B[a], fb);
+    c.pml.D[a] = dmalloc<float>(c, fd.size()); upload(c.pml.D[a], fd);
+  }
+  for (int f = 0; f < 6; f++) {
+    c.pml.on[f] = p.usePML ? on[f] : 0;
+    if (!c.pml.on[f]) continue;
+    int a = f / 2;
+    long n = 9L * p.PML_N * (a == 0 ? (long)nj * nk : a == 1 ? (long)nk * ni : (long)ni * nj);
+    c.pml.Y[f] = dmalloc<float>(c, n);
+    c.pml.S[f] = dmalloc<float>(c, n);
+    c.pml.T[f] = dmalloc<float>(c, n);
+    if (!c.pml.Y[f] || !c.pml.S[f] || !c.pml.T[f]) { err = "hipMalloc failed for PML"; return 6; }
+  }
+
+  // Cerjan: damp(i,j,k) = min of three 1-D profiles over the interior
+  // (init_cerjan_host builds the same cube face by face with min).
+  {
+    int n3[3] = {nx, ny, nz};
+    int ND = p.DAMP_N;
+    float alpha = std::sqrt(-std::log(0.92f)) / ND;
+    for (int a = 0; a < 3; a++) {
+      std::vector<float> prof(n3[a], 1.0f);
+      if (p.useCerjan) {
+        for (int i = 0; i < ND && 3 + i < n3[a] - 3; i++) prof[3 + i] = std::exp(-std::pow(alpha * (ND - i), 2.0f));
+        bool hi = a < 2 || !freenode;
+        if (hi) for (int i = 0; i < ND && n3[a] - 4 - i >= 3; i++) prof[n3[a] - 4 - i] = std::exp(-std::pow(alpha * (ND - i), 2.0f));
+      }
+      c.cj[a] = dmalloc<float>(c, prof.size());
+      upload(c.cj[a], prof);
+    }
+  }
+
+  // ---- fault
+  if (p.Rupture) {
+    FaultDev& F = c.f;
+    const int i0 = d.i0;
+    long n2 = (long)nj * nk;
+    std::vector<float> rho_f(2 * nyz), Dm(2L * 9 * nyz * 9), T1x(2 * nyz * 9), Vyx(2 * nyz * 9), Vzx(2 * nyz * 9);
+    std::vector<float> Vx2Vzf(2L * ny * 9, 0.f), Vy2Vzf(2L * ny * 9, 0.f), P2M(3L * ny * 9, 0.f), M2P(3L * ny * 9, 0.f);
+    std::vector<float> vn(nyz * 3), vs1(nyz * 3), vs2(nyz * 3), jacvec(nyz), jac0(nyz);
+    std::vector<double> vnd(nyz * 3), vs1d(nyz * 3), vs2d(nyz * 3);
+#pragma omp parallel for
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++) {
+        long q = P(i0, j, k), pf = j + (long)k * ny;
+        double e[3][3];
+        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = Md[a * nxyz + q];
+        double jac = Md[9 * nxyz + q];
+        double mt[3][3];
+        std::memcpy(mt, e, sizeof mt);
+        inv3(mt);
+        double x_et[3] = {mt[0][1], mt[1][1], mt[2][1]};
+        double s1[3] = {x_et[0], x_et[1], x_et[2]}, n[3] = {e[0][0], e[0][1], e[0][2]}, s2[3];
+        double ns1 = nrm(s1), nn = nrm(n);
+        for (int a = 0; a < 3; a++) { s1[a] /= ns1; n[a] /= nn; }
+        cross(n, s1, s2);
+        for (int a = 0; a < 3; a++) {
+          vn[pf * 3 + a] = (float)n[a]; vs1[pf * 3 + a] = (float)s1[a]; vs2[pf * 3 + a] = (float)s2[a];
+          vnd[pf * 3 + a] = n[a]; vs1d[pf * 3 + a] = s1[a]; vs2d[pf * 3 + a] = s2[a];
+        }
+        jacvec[pf] = (float)(jac * nn);
+        jac0[pf] = (float)jac;
+        double lamf[2] = {Md[10 * nxyz + q - nyz], Md[10 * nxyz + q + nyz]};
+        double muf[2] = {Md[11 * nxyz + q - nyz], Md[11 * nxyz + q + nyz]};
+        rho_f[pf] = (float)Md[12 * nxyz + q - nyz];
+        rho_f[nyz + pf] = (float)Md[12 * nxyz + q + nyz];
+        double Dl[2][3][3][3][3];
+        for (int m = 0; m < 2; m++) {
+          for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+              double eab = dot(e[a], e[b]);
+              for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++) {
+                  double v = jac * (lamf[m] * e[a][r] * e[b][cc] + muf[m] * e[a][cc] * e[b][r] + (r == cc ? muf[m] * eab : 0.0));
+                  Dl[m][a][b][r][cc] = v;
+                  Dm[(((long)(m * 9 + a * 3 + b)) * nyz + pf) * 9 + r * 3 + cc] = (float)v;
+                }
+            }
+          double D11[3][3], D12[3][3], D13[3][3], o[3][3];
+          std::memcpy(D11, Dl[m][0][0], sizeof D11);
+          std::memcpy(D12, Dl[m][0][1], sizeof D12);
+          std::memcpy(D13, Dl[m][0][2], sizeof D13);
+          inv3(D11);
+          for (int a = 0; a < 9; a++) T1x[(m * nyz + pf) * 9 + a] = (float)D11[a / 3][a % 3];
+          mul3(D11, D12, o);
+          for (int a = 0; a < 9; a++) Vyx[(m * nyz + pf) * 9 + a] = (float)o[a / 3][a % 3];
+          mul3(D11, D13, o);
+          for (int a = 0; a < 9; a++) Vzx[(m * nyz + pf) * 9 + a] = (float)o[a / 3][a % 3];
+        }
+        if (k == nz - 4) {
+          double V2Z[2][3][3], VY2Z[2][3][3];
+          for (int m = 0; m < 2; m++) {
+            double A[3][3], B[3][3], Cc[3][3];
+            for (int r = 0; r < 3; r++)
+              for (int cc = 0; cc < 3; cc++) { A[r][cc] = Dl[m][2][2][r][cc]; B[r][cc] = -Dl[m][2][0][r][cc]; Cc[r][cc] = -Dl[m][2][1][r][cc]; }
+            inv3(A);
+            mul3(A, B, V2Z[m]);
+            mul3(A, Cc, VY2Z[m]);
+            for (int a = 0; a < 9; a++) { Vx2Vzf[(m * ny + j) * 9 + a] = (float)V2Z[m][a / 3][a % 3]; Vy2Vzf[(m * ny + j) * 9 + a] = (float)VY2Z[m][a / 3][a % 3]; }
+          }
+          double v1[3] = {e[0][0], e[0][1], e[0][2]}, v3[3];
+          double v1n = nrm(v1);
+          cross(v1, x_et, v3);
+          double h[3] = {dot(v3, e[0]) / v1n, dot(v3, e[1]) / v1n, dot(v3, e[2]) / v1n};
+          double miu = Md[11 * nxyz + q];
+          double Vxf[2][3][3], Vyf[2][3][3];
+          for (int m = 0; m < 2; m++) {
+            double Df[3][3][3];
+            for (int b = 0; b < 3; b++)
+              for (int r = 0; r < 3; r++)
+                for (int cc = 0; cc < 3; cc++) Df[b][r][cc] = Dl[m][0][b][r][cc] + miu * h[b] * v3[r] * v1[cc];
+            double m1[3][3], m2[3][3];
+            mul3(Df[2], V2Z[m], m1);
+            mul3(Df[2], VY2Z[m], m2);
+            for (int r = 0; r < 3; r++)
+              for (int cc = 0; cc < 3; cc++) { Vxf[m][r][cc] = Df[0][r][cc] + m1[r][cc]; Vyf[m][r][cc] = Df[1][r][cc] + m2[r][cc]; }
+          }
+          double Ti[3][3], o[3][3];
+          std::memcpy(Ti, Vxf[0], sizeof Ti);
+          inv3(Ti);
+          mul3(Ti, Vxf[1], o); for (int a = 0; a < 9; a++) P2M[(0 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
+          mul3(Ti, Vyf[1], o); for (int a = 0; a < 9; a++) P2M[(1 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
+          mul3(Ti, Vyf[0], o); for (int a = 0; a < 9; a++) P2M[(2 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
+          std::memcpy(Ti, Vxf[1], sizeof Ti);
+          inv3(Ti);
+          mul3(Ti, Vxf[0], o); for (int a = 0; a < 9; a++) M2P[(0 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
+          mul3(Ti, Vyf[0], o); for (int a = 0; a < 9; a++) M2P[(1 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
+          mul3(Ti, Vyf[1], o); for (int a = 0; a < 9; a++) M2P[(2 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
+        }
+      }
+    // initial fault state (init_fault_cu, init.cu:160-849)
+    std::vector<float> Tn(n2), Ts1(n2), Ts2(n2), six(n2), siy(n2), siz(n2), peak(n2), initt0(n2, -9999.9f);
+    std::vector<int> riy(n2), riz(n2), uni(n2), fg(n2);
+    for (int k = 0; k < nk; k++)
+      for (int j = 0; j < nj; j++) {
+        long q = j + (long)k * nj, pv = (j + 3 + (long)(k + 3) * ny) * 3;
+        int gj1 = j + 1, gk1 = k + 1;
+        double tn = -120.0e6, ts1 = tn * p.mu_0, ts2 = 0.0;
+        if (gj1 >= p.Asp_grid[0] && gj1 <= p.Asp_grid[1] && gk1 >= p.Asp_grid[2] && gk1 <= p.Asp_grid[3])
+          ts1 = (p.mu_s + (p.mu_s - p.mu_d) * 0.001) * tn;
+        bool inF = gj1 >= p.Fault_grid[0] && gj1 <= p.Fault_grid[1] && gk1 >= p.Fault_grid[2] && gk1 <= p.Fault_grid[3];
+        peak[q] = inF ? (float)p.mu_s : 10000.f;
+        double si[3];
+        for (int a = 0; a < 3; a++) si[a] = tn * vnd[pv + a] + ts1 * vs1d[pv + a] + ts2 * vs2d[pv + a];
+        six[q] = (float)si[0]; siy[q] = (float)si[1]; siz[q] = (float)si[2];
+        Tn[q] = (float)tn; Ts1[q] = (float)ts1; Ts2[q] = (float)ts2;
+        riy[q] = (gj1 <= p.Fault_grid[0] + 3 || gj1 >= p.Fault_grid[1] - 3) ? 0 : 1;
+        riz[q] = (gk1 <= p.Fault_grid[2] + 3 || gk1 >= p.Fault_grid[3] - 3) ? 0 : 1;
+        uni[q] = (gj1 > 30 && gj1 < p.NY - 30 && gk1 > 30) ? 0 : 1;  // init.cu:806-816 overrides the PML rule
+        fg[q] = (gj1 >= p.Fault_grid[0] + 3 && gj1 <= p.Fault_grid[1] - 3 && gk1 >= p.Fault_grid[2] + 3 && gk1 <= p.Fault_grid[3] - 3) ? 1 : 0;
+        // rate-state: a-b is tested before a and b are read (init.cu:835-847
+        // precedes nc_read_init_stress), so it sees zeros.
+        if (p.Friction_type >= 1) fg[q] = 0;
+      }
+    auto get_in = [&](const char* nm) -> std::vector<float> {
+      auto it = c.stress_in.find(nm);
+      if (it == c.stress_in.end()) return std::vector<float>(n2, 0.f);
+      std::vector<float> v = it->second;
+      v.resize(n2);
+      return v;
+    };
+    std::vector<float> T0x = get_in("Tx"), T0y = get_in("Ty"), T0z = get_in("Tz");
+    std::vector<float> dT0x = get_in("dTx"), dT0y = get_in("dTy"), dT0z = get_in("dTz");
+    std::vector<float> Dcv = get_in("Dc"), C0v = get_in("C0"), av = get_in("a"), bv = get_in("b"), Lv = get_in("L"), Vwv = get_in("Vw");
+    std::vector<double> State(n2, 0.0);
+    if (p.INPORT_STRESS_TYPE == 1) {
+      if (p.Friction_type == 0) peak = get_in("mu_s");
+      else { std::vector<float> s = get_in("State"); for (long q = 0; q < n2; q++) State[q] = s[q]; }
+      for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++) {
+          long q = j + (long)k * nj, pv = (j + 3 + (long)(k + 3) * ny) * 3;
+          double T0[3] = {T0x[q], T0y[q], T0z[q]};
+          Tn[q] = (float)dot(T0, &vnd[pv]); Ts1[q] = (float)dot(T0, &vs1d[pv]); Ts2[q] = (float)dot(T0, &vs2d[pv]);
+        }
+    }
+#define UPF(dst, vec) do { F.dst = dmalloc<float>(c, (vec).size()); upload(F.dst, vec); } while (0)
+#define UPI(dst, vec) do { F.dst = dmalloc<int>(c, (vec).size()); upload(F.dst, vec); } while (0)
+    F.W = dmalloc<float>(c, 18 * nyz); F.mW = dmalloc<float>(c, 18 * nyz);
+    F.hW = dmalloc<float>(c, 18 * nyz); F.tW = dmalloc<float>(c, 18 * nyz);
+    F.T11 = dmalloc<float>(c, 7 * nyz); F.T12 = dmalloc<float>(c, 7 * nyz); F.T13 = dmalloc<float>(c, 7 * nyz);
+    F.mT11 = dmalloc<float>(c, nyz); F.mT12 = dmalloc<float>(c, nyz); F.mT13 = dmalloc<float>(c, nyz);
+    F.hT11 = dmalloc<float>(c, nyz); F.hT12 = dmalloc<float>(c, nyz); F.hT13 = dmalloc<float>(c, nyz);
+    UPF(rho_f, rho_f); UPF(Dmat, Dm); UPF(matT1toVx, T1x); UPF(matVytoVx, Vyx); UPF(matVztoVx, Vzx);
+    UPF(matVx2Vz_f, Vx2Vzf); UPF(matVy2Vz_f, Vy2Vzf); UPF(matP2Mf, P2M); UPF(matM2Pf, M2P);
+    UPF(vec_n, vn); UPF(vec_s1, vs1); UPF(vec_s2, vs2); UPF(jacvec, jacvec); UPF(jac_i0, jac0);
+    UPF(Tn, Tn); UPF(Ts1, Ts1); UPF(Ts2, Ts2);
+    F.tTn = dmalloc<float>(c, n2); F.tTs1 = dmalloc<float>(c, n2); F.tTs2 = dmalloc<float>(c, n2);
+    UPF(str_init_x, six); UPF(str_init_y, siy); UPF(str_init_z, siz);
+    UPF(T0x, T0x); UPF(T0y, T0y); UPF(T0z, T0z); UPF(dT0x, dT0x); UPF(dT0y, dT0y); UPF(dT0z, dT0z);
+    UPF(str_peak, peak); UPF(Dc, Dcv); UPF(C0, C0v); UPF(a, av); UPF(b, bv); UPF(L, Lv); UPF(Vw, Vwv);
+    F.slip = dmalloc<double>(c, n2); F.mslip = dmalloc<double>(c, n2); F.tslip = dmalloc<double>(c, n2);
+    F.State = dmalloc<double>(c, n2); upload(F.State, State);
+    F.mState = dmalloc<double>(c, n2); F.tState = dmalloc<double>(c, n2);
+    F.hslip = dmalloc<float>(c, n2); F.Vs1 = dmalloc<float>(c, n2); F.Vs2 = dmalloc<float>(c, n2);
+    UPF(init_t0, initt0);
+    F.friction = dmalloc<float>(c, n2); F.rsD_y = dmalloc<float>(c, n2); F.rsD_z = dmalloc<float>(c, n2); F.rup_sensor = dmalloc<float>(c, n2);
+    F.init_t0_flag = dmalloc<int>(c, n2); F.flag_rup = dmalloc<int>(c, n2);
+    UPI(rup_index_y, riy); UPI(rup_index_z, riz); UPI(united, uni); UPI(faultgrid, fg);
+    F.newton_stats = dmalloc<unsigned long long>(c, 4);
+#undef UPF
+#undef UPI
+    c.slabH = dmalloc<float>(c, 7 * 9 * nyz);
+    if (!c.slabH || !F.newton_stats) { err = "hipMalloc failed for the fault"; return 6; }
+  }
+  c.hW = nullptr;
+  if (hipDeviceSynchronize() != hipSuccess) { err = "HIP error during setup upload"; return 6; }
+  return 0;
+}
+
+}  // namespace cgfdm

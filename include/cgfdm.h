@@ -1,0 +1,123 @@
+/*
+ * cgfdm.h -- C ABI of the MI355X-native CGFDM dynamic-rupture solver.
+ *
+ * The reference (wenqiang-geophys/dynfd3dcuda) has no library API: its
+ * time-step path is a set of void CUDA launchers taking structs of device
+ * pointers (srcd/main.cu:49-84), driven by main() (srcd/main.cu:123-580) and
+ * configured by params.json (srcd/set_params.cu:56-298).  Each entry point
+ * below replaces one of those pieces; the comment names the reference
+ * interface it stands in for.  All functions return 0 on success or a
+ * positive CGFDM_E* code; the message of the last error is available from
+ * cgfdm_last_error().  Pointers are host pointers unless stated otherwise.
+ * One context owns one simulation on one GPU; calls on a context are not
+ * thread safe.
+ */
+#ifndef CGFDM_H
+#define CGFDM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cgfdm_ctx cgfdm_ctx;
+
+enum {
+  CGFDM_OK = 0,
+  CGFDM_EPARAM = 1,    /* bad argument / unknown name                        */
+  CGFDM_EFILE = 2,     /* params file missing or unreadable (ref: exit(-1))  */
+  CGFDM_EJSON = 3,     /* malformed JSON (ref: exit(-1))                     */
+  CGFDM_ENETCDF = 4,   /* netCDF input missing or malformed (ref: exit(2))   */
+  CGFDM_ECFL = 5,      /* DT >= 1.3 hmin / vpmax (ref: MPI_Abort 110)        */
+  CGFDM_EHIP = 6,      /* HIP runtime / allocation failure                   */
+  CGFDM_ESTATE = 7,    /* call out of order (e.g. step before setup)         */
+  CGFDM_ESIZE = 8      /* buffer size does not match the field               */
+};
+
+/* Message of the last error on this thread (never NULL). */
+const char* cgfdm_last_error(void);
+
+/* Parse a params.json file: defaults then JSON keys, exactly the key set and
+ * defaults of get_params (srcd/set_params.cu:56-298), plus optional runtime
+ * forms of the Makefile feature macros ("usePML", "useCerjan", "FreeSurface",
+ * "NormalRefine", "Rupture") with the Makefile defaults (Makefile:36-121).
+ * Input paths in the JSON are resolved relative to the current directory, as
+ * the reference does. */
+int cgfdm_create(const char* params_json_path, cgfdm_ctx** out);
+/* Same from a JSON text already in memory. */
+int cgfdm_create_from_string(const char* json_text, cgfdm_ctx** out);
+/* Override one parameter by its JSON key (numbers only; "Fault_grid0".."3"
+ * and "Asp_grid0".."3" address array elements).  Must precede setup. */
+int cgfdm_set_param(cgfdm_ctx* ctx, const char* key, double value);
+int cgfdm_get_param(cgfdm_ctx* ctx, const char* key, double* value);
+int cgfdm_destroy(cgfdm_ctx* ctx);
+
+/* Host arrays that replace the netCDF/text inputs (tests and embedding).
+ * Fault_geometry x,y,z [nk][nj] (srcd/coord.cu:71-151). */
+int cgfdm_set_fault_geometry(cgfdm_ctx* ctx, const float* x, const float* y, const float* z, size_t n);
+/* Full 3-D coordinates [3][nx][nz][ny] with ghosts (interior used; extension
+ * for topography that varies along the fault normal). */
+int cgfdm_set_coord3d(cgfdm_ctx* ctx, const double* xyz, size_t n);
+/* Media1D layers (srcd/media.cu:63-138). */
+int cgfdm_set_media1d(cgfdm_ctx* ctx, int nlayer, const float* depth, const float* rho, const float* vp, const float* vs);
+/* Fault_init_stress variable by its netCDF name: Tx Ty Tz dTx dTy dTz mu_s
+ * mu_d Dc C0 a b L Vw State, [nk][nj] (srcd/init.cu:904-1022). */
+int cgfdm_set_init_stress(cgfdm_ctx* ctx, const char* name, const float* data, size_t n);
+
+/* Build grid, metric, media, PML profiles, Cerjan profiles, fault
+ * coefficients, fault initial state and free-surface maps; allocate device
+ * memory and upload.  Replaces main.cu:238-396 (alloc_*, construct_coord,
+ * cal_metric, init_media*, abs_init, init_cerjan_host, init_fault_coef,
+ * init_fault, nc_read_init_stress, init_wave_free, init_wave_3d).  Reads the
+ * netCDF/text inputs named in the JSON unless the arrays were set above.
+ * Fails with CGFDM_ECFL under the reference's CFL rule (main.cu:323-345). */
+int cgfdm_setup(cgfdm_ctx* ctx);
+
+/* One time step it: cal_rup_sensor + 4 RK stages (main.cu:439-516). */
+int cgfdm_step(cgfdm_ctx* ctx, int it);
+/* One RK stage irk (0..3) of step it with the FD flag triple of that stage
+ * (main.cu:448-516): state_rk, trial_sw|trial_rs, fault2wave, wave_deriv,
+ * abs_deriv_{x,y,z}, fault_dvelo, fault_dstrs_{f,b}, wave_rk, fault_rk,
+ * fault2wave, apply_cerjan. */
+int cgfdm_stage(cgfdm_ctx* ctx, int it, int irk);
+/* Steps it0 .. it0+nt-1; writes output records when write_outputs != 0
+ * (main.cu:420-535). */
+int cgfdm_run(cgfdm_ctx* ctx, int it0, int nt, int write_outputs);
+/* Wait for all work queued on the context's stream. */
+int cgfdm_synchronize(cgfdm_ctx* ctx);
+
+/* Per-kernel entry points on the current stage buffers, for parity tests.
+ * wave_deriv + PML + RK combine + Cerjan for the whole volume with the fault
+ * slab diverted (wave.cu:8-494, pml{x,y,z}.cu, rk4.cu:154-257,
+ * cerjan.cu:166-200). */
+int cgfdm_wave_stage(cgfdm_ctx* ctx, int irk, int fx, int fy, int fz);
+/* Volume RHS only (no RK): fills the "hW" field (wave_deriv_cu, wave.cu:8-494,
+ * plus abs_deriv_{x,y,z} contributions when PML is on). */
+int cgfdm_wave_rhs(cgfdm_ctx* ctx, int fx, int fy, int fz);
+
+/* Host copies of named fields (see DESIGN.md for the list): "W" (current
+ * wavefield, 9 x nx*ny*nz, float), "hW", "M" (13 fields), "FW" (split nodes),
+ * fault 2-D outputs "Vs1" "Vs2" "Tn" "Ts1" "Ts2" "slip" "State" "init_t0"
+ * "rup_sensor" ..., PML aux "PMLx1".."PMLz2".  dtype: 0 float32, 1 float64,
+ * 2 int32. */
+int cgfdm_field_info(cgfdm_ctx* ctx, const char* name, size_t* count, int* dtype);
+int cgfdm_get_field(cgfdm_ctx* ctx, const char* name, void* dst, size_t bytes);
+int cgfdm_put_field(cgfdm_ctx* ctx, const char* name, const void* src, size_t bytes);
+/* Dimensions: ni nj nk nx ny nz i0 NT. */
+int cgfdm_get_dims(cgfdm_ctx* ctx, int dims[8]);
+
+/* Output files in OUT/ (io_fault.cu:16-275, io_wave_{xy,xz,yz}.cu): define
+ * them, write record it/EXPORT_TIME_SKIP, close. */
+int cgfdm_open_outputs(cgfdm_ctx* ctx);
+int cgfdm_write_outputs(cgfdm_ctx* ctx, int it);
+int cgfdm_close_outputs(cgfdm_ctx* ctx);
+
+/* Rate-and-state Newton statistics since setup (extension): number of node
+ * solves, total iterations, and solves that fell back to the plain guess. */
+int cgfdm_newton_stats(cgfdm_ctx* ctx, double* solves, double* iterations, double* fallbacks);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CGFDM_H */

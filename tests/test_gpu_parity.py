@@ -1,0 +1,150 @@
+"""HIP product path vs the FP64 oracle on the same seeded inputs (GPU).
+
+Tolerances are norm-wise relative errors, ||hip - oracle|| / ||oracle|| per
+wave component over the interior, stated in each test: FP32 kernels against
+an FP64 restatement.
+"""
+import numpy as np
+import pytest
+
+import cases
+from oracle_ref import Oracle
+
+pytestmark = pytest.mark.gpu
+
+dyn = pytest.importorskip("dynfd3dcuda_amd")
+
+FLAGS8 = [(fx, fy, fz) for fx in (-1, 1) for fy in (-1, 1) for fz in (-1, 1)]
+
+
+def make_pair(case):
+    s = dyn.Solver(case["params"])
+    cases.apply_to_solver(case, s)
+    s.setup()
+    o = Oracle(case["params"])
+    cases.apply_to_oracle(case, o)
+    o.setup()
+    return s, o
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / den if den > 0 else np.linalg.norm(a)
+
+
+def interior(v, d, ncomp):
+    v = v.reshape(ncomp, d["nx"], d["nz"], d["ny"])
+    return v[:, 3:-3, 3:-3, 3:-3]
+
+
+def test_setup_metric_matches_oracle():
+    case = cases.volume_case(curved=True)
+    s, o = make_pair(case)
+    d = o.dims()
+    m_h = interior(s.get("M"), d, 13)
+    m_o = interior(o.get("M"), d, 13)
+    for c in range(13):
+        assert relerr(m_h[c], m_o[c]) < 1e-6, c
+
+
+@pytest.mark.parametrize("flags", FLAGS8)
+def test_wave_rhs_curved_free_surface(flags):
+    """wave_deriv_cu (wave.cu:8-494) incl. free-surface DzV and traction
+    image, curvilinear grid, PML off: tolerance 2e-6 per component."""
+    case = cases.volume_case(curved=True, usePML=0)
+    s, o = make_pair(case)
+    d = o.dims()
+    W = cases.random_wavefield(d["nx"], d["ny"], d["nz"], seed=3)
+    s.put("W", W)
+    o.put("W", W)
+    s.wave_rhs(*flags)
+    o.wave_deriv(*flags)
+    h_h = interior(s.get("hW"), d, 9)
+    h_o = interior(o.get("hW"), d, 9)
+    for c in range(9):
+        e = relerr(h_h[c], h_o[c])
+        assert e < 2e-6, (flags, c, e)
+
+
+@pytest.mark.parametrize("pml,cerjan", [(1, 0), (0, 1), (1, 1)])
+def test_wave_steps_absorbing(pml, cerjan):
+    """Three full RK4 steps without the fault: stencil + ADE-PML + RK +
+    Cerjan against the oracle; tolerance 1e-5 per component."""
+    case = cases.volume_case(curved=True, usePML=pml, useCerjan=cerjan)
+    s, o = make_pair(case)
+    d = o.dims()
+    W = cases.random_wavefield(d["nx"], d["ny"], d["nz"], seed=5)
+    s.put("W", W)
+    o.put("W", W)
+    for it in range(3):
+        s.step(it)
+        o.step(it)
+    w_h = interior(s.get("W"), d, 9)
+    w_o = interior(o.get("W"), d, 9)
+    for c in range(9):
+        e = relerr(w_h[c], w_o[c])
+        assert e < 1e-5, (c, e)
+    if pml:
+        for f in ("PMLx1", "PMLy2", "PMLz1"):
+            e = relerr(s.get(f), o.get(f))
+            assert e < 1e-5, (f, e)
+
+
+def test_stage_sequence_flags():
+    """One step = 4 stages with flags negated each stage (main.cu:440-515)."""
+    case = cases.volume_case(curved=False, usePML=0)
+    s, o = make_pair(case)
+    d = o.dims()
+    W = cases.random_wavefield(d["nx"], d["ny"], d["nz"], seed=7)
+    s.put("W", W)
+    o.put("W", W)
+    for irk in range(4):
+        s.stage(5, irk)
+        o.stage(5, irk, *dyn.stage_flags(5, irk))
+    e = relerr(interior(s.get("W"), d, 9), interior(o.get("W"), d, 9))
+    assert e < 1e-5, e
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_fault_sw_steps(curved):
+    """Slip-weakening rupture, 40 steps: split-node fault kernels, trial_sw,
+    fault2wave, fault_rk and state (slip) RK against the oracle.
+    Tolerances: wavefield and split-node fields 1e-4, slip 1e-4."""
+    case = cases.sw_case(curved=curved)
+    s, o = make_pair(case)
+    d = o.dims()
+    nt = 40
+    for it in range(nt):
+        s.step(it)
+        o.step(it)
+    for name in ("Vs1", "Vs2", "Ts1", "Tn"):
+        e = relerr(s.get(name), o.get(name))
+        assert e < 1e-4, (name, e)
+    e = relerr(s.get("slip"), o.get("slip"))
+    assert e < 1e-4, ("slip", e)
+    e = relerr(s.get("FW"), o.get("FW"))
+    assert e < 1e-4, ("FW", e)
+    w_h = interior(s.get("W"), d, 9)
+    w_o = interior(o.get("W"), d, 9)
+    for c in range(9):
+        e = relerr(w_h[c], w_o[c])
+        assert e < 1e-4, (c, e)
+    # the rupture front reached the same nodes at the same step
+    t_h, t_o = s.get("init_t0"), o.get("init_t0")
+    assert np.array_equal(t_h > -1, t_o > -1)
+    assert np.max(np.abs(t_h - t_o)) <= case["params"]["DT"] + 1e-6
+
+
+def test_fault_rs_steps():
+    """Rate-and-state (aging law) with FP64 Newton, 30 steps."""
+    case = cases.rs_case()
+    s, o = make_pair(case)
+    nt = 30
+    for it in range(nt):
+        s.step(it)
+        o.step(it)
+    for name in ("Vs1", "Ts1", "Tn", "slip", "State"):
+        e = relerr(s.get(name), o.get(name))
+        assert e < 1e-4, (name, e)
