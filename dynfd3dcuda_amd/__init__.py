@@ -26,7 +26,7 @@ FD_FLAGS = [(-1, -1, -1), (1, 1, -1), (1, 1, 1), (-1, -1, 1),
 EXPORTED = [
     "cgfdm_last_error", "cgfdm_create", "cgfdm_create_from_string", "cgfdm_set_param",
     "cgfdm_get_param", "cgfdm_destroy", "cgfdm_set_fault_geometry", "cgfdm_set_coord3d",
-    "cgfdm_set_media1d", "cgfdm_set_init_stress", "cgfdm_setup", "cgfdm_step", "cgfdm_stage",
+    "cgfdm_set_topography", "cgfdm_set_media1d", "cgfdm_set_init_stress", "cgfdm_setup", "cgfdm_step", "cgfdm_stage",
     "cgfdm_run", "cgfdm_synchronize", "cgfdm_wave_stage", "cgfdm_wave_rhs", "cgfdm_field_info",
     "cgfdm_get_field", "cgfdm_put_field", "cgfdm_get_dims", "cgfdm_open_outputs",
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats",
@@ -67,6 +67,7 @@ def load_library(path=None):
     lib.cgfdm_destroy.argtypes = [c_p]
     lib.cgfdm_set_fault_geometry.argtypes = [c_p, c_p, c_p, c_p, ctypes.c_size_t]
     lib.cgfdm_set_coord3d.argtypes = [c_p, c_p, ctypes.c_size_t]
+    lib.cgfdm_set_topography.argtypes = [c_p, c_p, ctypes.c_size_t]
     lib.cgfdm_set_media1d.argtypes = [c_p, ctypes.c_int, c_p, c_p, c_p, c_p]
     lib.cgfdm_set_init_stress.argtypes = [c_p, ctypes.c_char_p, c_p, ctypes.c_size_t]
     lib.cgfdm_setup.argtypes = [c_p]
@@ -155,6 +156,11 @@ class Solver:
     def set_coord3d(self, xyz):
         a = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
         self._check(self.lib.cgfdm_set_coord3d(self.h, a.ctypes.data, a.size))
+
+    def set_topography(self, h):
+        a = np.ascontiguousarray(h, dtype=np.float32).ravel()
+        self._keep_topo = a
+        self._check(self.lib.cgfdm_set_topography(self.h, a.ctypes.data, a.size))
 
     def set_media1d(self, depth, rho, vp, vs):
         arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (depth, rho, vp, vs)]

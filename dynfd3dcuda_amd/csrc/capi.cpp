@@ -236,6 +236,12 @@ int cgfdm_set_coord3d(cgfdm_ctx* h, const double* xyz, size_t n) {
   return 0;
 }
 
+int cgfdm_set_topography(cgfdm_ctx* h, const float* t, size_t n) {
+  if (!h || !t) return fail(CGFDM_EPARAM, "null argument");
+  h->c.topo.assign(t, t + n);
+  return 0;
+}
+
 int cgfdm_set_media1d(cgfdm_ctx* h, int n, const float* depth, const float* rho, const float* vp, const float* vs) {
   if (!h || n <= 0 || !depth || !rho || !vp || !vs) return fail(CGFDM_EPARAM, "bad media1d arguments");
   Ctx& c = h->c;

@@ -97,6 +97,7 @@ struct Ctx {
   // host inputs
   std::vector<float> gx, gy, gz;
   std::vector<double> coord3d;
+  std::vector<float> topo;       // [ni][nj] surface height (extension)
   std::vector<float> m1d_depth, m1d_rho, m1d_vp, m1d_vs;
   std::map<std::string, std::vector<float>> stress_in;
   // device volume buffers
@@ -107,6 +108,7 @@ struct Ctx {
   float *slabH = nullptr;       // [7][9][nz][ny] fault-slab RHS
   float *matVx2Vz = nullptr, *matVy2Vz = nullptr;  // [nx][ny][9]
   float *cj[3] = {nullptr, nullptr, nullptr};      // Cerjan 1-D profiles (nx, ny, nz incl. ghosts)
+  void* wave_cold = nullptr;    // device copy of the off-hot-path pointer table
   PmlDev pml{};
   FaultDev f{};
   std::vector<void*> allocations;
@@ -130,5 +132,6 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz);
 void launch_fault_post(Ctx& c, const StageBufs& b, int irk);
 void launch_rup_sensor(Ctx& c);
 void launch_fill(Ctx& c, float* p, long n, float v);
+void upload_wave_cold(Ctx& c);
 
 }  // namespace cgfdm

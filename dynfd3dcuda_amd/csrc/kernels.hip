@@ -69,6 +69,52 @@ __device__ __forceinline__ void axis_rhs(const float* d, float e0, float e1, flo
   o[8] = (d[1] * e2 + d[2] * e1) * mu;
 }
 
+// Pointer tables used off the hot path (PML, Cerjan, free-surface maps) live
+// in device memory so that runtime face indices do not force the kernel
+// argument block into scratch.
+struct WaveCold {
+  const float* pA[3];
+  const float* pB[3];
+  const float* pD[3];
+  float* auxY[6];
+  float* auxS[6];
+  float* auxT[6];
+  const float* cj[3];
+  const float* mVx;
+  const float* mVy;
+  int on[6];
+  int N;
+};
+
+// Adds the RHS terms of one axis, derivative by derivative: D(c) is the
+// derivative of field c, (e0,e1,e2) the metric row of the axis (already
+// multiplied by 1/beta).  vel=false skips the velocity derivatives.
+template <class F>
+__device__ __forceinline__ void accum_axis(float* h, F D, float e0, float e1, float e2, float rrho, float lam, float l2m,
+                                           float mu, bool vel) {
+  if (vel) {
+    float d = D(0);  // Vx
+    h[3] += l2m * e0 * d; h[4] += lam * e0 * d; h[5] += lam * e0 * d; h[6] += mu * e1 * d; h[7] += mu * e2 * d;
+    d = D(1);  // Vy
+    h[3] += lam * e1 * d; h[4] += l2m * e1 * d; h[5] += lam * e1 * d; h[6] += mu * e0 * d; h[8] += mu * e2 * d;
+    d = D(2);  // Vz
+    h[3] += lam * e2 * d; h[4] += lam * e2 * d; h[5] += l2m * e2 * d; h[7] += mu * e0 * d; h[8] += mu * e1 * d;
+  }
+  const float r0 = rrho * e0, r1 = rrho * e1, r2 = rrho * e2;
+  float d = D(3);  // Txx
+  h[0] += r0 * d;
+  d = D(4);  // Tyy
+  h[1] += r1 * d;
+  d = D(5);  // Tzz
+  h[2] += r2 * d;
+  d = D(6);  // Txy
+  h[0] += r1 * d; h[1] += r0 * d;
+  d = D(7);  // Txz
+  h[0] += r2 * d; h[2] += r0 * d;
+  d = D(8);  // Tyz
+  h[1] += r2 * d; h[2] += r1 * d;
+}
+
 template <class MT>
 struct WaveArgs {
   const float* __restrict__ Win;
@@ -78,21 +124,11 @@ struct WaveArgs {
   const MT* __restrict__ met;
   float* __restrict__ slabH;
   float* __restrict__ rhs;
-  const float* __restrict__ mVx;
-  const float* __restrict__ mVy;
-  const float* pA[3];
-  const float* pB[3];
-  const float* pD[3];
-  float* auxY[6];
-  float* auxS[6];
-  float* auxT[6];
-  const float* cj[3];
-  int on[6];
-  int N;
+  const WaveCold* __restrict__ cold;
   int ni, nj, nk, ny, ii0;
   long nxyz, nyz;
   float rDH, rka, rkb;
-  int irk, freenode, cerjan, fault, rhs_only, ichunk;
+  int irk, freenode, cerjan, fault, rhs_only, ichunk, pml;
 };
 
 // Traction-image hV for the top three layers (wave.cu:398-487): the
@@ -105,10 +141,12 @@ __device__ void traction_image(const WaveArgs<MT>& a, long p, int k, float rb1, 
   const long nxyz = a.nxyz;
   int ik = a.nk - 1 - k;
   float rrhojac = rrho / ldm(M, 9 * nxyz + p);
-  float v[3][3][7];
   const long st[3] = {a.nyz, 1, (long)a.ny};
+  const int fl[3] = {FX, FY, FZ};
+  float d[3][3];  // d[axis][component]
 #pragma unroll
-  for (int ax = 0; ax < 3; ax++)
+  for (int ax = 0; ax < 3; ax++) {
+    float v[3][7];
 #pragma unroll
     for (int l = 0; l < 7; l++) {
       long q = p + (l - 3) * st[ax];
@@ -116,25 +154,25 @@ __device__ void traction_image(const WaveArgs<MT>& a, long p, int k, float rb1, 
       float J = ldm(M, 9 * nxyz + q);
       float txx = W[3 * nxyz + q], tyy = W[4 * nxyz + q], tzz = W[5 * nxyz + q];
       float txy = W[6 * nxyz + q], txz = W[7 * nxyz + q], tyz = W[8 * nxyz + q];
-      v[ax][0][l] = (e0 * txx + e1 * txy + e2 * txz) * J;
-      v[ax][1][l] = (e0 * txy + e1 * tyy + e2 * tyz) * J;
-      v[ax][2][l] = (e0 * txz + e1 * tyz + e2 * tzz) * J;
+      v[0][l] = (e0 * txx + e1 * txy + e2 * txz) * J;
+      v[1][l] = (e0 * txy + e1 * tyy + e2 * tyz) * J;
+      v[2][l] = (e0 * txz + e1 * tyz + e2 * tzz) * J;
+    }
+    if (ax == 2) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (ik == 0) { v[c][3] = 0.0f; v[c][6] = -v[c][0]; v[c][5] = -v[c][1]; v[c][4] = -v[c][2]; }
+        else if (ik == 1) { v[c][4] = 0.0f; v[c][6] = -v[c][2]; v[c][5] = -v[c][3]; }
+        else { v[c][5] = 0.0f; v[c][6] = -v[c][4]; }
+      }
     }
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    v[2][c][ik + 3] = 0.0f;
-    for (int l = 0; l < 3 - ik; l++) v[2][c][6 - l] = -v[2][c][l + 2 * ik];
+    for (int c = 0; c < 3; c++)
+      d[ax][c] = a.rDH * (fl[ax] > 0 ? drp5<1>(v[c][2], v[c][3], v[c][4], v[c][5], v[c][6])
+                                     : drp5<-1>(v[c][0], v[c][1], v[c][2], v[c][3], v[c][4]));
   }
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    float d1 = a.rDH * (FX > 0 ? drp5<1>(v[0][c][2], v[0][c][3], v[0][c][4], v[0][c][5], v[0][c][6])
-                               : drp5<-1>(v[0][c][0], v[0][c][1], v[0][c][2], v[0][c][3], v[0][c][4]));
-    float d2 = a.rDH * (FY > 0 ? drp5<1>(v[1][c][2], v[1][c][3], v[1][c][4], v[1][c][5], v[1][c][6])
-                               : drp5<-1>(v[1][c][0], v[1][c][1], v[1][c][2], v[1][c][3], v[1][c][4]));
-    float d3 = a.rDH * (FZ > 0 ? drp5<1>(v[2][c][2], v[2][c][3], v[2][c][4], v[2][c][5], v[2][c][6])
-                               : drp5<-1>(v[2][c][0], v[2][c][1], v[2][c][2], v[2][c][3], v[2][c][4]));
-    h[c] = (d1 * rb1 + d2 * rb2 + d3 * rb3) * rrhojac;
-  }
+  for (int c = 0; c < 3; c++) h[c] = (d[0][c] * rb1 + d[1][c] * rb2 + d[2][c] * rb3) * rrhojac;
 }
 
 // RK_Syn stage combine (rk4.cu:20-33) of one value.
@@ -214,79 +252,94 @@ __global__ void __launch_bounds__(TJ* TK) wave_stage_kernel(WaveArgs<MT> a) {
     if (act) {
       const long p = col + (long)(i + 3) * nyz;
       const MT* __restrict__ M = a.met;
-      float xix = ldm(M, p), xiy = ldm(M, nxyz + p), xiz = ldm(M, 2 * nxyz + p);
-      float etx = ldm(M, 3 * nxyz + p), ety = ldm(M, 4 * nxyz + p), etz = ldm(M, 5 * nxyz + p);
-      float ztx = ldm(M, 6 * nxyz + p), zty = ldm(M, 7 * nxyz + p), ztz = ldm(M, 8 * nxyz + p);
-      float lam = ldm(M, 10 * nxyz + p), mu = ldm(M, 11 * nxyz + p), rrho = 1.0f / ldm(M, 12 * nxyz + p);
+      const float xix = ldm(M, p), xiy = ldm(M, nxyz + p), xiz = ldm(M, 2 * nxyz + p);
+      const float etx = ldm(M, 3 * nxyz + p), ety = ldm(M, 4 * nxyz + p), etz = ldm(M, 5 * nxyz + p);
+      const float ztx = ldm(M, 6 * nxyz + p), zty = ldm(M, 7 * nxyz + p), ztz = ldm(M, 8 * nxyz + p);
+      const float lam = ldm(M, 10 * nxyz + p), mu = ldm(M, 11 * nxyz + p), rrho = 1.0f / ldm(M, 12 * nxyz + p);
+      const float l2m = lam + 2.0f * mu;
       const float rDH = a.rDH;
-      float dx[9], dy[9], dz[9];
-#pragma unroll
-      for (int c = 0; c < 9; c++) {
-        dx[c] = rDH * drp5<FX>(q[0][c], q[1][c], q[2][c], q[3][c], q[4][c]);
-        const int lj = tj + HJL, lk = tk + HKL;
-        if (FY > 0) dy[c] = rDH * drp5<1>(lds[c][lk][lj - 1], lds[c][lk][lj], lds[c][lk][lj + 1], lds[c][lk][lj + 2], lds[c][lk][lj + 3]);
-        else dy[c] = rDH * drp5<-1>(lds[c][lk][lj - 3], lds[c][lk][lj - 2], lds[c][lk][lj - 1], lds[c][lk][lj], lds[c][lk][lj + 1]);
-        if (FZ > 0) dz[c] = rDH * drp5<1>(lds[c][lk - 1][lj], lds[c][lk][lj], lds[c][lk + 1][lj], lds[c][lk + 2][lj], lds[c][lk + 3][lj]);
-        else dz[c] = rDH * drp5<-1>(lds[c][lk - 3][lj], lds[c][lk - 2][lj], lds[c][lk - 1][lj], lds[c][lk][lj], lds[c][lk + 1][lj]);
-      }
+      const WaveCold* __restrict__ cold = a.cold;
+      const float rb1 = a.pml ? 1.0f / cold->pB[0][i] : 1.0f;
+      const float rb2 = a.pml ? 1.0f / cold->pB[1][j] : 1.0f;
+      const float rb3 = a.pml ? 1.0f / cold->pB[2][k] : 1.0f;
+      const int lj = tj + HJL, lk = tk + HKL;
+      // one-sided DRP derivative of field c along each axis (macdrp.h:91-95)
+      auto Dx = [&](int c) { return rDH * drp5<FX>(q[0][c], q[1][c], q[2][c], q[3][c], q[4][c]); };
+      auto Dy = [&](int c) {
+        return FY > 0 ? rDH * drp5<1>(lds[c][lk][lj - 1], lds[c][lk][lj], lds[c][lk][lj + 1], lds[c][lk][lj + 2], lds[c][lk][lj + 3])
+                      : rDH * drp5<-1>(lds[c][lk][lj - 3], lds[c][lk][lj - 2], lds[c][lk][lj - 1], lds[c][lk][lj], lds[c][lk][lj + 1]);
+      };
+      auto Dz = [&](int c) {
+        return FZ > 0 ? rDH * drp5<1>(lds[c][lk - 1][lj], lds[c][lk][lj], lds[c][lk + 1][lj], lds[c][lk + 2][lj], lds[c][lk + 3][lj])
+                      : rDH * drp5<-1>(lds[c][lk - 3][lj], lds[c][lk - 2][lj], lds[c][lk - 1][lj], lds[c][lk][lj], lds[c][lk + 1][lj]);
+      };
       const bool top = a.freenode && k >= nk - 3;
-      float dz_plain[3] = {dz[0], dz[1], dz[2]};
+      // RHS accumulated derivative by derivative (wave.cu:133-276): each
+      // field's derivative adds its momentum / Hooke terms, metric rows
+      // pre-scaled by the PML 1/beta of their axis.
+      float h[9];
+#pragma unroll
+      for (int c = 0; c < 9; c++) h[c] = 0.0f;
+      accum_axis(h, Dx, xix * rb1, xiy * rb1, xiz * rb1, rrho, lam, l2m, mu, true);
+      accum_axis(h, Dy, etx * rb2, ety * rb2, etz * rb2, rrho, lam, l2m, mu, true);
+      accum_axis(h, Dz, ztx * rb3, zty * rb3, ztz * rb3, rrho, lam, l2m, mu, !top);
       if (top) {
-        const int lj = tj + HJL, lk = tk + HKL;
+        float dzv[3];
         if (k == nk - 1) {  // DzV from the traction-free condition (wave.cu:207-232)
-          long ij = ((j + 3) + (long)(i + 3) * ny) * 9;
+          const long ij = ((j + 3) + (long)(i + 3) * ny) * 9;
+          const float* mVx = cold->mVx;
+          const float* mVy = cold->mVy;
+          const float dxv[3] = {Dx(0), Dx(1), Dx(2)}, dyv[3] = {Dy(0), Dy(1), Dy(2)};
 #pragma unroll
           for (int r = 0; r < 3; r++)
-            dz[r] = a.mVx[ij + 3 * r] * dx[0] + a.mVx[ij + 3 * r + 1] * dx[1] + a.mVx[ij + 3 * r + 2] * dx[2]
-                  + a.mVy[ij + 3 * r] * dy[0] + a.mVy[ij + 3 * r + 1] * dy[1] + a.mVy[ij + 3 * r + 2] * dy[2];
+            dzv[r] = mVx[ij + 3 * r] * dxv[0] + mVx[ij + 3 * r + 1] * dxv[1] + mVx[ij + 3 * r + 2] * dxv[2]
+                   + mVy[ij + 3 * r] * dyv[0] + mVy[ij + 3 * r + 1] * dyv[1] + mVy[ij + 3 * r + 2] * dyv[2];
         } else if (k == nk - 2) {  // L22 (wave.cu:233-238)
 #pragma unroll
-          for (int c = 0; c < 3; c++) dz[c] = rDH * (FZ > 0 ? lds[c][lk + 1][lj] - lds[c][lk][lj] : lds[c][lk][lj] - lds[c][lk - 1][lj]);
+          for (int c = 0; c < 3; c++) dzv[c] = rDH * (FZ > 0 ? lds[c][lk + 1][lj] - lds[c][lk][lj] : lds[c][lk][lj] - lds[c][lk - 1][lj]);
         } else {  // L24 (wave.cu:239-245)
 #pragma unroll
           for (int c = 0; c < 3; c++)
-            dz[c] = rDH * (FZ > 0 ? -C24_1 * lds[c][lk][lj] + C24_2 * lds[c][lk + 1][lj] - C24_3 * lds[c][lk + 2][lj]
-                                  : C24_3 * lds[c][lk - 2][lj] - C24_2 * lds[c][lk - 1][lj] + C24_1 * lds[c][lk][lj]);
+            dzv[c] = rDH * (FZ > 0 ? -C24_1 * lds[c][lk][lj] + C24_2 * lds[c][lk + 1][lj] - C24_3 * lds[c][lk + 2][lj]
+                                   : C24_3 * lds[c][lk - 2][lj] - C24_2 * lds[c][lk - 1][lj] + C24_1 * lds[c][lk][lj]);
         }
+        const float e0 = ztx * rb3, e1 = zty * rb3, e2 = ztz * rb3;
+        h[3] += (l2m * e0 * dzv[0] + lam * e1 * dzv[1] + lam * e2 * dzv[2]);
+        h[4] += (lam * e0 * dzv[0] + l2m * e1 * dzv[1] + lam * e2 * dzv[2]);
+        h[5] += (lam * e0 * dzv[0] + lam * e1 * dzv[1] + l2m * e2 * dzv[2]);
+        h[6] += (e1 * dzv[0] + e0 * dzv[1]) * mu;
+        h[7] += (e2 * dzv[0] + e0 * dzv[2]) * mu;
+        h[8] += (e2 * dzv[1] + e1 * dzv[2]) * mu;
+        traction_image<FX, FY, FZ, MT>(a, p, k, rb1, rb2, rb3, rrho, h);
       }
-      (void)dz_plain;
-      const float rb1 = 1.0f / a.pB[0][i], rb2 = 1.0f / a.pB[1][j], rb3 = 1.0f / a.pB[2][k];
-      float ox[9], oy[9], oz[9], h[9];
-      axis_rhs(dx, xix, xiy, xiz, rrho, lam, mu, ox);
-      axis_rhs(dy, etx, ety, etz, rrho, lam, mu, oy);
-      axis_rhs(dz, ztx, zty, ztz, rrho, lam, mu, oz);
-#pragma unroll
-      for (int c = 0; c < 9; c++) h[c] = ox[c] * rb1 + oy[c] * rb2 + oz[c] * rb3;
-      if (top) traction_image<FX, FY, FZ, MT>(a, p, k, rb1, rb2, rb3, rrho, h);
 
-      // ADE-PML (pmlx.cu:126-210 and y/z variants)
-      if (a.N > 0) {
-        const int N = a.N;
-        int face[3] = {-1, -1, -1};
-        if (a.on[0] && i < N) face[0] = 0; else if (a.on[1] && i >= ni - N) face[0] = 1;
-        if (a.on[2] && j < N) face[1] = 2; else if (a.on[3] && j >= nj - N) face[1] = 3;
-        if (a.on[4] && k < N) face[2] = 4; else if (a.on[5] && k >= nk - N) face[2] = 5;
-        // a point may also sit in both slabs of one axis on tiny grids
-        int face2[3] = {-1, -1, -1};
-        if (face[0] == 0 && a.on[1] && i >= ni - N) face2[0] = 1;
-        if (face[1] == 2 && a.on[3] && j >= nj - N) face2[1] = 3;
-        if (face[2] == 4 && a.on[5] && k >= nk - N) face2[2] = 5;
+      // ADE-PML (pmlx.cu:126-210 and the y/z variants): -aux/beta in the
+      // RHS and the auxiliary ODE from the plain derivative along the axis.
+      if (a.pml) {
+        const int N = cold->N;
+        const int n3[3] = {ni, nj, nk};
+        const int pos3[3] = {i, j, k};
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
+          const int li = pos3[ax];
 #pragma unroll
-          for (int rep = 0; rep < 2; rep++) {
-            int f = rep == 0 ? face[ax] : face2[ax];
-            if (f < 0) continue;
-            int li = ax == 0 ? i : ax == 1 ? j : k;
-            int off = (f & 1) ? (ax == 0 ? ni : ax == 1 ? nj : nk) - N : 0;
-            int pi = ax == 0 ? i - off : i, pj = ax == 1 ? j - off : j, pk = ax == 2 ? k - off : k;
-            int pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
-            long ps = (long)pni * pnj * pnk;
-            long pa = pj + (long)pk * pnj + (long)pi * pnj * pnk;
-            float b1 = a.pB[ax][li], d1 = a.pD[ax][li], ad1 = -(a.pA[ax][li] + d1);
-            float rbb = 1.0f / b1;
-            const float* Dr = ax == 0 ? ox : ax == 1 ? oy : oz;
-            float* Y = a.auxY[f];
+          for (int side = 0; side < 2; side++) {
+            const int f = 2 * ax + side;
+            const int off = side ? n3[ax] - N : 0;
+            if (!cold->on[f] || li < off || li >= off + N) continue;
+            float dd[9], Dr[9];
+#pragma unroll
+            for (int c = 0; c < 9; c++) dd[c] = ax == 0 ? Dx(c) : ax == 1 ? Dy(c) : Dz(c);
+            if (ax == 0) axis_rhs(dd, xix, xiy, xiz, rrho, lam, mu, Dr);
+            else if (ax == 1) axis_rhs(dd, etx, ety, etz, rrho, lam, mu, Dr);
+            else axis_rhs(dd, ztx, zty, ztz, rrho, lam, mu, Dr);
+            const int pi = ax == 0 ? i - off : i, pj = ax == 1 ? j - off : j, pk = ax == 2 ? k - off : k;
+            const int pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
+            const long ps = (long)pni * pnj * pnk;
+            const long pa = pj + (long)pk * pnj + (long)pi * pnj * pnk;
+            const float b1 = cold->pB[ax][li], d1 = cold->pD[ax][li], ad1 = -(cold->pA[ax][li] + d1);
+            const float rbb = 1.0f / b1;
+            float* Y = cold->auxY[f];
             float ah[9];
 #pragma unroll
             for (int c = 0; c < 9; c++) {
@@ -295,13 +348,12 @@ __global__ void __launch_bounds__(TJ* TK) wave_stage_kernel(WaveArgs<MT> a) {
               ah[c] = ad1 * y + d1 * Dr[c];
             }
             if (ax < 2 && a.freenode && k == nk - 1) {  // pmlx.cu:185-210
-              const float* mat = ax == 0 ? a.mVx : a.mVy;
-              const float* dd = ax == 0 ? dx : dy;
-              long ij = ((j + 3) + (long)(i + 3) * ny) * 9;
+              const float* mat = ax == 0 ? cold->mVx : cold->mVy;
+              const long ij = ((j + 3) + (long)(i + 3) * ny) * 9;
               float z1[3];
 #pragma unroll
               for (int r = 0; r < 3; r++) z1[r] = mat[ij + 3 * r] * dd[0] + mat[ij + 3 * r + 1] * dd[1] + mat[ij + 3 * r + 2] * dd[2];
-              float c0 = d1 * b1, l2m = lam + 2.0f * mu;
+              const float c0 = d1 * b1;
               ah[3] += c0 * (l2m * ztx * z1[0] + lam * zty * z1[1] + lam * ztz * z1[2]);
               ah[4] += c0 * (lam * ztx * z1[0] + l2m * zty * z1[1] + lam * ztz * z1[2]);
               ah[5] += c0 * (lam * ztx * z1[0] + lam * zty * z1[1] + l2m * ztz * z1[2]);
@@ -310,12 +362,12 @@ __global__ void __launch_bounds__(TJ* TK) wave_stage_kernel(WaveArgs<MT> a) {
               ah[8] += c0 * mu * (ztz * z1[1] + zty * z1[2]);
             }
             if (!a.rhs_only) {
-              float* S = a.auxS[f];
-              float* T = a.auxT[f];
+              float* S = cold->auxS[f];
+              float* T = cold->auxT[f];
 #pragma unroll
               for (int c = 0; c < 9; c++) {
-                long x = c * ps + pa;
-                float y = Y[x];
+                const long x = c * ps + pa;
+                const float y = Y[x];
                 if (a.irk == 0) S[x] = y;
                 Y[x] = rk_combine(a.irk, a.rka, a.rkb, y, ah[c], S, T, x);
               }
@@ -333,7 +385,7 @@ __global__ void __launch_bounds__(TJ* TK) wave_stage_kernel(WaveArgs<MT> a) {
         for (int c = 0; c < 9; c++) a.slabH[sp + c * nyz] = h[c];
       } else {
         float damp = 1.0f;
-        if (a.cerjan) damp = fminf(a.cj[0][i + 3], fminf(a.cj[1][j + 3], a.cj[2][k + 3]));
+        if (a.cerjan) damp = fminf(cold->cj[0][i + 3], fminf(cold->cj[1][j + 3], cold->cj[2][k + 3]));
 #pragma unroll
         for (int c = 0; c < 9; c++) {
           long x = c * nxyz + p;
@@ -382,11 +434,10 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
   a.met = (const MT*)c.met;
   a.slabH = c.slabH;
   a.rhs = rhs_out;
-  a.mVx = c.matVx2Vz; a.mVy = c.matVy2Vz;
-  for (int x = 0; x < 3; x++) { a.pA[x] = c.pml.A[x]; a.pB[x] = c.pml.B[x]; a.pD[x] = c.pml.D[x]; a.cj[x] = c.cj[x]; }
+  a.cold = (const WaveCold*)c.wave_cold;
   bool any = false;
-  for (int f = 0; f < 6; f++) { a.on[f] = c.pml.on[f]; a.auxY[f] = c.pml.Y[f]; a.auxS[f] = c.pml.S[f]; a.auxT[f] = c.pml.T[f]; any |= a.on[f] != 0; }
-  a.N = any ? c.pml.N : 0;
+  for (int f = 0; f < 6; f++) any |= c.pml.on[f] != 0;
+  a.pml = any ? 1 : 0;
   a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.ny = c.d.ny; a.ii0 = c.d.i0 - 3;
   a.nxyz = c.d.nxyz; a.nyz = c.d.nyz;
   a.rDH = (float)(1.0 / c.p.DH);
@@ -1108,6 +1159,19 @@ FArgs make_fargs(Ctx& c) {
 dim3 fgrid(const Ctx& c) { return dim3((c.d.nj + 63) / 64, (c.d.nk + 3) / 4, 1); }
 
 }  // namespace
+
+void upload_wave_cold(Ctx& c) {
+  WaveCold w{};
+  for (int x = 0; x < 3; x++) { w.pA[x] = c.pml.A[x]; w.pB[x] = c.pml.B[x]; w.pD[x] = c.pml.D[x]; w.cj[x] = c.cj[x]; }
+  for (int f = 0; f < 6; f++) { w.auxY[f] = c.pml.Y[f]; w.auxS[f] = c.pml.S[f]; w.auxT[f] = c.pml.T[f]; w.on[f] = c.pml.on[f]; }
+  w.mVx = c.matVx2Vz; w.mVy = c.matVy2Vz;
+  w.N = c.pml.N;
+  if (!c.wave_cold) {
+    hipMalloc(&c.wave_cold, sizeof(WaveCold));
+    c.allocations.push_back(c.wave_cold);
+  }
+  hipMemcpy(c.wave_cold, &w, sizeof w, hipMemcpyHostToDevice);
+}
 
 void launch_wave_stage(Ctx& c, const StageBufs& b, int irk, int fx, int fy, int fz, bool rhs_only, float* rhs_out) {
   if (c.p.MetricBF16) {

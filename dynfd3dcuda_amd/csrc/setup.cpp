@@ -169,6 +169,7 @@ int host_setup(Ctx& c, std::string& err) {
           y = c.gy[f];
           z = c.gz[f];
         }
+        if (!c.topo.empty()) z += c.topo[(long)gi * nj + gj] * (double)gk / (double)(p.NZ - 1);
         X[q] = x; Y[q] = y; Z[q] = z;
       }
   for (int l = 0; l < 3; l++) {
@@ -580,6 +581,7 @@ int host_setup(Ctx& c, std::string& err) {
     if (!c.slabH || !F.newton_stats) { err = "hipMalloc failed for the fault"; return 6; }
   }
   c.hW = nullptr;
+  upload_wave_cold(c);
   if (hipDeviceSynchronize() != hipSuccess) { err = "HIP error during setup upload"; return 6; }
   return 0;
 }

@@ -59,6 +59,10 @@ int cgfdm_set_fault_geometry(cgfdm_ctx* ctx, const float* x, const float* y, con
 /* Full 3-D coordinates [3][nx][nz][ny] with ghosts (interior used; extension
  * for topography that varies along the fault normal). */
 int cgfdm_set_coord3d(cgfdm_ctx* ctx, const double* xyz, size_t n);
+/* Surface topography h [ni][nj] in metres (extension, no counterpart in
+ * the reference, whose grid translates the fault surface along x so the
+ * surface height cannot vary with x): z(i,j,k) += h(i,j) * k/(NZ-1). */
+int cgfdm_set_topography(cgfdm_ctx* ctx, const float* h, size_t n);
 /* Media1D layers (srcd/media.cu:63-138). */
 int cgfdm_set_media1d(cgfdm_ctx* ctx, int nlayer, const float* depth, const float* rho, const float* vp, const float* vs);
 /* Fault_init_stress variable by its netCDF name: Tx Ty Tz dTx dTy dTz mu_s

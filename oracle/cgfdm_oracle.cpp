@@ -159,6 +159,7 @@ struct Oracle {
   std::vector<real> gx, gy, gz;  // Fault_geometry x,y,z [nk][nj]
   bool have_geom = false;
   std::vector<real> coord3d;     // optional full coordinates [3][nxyz] (interior used)
+  std::vector<real> topo;        // optional surface height [ni][nj] (extension)
   std::vector<real> m1d_depth, m1d_rho, m1d_vp, m1d_vs;  // Media1D layers
   std::vector<real> m3d_vp, m3d_vs, m3d_rho;            // Media3D [ni][nk][nj]
   std::vector<real> in_Tx, in_Ty, in_Tz, in_dTx, in_dTy, in_dTz;
@@ -273,6 +274,7 @@ struct Oracle {
             y = gy[f];
             z = gz[f];
           }
+          if (!topo.empty()) z += topo[(long)gi * nj + gj] * (real)gk / (real)(NZ - 1);
           long p = P(i, j, k);
           X[p] = x; Y[p] = y; Z[p] = z;
           if (!coord3d.empty()) {  // full 3-D coordinate input (extension)
@@ -1529,6 +1531,10 @@ int orc_set_geometry(void* h, const double* x, const double* y, const double* z,
 }
 int orc_set_coord3d(void* h, const double* c, long n) {
   ((Oracle*)h)->coord3d.assign(c, c + n);
+  return 0;
+}
+int orc_set_topography(void* h, const double* t, long n) {
+  ((Oracle*)h)->topo.assign(t, t + n);
   return 0;
 }
 int orc_set_media1d(void* h, int n, const double* depth, const double* rho, const double* vp, const double* vs) {

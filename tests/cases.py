@@ -104,6 +104,8 @@ def apply_to_solver(case, solver):
         solver.set_fault_geometry(*case["geom"])
     if "media1d" in case:
         solver.set_media1d(*case["media1d"])
+    if case.get("topo") is not None:
+        solver.set_topography(case["topo"])
     for k, v in case.get("stress", {}).items():
         solver.set_init_stress(k, v)
 
@@ -113,6 +115,8 @@ def apply_to_oracle(case, orc):
         orc.set_geometry(*case["geom"])
     if "media1d" in case:
         orc.set_media1d(*case["media1d"])
+    if case.get("topo") is not None:
+        orc.set_topography(case["topo"])
     for k, v in case.get("stress", {}).items():
         orc.set_stress(k, v)
 

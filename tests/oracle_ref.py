@@ -31,6 +31,7 @@ def load():
     lib.orc_set_param.argtypes = [P, ctypes.c_char_p, D]
     lib.orc_set_geometry.argtypes = [P, P, P, P, L]
     lib.orc_set_coord3d.argtypes = [P, P, L]
+    lib.orc_set_topography.argtypes = [P, P, L]
     lib.orc_set_media1d.argtypes = [P, ctypes.c_int, P, P, P, P]
     lib.orc_set_media3d.argtypes = [P, P, P, P, L]
     lib.orc_set_stress.argtypes = [P, ctypes.c_char_p, P, L]
@@ -49,7 +50,7 @@ def load():
     lib.orc_size.restype = L
     lib.orc_get.argtypes = [P, ctypes.c_char_p, P, L]
     lib.orc_put.argtypes = [P, ctypes.c_char_p, P, L]
-    for n in ["orc_set_param", "orc_set_geometry", "orc_set_coord3d", "orc_set_media1d", "orc_set_media3d",
+    for n in ["orc_set_param", "orc_set_geometry", "orc_set_coord3d", "orc_set_topography", "orc_set_media1d", "orc_set_media3d",
               "orc_set_stress", "orc_setup", "orc_step", "orc_stage", "orc_rup_sensor", "orc_wave_deriv",
               "orc_pml_deriv", "orc_wave_rk", "orc_apply_cerjan", "orc_nr_search", "orc_get", "orc_put"]:
         getattr(lib, n).restype = ctypes.c_int
@@ -82,6 +83,11 @@ class Oracle:
     def set_coord3d(self, xyz):
         a = np.ascontiguousarray(xyz, dtype=np.float64).ravel()
         assert self.lib.orc_set_coord3d(self.h, a.ctypes.data, a.size) == 0
+
+    def set_topography(self, h):
+        a = np.ascontiguousarray(h, dtype=np.float64).ravel()
+        self._t = a
+        assert self.lib.orc_set_topography(self.h, a.ctypes.data, a.size) == 0
 
     def set_media1d(self, depth, rho, vp, vs):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (depth, rho, vp, vs)]
