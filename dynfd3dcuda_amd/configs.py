@@ -152,7 +152,7 @@ def config1(seed=1, nt=4000, scale=1):
     """Curved grid: 400x300x200 at 50 m, topography, bent fault, 3-layer
     medium, slip weakening, 20-layer ADE-PML, FP32."""
     rng = np.random.default_rng(seed)
-    NX, NY, NZ, DH, DT = 400 * scale, 300 * scale, 200 * scale, 50.0 / scale, 0.004 / scale
+    NX, NY, NZ, DH, DT = 400 * scale, 300 * scale, 200 * scale, 50.0 / scale, 0.003 / scale
     p = _params(NX, NY, NZ, DH, DT, nt)
     return {"name": "config1", "params": p,
             "geom": bent_fault_geometry(NY, NZ, DH, 20.0, rng),
@@ -188,7 +188,7 @@ def config4(seed=4, nt=8000):
     guess, bf16 metric/media storage."""
     c = config3(seed=seed, nt=nt)
     rng = np.random.default_rng(seed)
-    NX, NY, NZ, DH, DT = 1600, 1000, 500, 25.0, 0.002
+    NX, NY, NZ, DH, DT = 1600, 1000, 500, 25.0, 0.0015
     p = _params(NX, NY, NZ, DH, DT, nt, Friction_type=1, RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12,
                 RS_L=0.02, RS_fw=0.2, smooth_load_T=1.0, MetricBF16=1, NewtonMode=2,
                 NewtonTol=1e-10, NewtonFallbackIter=20)
