@@ -57,7 +57,8 @@ int check_hip(const char* what) {
 struct FieldRef {
   void* ptr = nullptr;
   size_t count = 0;
-  int dtype = 0;  // 0 f32, 1 f64, 2 i32
+  int dtype = 0;   // 0 f32, 1 f64, 2 i32, 3 bf16
+  int nf = 0;      // >0: plane-interleaved volume array with nf fields per plane on the device
 };
 
 FieldRef field(Ctx& c, const std::string& n) {
@@ -67,10 +68,10 @@ FieldRef field(Ctx& c, const std::string& n) {
   auto f32 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 0; };
   auto f64 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 1; };
   auto i32 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 2; };
-  if (n == "W") f32(c.cur, 9 * nxyz);
-  else if (n == "hW") f32(c.hW, 9 * nxyz);
-  else if (n == "M") f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz);
-  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; }
+  if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; }
+  else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; }
+  else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; }
+  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; r.nf = 13; }
   else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "matVy2Vz") f32(c.matVy2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "Bx") f32(c.pml.B[0], c.d.ni);
@@ -220,6 +221,7 @@ int cgfdm_destroy(cgfdm_ctx* h) {
   if (c.stream) { hipStreamSynchronize(c.stream); hipStreamDestroy(c.stream); }
   for (void* p : c.allocations) hipFree(p);
   if (c.hW) hipFree(c.hW);
+  if (c.wave_work) hipFree(c.wave_work);
   delete h;
   return 0;
 }
@@ -354,6 +356,18 @@ int cgfdm_wave_rhs(cgfdm_ctx* h, int fx, int fy, int fz) {
   return check_hip("wave_rhs launch");
 }
 
+// Between the caller's layout [field][plane][jk] and the device's
+// plane-interleaved [plane][field][jk]; to_dev selects the direction.
+static void interleave_copy(const Ctx& c, const FieldRef& r, const char* from, char* to, bool to_dev) {
+  const size_t es = dsize(r.dtype), nyz = (size_t)c.d.nyz, nx = (size_t)c.d.nx, nf = (size_t)r.nf;
+  const size_t row = nyz * es;
+  for (size_t i = 0; i < nx; i++)
+    for (size_t f = 0; f < nf; f++) {
+      const size_t dev = (i * nf + f) * row, ref = (f * nx + i) * row;
+      std::memcpy(to + (to_dev ? dev : ref), from + (to_dev ? ref : dev), row);
+    }
+}
+
 int cgfdm_field_info(cgfdm_ctx* h, const char* name, size_t* count, int* dtype) {
   if (!h || !name) return fail(CGFDM_EPARAM, "null argument");
   FieldRef r = field(h->c, name);
@@ -371,7 +385,14 @@ int cgfdm_get_field(cgfdm_ctx* h, const char* name, void* dst, size_t bytes) {
   if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
   if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
   hipStreamSynchronize(c.stream);
-  if (hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  if (!r.nf) {
+    if (hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+    return 0;
+  }
+  // device [plane][field][jk] -> caller [field][plane][jk] (the reference's layout)
+  std::vector<char> tmp(bytes);
+  if (hipMemcpy(tmp.data(), r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  interleave_copy(c, r, tmp.data(), (char*)dst, false);
   return 0;
 }
 
@@ -387,7 +408,13 @@ int cgfdm_put_field(cgfdm_ctx* h, const char* name, const void* src, size_t byte
   if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
   if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
   hipStreamSynchronize(c.stream);
-  if (hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  if (!r.nf) {
+    if (hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+    return 0;
+  }
+  std::vector<char> tmp(bytes);
+  interleave_copy(c, r, (const char*)src, tmp.data(), true);
+  if (hipMemcpy(r.ptr, tmp.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
   return 0;
 }
 
@@ -546,18 +573,18 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
       if (axis == 2) {  // xy: rows of ny for each i
         line.resize(ny);
         for (int a = 0; a < d1n; a++) {
-          hipMemcpy(line.data(), c.cur + comp * nxyz + (long)(idx + 3) * ny + (long)(a + 3) * nyz, ny * 4, hipMemcpyDeviceToHost);
+          hipMemcpy(line.data(), c.cur + ((long)(a + 3) * 9 + comp) * nyz + (long)(idx + 3) * ny, ny * 4, hipMemcpyDeviceToHost);
           std::memcpy(&buf[(long)a * d2n], &line[3], d2n * 4);
         }
       } else if (axis == 1) {  // xz
         std::vector<float> plane(nyz);
         for (int a = 0; a < d1n; a++) {
-          hipMemcpy(plane.data(), c.cur + comp * nxyz + (long)(a + 3) * nyz, nyz * 4, hipMemcpyDeviceToHost);
+          hipMemcpy(plane.data(), c.cur + ((long)(a + 3) * 9 + comp) * nyz, nyz * 4, hipMemcpyDeviceToHost);
           for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[(idx + 3) + (long)(b + 3) * ny];
         }
       } else {  // yz plane at i
         std::vector<float> plane(nyz);
-        hipMemcpy(plane.data(), c.cur + comp * nxyz + (long)(idx + 3) * nyz, nyz * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(plane.data(), c.cur + ((long)(idx + 3) * 9 + comp) * nyz, nyz * 4, hipMemcpyDeviceToHost);
         for (int a = 0; a < d1n; a++)
           for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[(b + 3) + (long)(a + 3) * ny];
       }

@@ -39,6 +39,7 @@ struct Params {
   double NewtonTol = 1e-10;
   int NewtonFallbackIter = 20;
   std::string NewtonMLP; // path of trained MLP weights for the Newton guess
+  int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
 
   // derived (set_params.cu:286-295)
   int ni = 0, nj = 0, nk = 0, nx = 0, ny = 0, nz = 0, NT = 0;
@@ -109,6 +110,8 @@ struct Ctx {
   float *matVx2Vz = nullptr, *matVy2Vz = nullptr;  // [nx][ny][9]
   float *cj[3] = {nullptr, nullptr, nullptr};      // Cerjan 1-D profiles (nx, ny, nz incl. ghosts)
   void* wave_cold = nullptr;    // device copy of the off-hot-path pointer table
+  void* wave_work = nullptr;    // stage-kernel work list (two int4 per block)
+  int wave_nwork = 0, wave_nwork_int = 0;  // all items; leading interior items
   PmlDev pml{};
   FaultDev f{};
   std::vector<void*> allocations;
@@ -132,6 +135,6 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz);
 void launch_fault_post(Ctx& c, const StageBufs& b, int irk);
 void launch_rup_sensor(Ctx& c);
 void launch_fill(Ctx& c, float* p, long n, float v);
-void upload_wave_cold(Ctx& c);
+int upload_wave_cold(Ctx& c);  // nonzero on a HIP error
 
 }  // namespace cgfdm

@@ -105,6 +105,9 @@ int host_setup(Ctx& c, std::string& err) {
   d.nxyz = (long)d.nx * d.ny * d.nz;
   d.nyz = (long)d.ny * d.nz;
   d.i0 = d.nx / 2;
+  // the stage kernel addresses each field with 32-bit byte offsets
+  if (d.nxyz >= (1L << 30)) { err = "grid too large: (NX+6)(NY+6)(NZ+6) must stay below 2^30 points per GPU"; return 1; }
+  if (p.NX < 7 || p.NY < 7 || p.NZ < 7) { err = "NX, NY and NZ must be at least 7"; return 1; }
   const int nx = d.nx, ny = d.ny, nz = d.nz, ni = d.ni, nj = d.nj, nk = d.nk;
   const long nxyz = d.nxyz, nyz = d.nyz;
   auto P = [&](int i, int j, int k) { return j + (long)k * ny + (long)i * nyz; };
@@ -351,17 +354,22 @@ int host_setup(Ctx& c, std::string& err) {
   c.Tacc = dmalloc<float>(c, 9 * nxyz);
   if (!c.S || !c.A || !c.B || !c.Tacc) { err = "hipMalloc failed for the wavefield"; return 6; }
   c.cur = c.S;
+  // device metric is plane-interleaved: field m of plane i at (i*13 + m)*nyz + jk
   if (p.MetricBF16) {
     std::vector<uint16_t> h(13 * nxyz);
 #pragma omp parallel for
-    for (long q = 0; q < 13 * nxyz; q++) h[q] = to_bf16((float)Md[q]);
+    for (int i = 0; i < nx; i++)
+      for (int m = 0; m < 13; m++)
+        for (long jk = 0; jk < nyz; jk++) h[((long)i * 13 + m) * nyz + jk] = to_bf16((float)Md[m * nxyz + i * nyz + jk]);
     uint16_t* dm = dmalloc<uint16_t>(c, 13 * nxyz);
     if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
     c.met = dm;
   } else {
     std::vector<float> h(13 * nxyz);
 #pragma omp parallel for
-    for (long q = 0; q < 13 * nxyz; q++) h[q] = (float)Md[q];
+    for (int i = 0; i < nx; i++)
+      for (int m = 0; m < 13; m++)
+        for (long jk = 0; jk < nyz; jk++) h[((long)i * 13 + m) * nyz + jk] = (float)Md[m * nxyz + i * nyz + jk];
     float* dm = dmalloc<float>(c, 13 * nxyz);
     if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
     c.met = dm;
@@ -581,7 +589,7 @@ int host_setup(Ctx& c, std::string& err) {
     if (!c.slabH || !F.newton_stats) { err = "hipMalloc failed for the fault"; return 6; }
   }
   c.hW = nullptr;
-  upload_wave_cold(c);
+  if (upload_wave_cold(c)) { err = "hipMalloc/hipMemcpy failed for the stage-kernel tables"; return 6; }
   if (hipDeviceSynchronize() != hipSuccess) { err = "HIP error during setup upload"; return 6; }
   return 0;
 }

@@ -1,0 +1,575 @@
+// Wave stage kernel of the CGFDM time step for CDNA4 (gfx950): the
+// HBM-bound hot path.
+//
+// One fused kernel computes the velocity-stress RHS on the curvilinear grid
+// (wave_deriv_cu, wave.cu:8-494), adds the ADE-PML terms and advances their
+// auxiliary fields (abs_deriv_{x,y,z}_cu, pml{x,y,z}.cu), applies the RK4
+// stage combine (wave_rk_cu/RK_Syn, rk4.cu:20-33,154-257) and the Cerjan
+// sponge (apply_cerjan_cu, cerjan.cu:166-200), reading each wavefield word
+// once from HBM.
+//
+// Work decomposition: a block owns a TJ x TK tile of (j,k) columns and
+// streams a chunk of i planes (i is the fault-normal, slowest axis).  The
+// five planes the x-stencil needs sit in a per-thread register queue; the
+// current plane plus its y/z halo is staged in a double-buffered LDS tile,
+// so each plane costs one barrier and the next plane's loads are in flight
+// while the current plane is computed.  Blocks are numbered so that each
+// XCD (and its private L2) receives a contiguous range of work items of one
+// i-chunk, which keeps the k-halo rows of neighbouring tiles L2 hits.
+//
+// Addressing: every global access is a 32-bit byte offset from a
+// wave-uniform field base, so one VGPR addresses all 9 wave fields and the
+// metric fields of a point (the grid is limited to 2^30 points).
+//
+// Points of the fault slab (i0-3..i0+3 without i0) keep their RHS in a slab
+// buffer so the split-node kernels can overwrite it (fault_dvelo/
+// fault_dstrs) before fault_post advances the slab.
+#include "fd_ops.h"
+#include <algorithm>
+#include <vector>
+
+namespace cgfdm {
+using namespace fd;
+
+namespace {
+
+constexpr int TJ = 64;  // one wavefront per k-row of the tile, j contiguous
+constexpr int TK = 8;
+constexpr int NTHR = TJ * TK;
+constexpr int LJ = TJ + 4, LK = TK + 4;
+constexpr int NH = 4 * LJ + 4 * TK;  // halo cells of one plane: 4 rows + 4 side cells per row
+static_assert(NH <= NTHR, "one halo cell per thread");
+
+
+// Volume arrays are plane-interleaved: word (field c, padded plane x, column
+// jk = j + k*ny) sits at (x*nf + c)*nyz + jk, nf = 9 for wavefields, 13 for
+// the metric.  A plane of an array is addressed as one buffer resource
+// (base = the plane's first word, range = the plane's bytes) with the lane's
+// column byte offset as voffset and the field's offset c*nyz*size as
+// soffset: no per-lane 64-bit address arithmetic, a handful of scalar
+// registers per plane, and one block touches few memory pages per plane.
+// Word 3 of the resource is the gfx9-family default (32-bit data, no swizzle).
+constexpr int kBufWord3 = 0x00020000;
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+template <class T>
+__device__ __forceinline__ Rsrc plane_rsrc(const T* a, int plane, uint32_t nf, uint32_t nyz) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(a) + (size_t)plane * nf * nyz, (short)0,
+                                           (int)(nf * nyz * (uint32_t)sizeof(T)), kBufWord3);
+}
+__device__ __forceinline__ float bld(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t voff, uint32_t soff, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+}
+template <class MT>
+__device__ __forceinline__ float bldm(Rsrc r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(MT) == 2) {
+    const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+    return __uint_as_float(((unsigned)v) << 16);
+  } else {
+    return bld(r, voff, soff);
+  }
+}
+// Word of field c at padded plane x, column jk (plain loads, off the hot path).
+__device__ __forceinline__ float wat(const float* W, int c, int x, uint32_t jk, uint32_t nyz) {
+  return W[((size_t)x * 9 + c) * nyz + jk];
+}
+__device__ __forceinline__ float mat_(const float* M, int c, int x, uint32_t jk, uint32_t nyz) {
+  return M[((size_t)x * 13 + c) * nyz + jk];
+}
+__device__ __forceinline__ float mat_(const uint16_t* M, int c, int x, uint32_t jk, uint32_t nyz) {
+  return __uint_as_float(((unsigned)M[((size_t)x * 13 + c) * nyz + jk]) << 16);
+}
+
+// RK_Syn combine (rk4.cu:20-33) of field c of the planes rs (S) and rt (T).
+__device__ __forceinline__ float rk_combine_b(int irk, float rka, float rkb, float w, float h, Rsrc rs, Rsrc rt,
+                                              uint32_t voff, uint32_t soff) {
+  if (irk == 0) {
+    bst(rt, voff, soff, w + rkb * h);
+    return w + rka * h;
+  }
+  if (irk == 3) return bld(rt, voff, soff) + rkb * h;
+  bst(rt, voff, soff, bld(rt, voff, soff) + rkb * h);
+  return bld(rs, voff, soff) + rka * h;
+}
+
+template <class MT>
+struct WaveArgs {
+  const float* __restrict__ Win;
+  float* __restrict__ Wout;
+  const float* __restrict__ S;
+  float* __restrict__ T;
+  const MT* __restrict__ met;
+  float* __restrict__ slabH;
+  float* __restrict__ rhs;
+  const WaveCold* __restrict__ cold;
+  const int4* __restrict__ work;  // two int4 per block: (j0, k0, first plane, end plane), (j extent, k extent, -, -)
+  unsigned nyz;                   // words of one field of one plane (ny*nz)
+  int nwork, ni, nj, nk, ny, ii0;
+  float rDH, rka, rkb;
+  int irk, freenode, cerjan, fault, rhs_only, pml;
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block L runs on
+// XCD L % 8).  Give XCD x the contiguous work range [x*per, ...), so blocks
+// resident on one XCD at the same time are spatial neighbours.
+__device__ __forceinline__ int xcd_remap(int L, int n) {
+  const int per = (n + 7) >> 3;
+  const int x = L & 7, s = L >> 3;
+  const int full = n - 8 * (per - 1);  // XCDs that take 'per' items; the rest take per-1
+  return x < full ? x * per + s : full * per + (x - full) * (per - 1) + s;
+}
+
+// Traction-image hV for the top three layers (wave.cu:398-487): the
+// conservative traction vectors along each axis, zero at the surface and
+// odd-mirrored above it.  Direct (cached) loads: only 3 of nk layers.
+template <int FX, int FY, int FZ, class MT>
+__device__ void traction_image(const WaveArgs<MT>& a, int x, uint32_t col, int k, float rb1, float rb2, float rb3,
+                               float rrho, float* h) {
+  const float* W = a.Win;
+  const MT* M = a.met;
+  const uint32_t nyz = a.nyz;
+  const int ik = a.nk - 1 - k;
+  const float rrhojac = rrho / mat_(M, 9, x, col, nyz);
+  const int fl[3] = {FX, FY, FZ};
+  float d[3][3];  // d[axis][component]
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++) {
+    float v[3][7];
+#pragma unroll
+    for (int l = 0; l < 7; l++) {
+      const int xq = ax == 0 ? x + l - 3 : x;
+      const uint32_t q = ax == 1 ? col + (l - 3) : ax == 2 ? col + (l - 3) * a.ny : col;
+      const float e0 = mat_(M, 3 * ax, xq, q, nyz), e1 = mat_(M, 3 * ax + 1, xq, q, nyz);
+      const float e2 = mat_(M, 3 * ax + 2, xq, q, nyz), J = mat_(M, 9, xq, q, nyz);
+      const float txx = wat(W, 3, xq, q, nyz), tyy = wat(W, 4, xq, q, nyz), tzz = wat(W, 5, xq, q, nyz);
+      const float txy = wat(W, 6, xq, q, nyz), txz = wat(W, 7, xq, q, nyz), tyz = wat(W, 8, xq, q, nyz);
+      v[0][l] = (e0 * txx + e1 * txy + e2 * txz) * J;
+      v[1][l] = (e0 * txy + e1 * tyy + e2 * tyz) * J;
+      v[2][l] = (e0 * txz + e1 * tyz + e2 * tzz) * J;
+    }
+    if (ax == 2) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (ik == 0) { v[c][3] = 0.0f; v[c][6] = -v[c][0]; v[c][5] = -v[c][1]; v[c][4] = -v[c][2]; }
+        else if (ik == 1) { v[c][4] = 0.0f; v[c][6] = -v[c][2]; v[c][5] = -v[c][3]; }
+        else { v[c][5] = 0.0f; v[c][6] = -v[c][4]; }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      d[ax][c] = a.rDH * (fl[ax] > 0 ? drp5<1>(v[c][2], v[c][3], v[c][4], v[c][5], v[c][6])
+                                     : drp5<-1>(v[c][0], v[c][1], v[c][2], v[c][3], v[c][4]));
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) h[c] = (d[0][c] * rb1 + d[1][c] * rb2 + d[2][c] * rb3) * rrhojac;
+}
+
+// Extra auxiliary-field source of the x/y PML faces on the free surface
+// (pmlx.cu:185-210, pmly.cu): the z-derivative of velocity eliminated through
+// the traction-free condition, component c in 3..8.
+__device__ __forceinline__ float pml_fs_term(int c, const float* z1, float ztx, float zty, float ztz, float lam,
+                                             float l2m, float mu, float c0) {
+  if (c == 3) return c0 * (l2m * ztx * z1[0] + lam * zty * z1[1] + lam * ztz * z1[2]);
+  if (c == 4) return c0 * (lam * ztx * z1[0] + l2m * zty * z1[1] + lam * ztz * z1[2]);
+  if (c == 5) return c0 * (lam * ztx * z1[0] + lam * zty * z1[1] + l2m * ztz * z1[2]);
+  if (c == 6) return c0 * mu * (zty * z1[0] + ztx * z1[1]);
+  if (c == 7) return c0 * mu * (ztz * z1[0] + ztx * z1[2]);
+  if (c == 8) return c0 * mu * (ztz * z1[1] + zty * z1[2]);
+  return 0.0f;
+}
+
+// EDGE=false is the interior instantiation: no PML, free-surface or Cerjan
+// code, so it fits 128 VGPRs (two blocks per CU); the host gives it only
+// points where every absorbing profile is trivial and no free-surface row.
+template <int FX, int FY, int FZ, class MT, bool EDGE>
+__global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
+  constexpr int HJL = FY > 0 ? 1 : 3;    // halo width below j (the stencil reaches HJL back, 4-HJL forward)
+  constexpr int HKL = FZ > 0 ? 1 : 3;
+  constexpr int QLO = FX > 0 ? -1 : -3;  // queue holds planes i+QLO .. i+QLO+4
+  constexpr int CTR = -QLO;
+  __shared__ float lds[2][9][LK][LJ];
+
+  const int item = xcd_remap((int)blockIdx.x, a.nwork);
+  const int4 wk = a.work[2 * item], we = a.work[2 * item + 1];
+  const int j0 = wk.x, k0 = wk.y, ibeg = wk.z, iend = wk.w;
+  const int tj = threadIdx.x, tk = threadIdx.y, tid = tj + tk * TJ;
+  const int ni = a.ni, nj = a.nj, nk = a.nk, ny = a.ny;
+  const int j = j0 + tj, k = k0 + tk;
+  const bool act = tj < we.x && tk < we.y;
+  const bool pml = EDGE && a.pml, cerjan = EDGE && a.cerjan;
+  const int nxp = ni + 6;  // padded planes
+  const uint32_t nyz = a.nyz;
+  // Every lane loads (no predicated loads): lanes past the padded array read
+  // a clamped in-bounds column whose value no active point ever uses.
+  const uint32_t col = (uint32_t)(min(j, nj + 2) + 3) + (uint32_t)(min(k, nk + 2) + 3) * (uint32_t)ny;
+  const uint32_t col4 = col << 2;
+  const uint32_t mcol = col * (uint32_t)sizeof(MT);
+  const uint32_t pb = nyz * 4, mpb = nyz * (uint32_t)sizeof(MT);  // bytes of one field of one plane
+  const float* __restrict__ Win = a.Win;
+  const WaveCold* __restrict__ cold = a.cold;
+
+  // this thread's halo cell: 4 full rows of LJ cells, then 4 side cells per row
+  int hlk = 0, hlj = 0;
+  uint32_t hcol = col;
+  if (tid < NH) {
+    if (tid < 4 * LJ) {
+      const int r = tid / LJ;
+      hlk = r < HKL ? r : r + TK;
+      hlj = tid % LJ;
+    } else {
+      const int h2 = tid - 4 * LJ;
+      hlk = HKL + h2 / 4;
+      const int cc = h2 % 4;
+      hlj = cc < HJL ? cc : cc + TJ;
+    }
+    const int gj = min(max(j0 + hlj - HJL, -3), nj + 2), gk = min(max(k0 + hlk - HKL, -3), nk + 2);
+    hcol = (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * (uint32_t)ny;
+  }
+  const uint32_t hcol4 = hcol << 2;
+
+  float rb2 = 1.0f, rb3 = 1.0f, cjk = 1.0f;
+  if (act && pml) { rb2 = 1.0f / cold->pB[1][j]; rb3 = 1.0f / cold->pB[2][k]; }
+  if (act && cerjan) cjk = fminf(cold->cj[1][j + 3], cold->cj[2][k + 3]);
+
+  float q[5][9];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const Rsrc r = plane_rsrc(Win, ibeg + QLO + m + 3, 9, nyz);
+#pragma unroll
+    for (int c = 0; c < 9; c++) q[m][c] = bld(r, col4, c * pb);
+  }
+  float hv[9];
+  {
+    const Rsrc r = plane_rsrc(Win, ibeg + 3, 9, nyz);
+#pragma unroll
+    for (int c = 0; c < 9; c++) hv[c] = bld(r, hcol4, c * pb);
+  }
+#pragma unroll
+  for (int c = 0; c < 9; c++) lds[0][c][tk + HKL][tj + HJL] = q[CTR][c];
+  if (tid < NH) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) lds[0][c][hlk][hlj] = hv[c];
+  }
+  __syncthreads();
+
+  for (int i = ibeg; i < iend; ++i) {
+    const int b = (i - ibeg) & 1;
+    const bool more = i + 1 < iend;  // uniform per block
+    // next plane of the queue and the halo of plane i+1, in flight during the compute
+    // (planes clamped in-bounds: past the chunk's end the values go unused)
+    float nq[9];
+    {
+      const Rsrc r = plane_rsrc(Win, min(i + QLO + 8, nxp - 1), 9, nyz);
+#pragma unroll
+      for (int c = 0; c < 9; c++) nq[c] = bld(r, col4, c * pb);
+    }
+    {
+      const Rsrc r = plane_rsrc(Win, min(i + 4, nxp - 1), 9, nyz);
+#pragma unroll
+      for (int c = 0; c < 9; c++) hv[c] = bld(r, hcol4, c * pb);
+    }
+
+    if (act) {
+      const Rsrc rm = plane_rsrc(a.met, i + 3, 13, nyz);
+      auto MET = [&](int c) { return bldm<MT>(rm, mcol, c * mpb); };
+      const float xix = MET(0), xiy = MET(1), xiz = MET(2);
+      const float etx = MET(3), ety = MET(4), etz = MET(5);
+      const float ztx = MET(6), zty = MET(7), ztz = MET(8);
+      const float lam = MET(10), mu = MET(11);
+      const float rrho = 1.0f / MET(12);
+      const float l2m = lam + 2.0f * mu;
+      const float rDH = a.rDH;
+      const float rb1 = pml ? 1.0f / cold->pB[0][i] : 1.0f;
+      const float(*L)[LK][LJ] = lds[b];
+      const int lj = tj + HJL, lk = tk + HKL;
+      // one-sided DRP difference of field c along each axis (macdrp.h:91-95),
+      // without the 1/DH factor (folded into the metric rows below)
+      auto Dx = [&](int c) { return drp5<FX>(q[0][c], q[1][c], q[2][c], q[3][c], q[4][c]); };
+      auto Dy = [&](int c) {
+        return FY > 0 ? drp5<1>(L[c][lk][lj - 1], L[c][lk][lj], L[c][lk][lj + 1], L[c][lk][lj + 2], L[c][lk][lj + 3])
+                      : drp5<-1>(L[c][lk][lj - 3], L[c][lk][lj - 2], L[c][lk][lj - 1], L[c][lk][lj], L[c][lk][lj + 1]);
+      };
+      auto Dz = [&](int c) {
+        return FZ > 0 ? drp5<1>(L[c][lk - 1][lj], L[c][lk][lj], L[c][lk + 1][lj], L[c][lk + 2][lj], L[c][lk + 3][lj])
+                      : drp5<-1>(L[c][lk - 3][lj], L[c][lk - 2][lj], L[c][lk - 1][lj], L[c][lk][lj], L[c][lk + 1][lj]);
+      };
+      const bool top = EDGE && a.freenode && k >= nk - 3;  // uniform per wavefront (one k-row)
+      // RHS accumulated derivative by derivative (wave.cu:133-276): each
+      // field's derivative adds its momentum / Hooke terms, metric rows
+      // pre-scaled by the PML 1/beta of their axis.
+      float h[9];
+#pragma unroll
+      for (int c = 0; c < 9; c++) h[c] = 0.0f;
+      {
+        const float s1 = rb1 * rDH, s2 = rb2 * rDH, s3 = rb3 * rDH;
+        accum_axis(h, Dx, xix * s1, xiy * s1, xiz * s1, rrho, lam, l2m, mu, true);
+        accum_axis(h, Dy, etx * s2, ety * s2, etz * s2, rrho, lam, l2m, mu, true);
+        accum_axis(h, Dz, ztx * s3, zty * s3, ztz * s3, rrho, lam, l2m, mu, !top);
+      }
+      if (top) {
+        float dzv[3];
+        if (k == nk - 1) {  // DzV from the traction-free condition (wave.cu:207-232)
+          const uint32_t ij = ((uint32_t)(j + 3) + (uint32_t)(i + 3) * (uint32_t)ny) * 9;
+          const float* mVx = cold->mVx;
+          const float* mVy = cold->mVy;
+          const float dxv[3] = {rDH * Dx(0), rDH * Dx(1), rDH * Dx(2)}, dyv[3] = {rDH * Dy(0), rDH * Dy(1), rDH * Dy(2)};
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+            dzv[r] = mVx[ij + 3 * r] * dxv[0] + mVx[ij + 3 * r + 1] * dxv[1] + mVx[ij + 3 * r + 2] * dxv[2]
+                   + mVy[ij + 3 * r] * dyv[0] + mVy[ij + 3 * r + 1] * dyv[1] + mVy[ij + 3 * r + 2] * dyv[2];
+        } else if (k == nk - 2) {  // L22 (wave.cu:233-238)
+#pragma unroll
+          for (int c = 0; c < 3; c++) dzv[c] = rDH * (FZ > 0 ? L[c][lk + 1][lj] - L[c][lk][lj] : L[c][lk][lj] - L[c][lk - 1][lj]);
+        } else {  // L24 (wave.cu:239-245)
+#pragma unroll
+          for (int c = 0; c < 3; c++)
+            dzv[c] = rDH * (FZ > 0 ? -C24_1 * L[c][lk][lj] + C24_2 * L[c][lk + 1][lj] - C24_3 * L[c][lk + 2][lj]
+                                   : C24_3 * L[c][lk - 2][lj] - C24_2 * L[c][lk - 1][lj] + C24_1 * L[c][lk][lj]);
+        }
+        const float e0 = ztx * rb3, e1 = zty * rb3, e2 = ztz * rb3;
+        h[3] += (l2m * e0 * dzv[0] + lam * e1 * dzv[1] + lam * e2 * dzv[2]);
+        h[4] += (lam * e0 * dzv[0] + l2m * e1 * dzv[1] + lam * e2 * dzv[2]);
+        h[5] += (lam * e0 * dzv[0] + lam * e1 * dzv[1] + l2m * e2 * dzv[2]);
+        h[6] += (e1 * dzv[0] + e0 * dzv[1]) * mu;
+        h[7] += (e2 * dzv[0] + e0 * dzv[2]) * mu;
+        h[8] += (e2 * dzv[1] + e1 * dzv[2]) * mu;
+        traction_image<FX, FY, FZ, MT>(a, i + 3, col, k, rb1, rb2, rb3, rrho, h);
+      }
+
+      // ADE-PML (pmlx.cu:126-210 and the y/z variants): -aux/beta in the
+      // RHS and the auxiliary ODE from the plain derivative along the axis,
+      // one component at a time.
+      if (pml) {
+        const int N = cold->N;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+          const int li = ax == 0 ? i : ax == 1 ? j : k;
+          const int nn = ax == 0 ? ni : ax == 1 ? nj : nk;
+#pragma unroll
+          for (int side = 0; side < 2; side++) {
+            const int f = 2 * ax + side;
+            const int off = side ? nn - N : 0;
+            if (!cold->on[f] || li < off || li >= off + N) continue;
+            float dd[9];
+#pragma unroll
+            for (int c = 0; c < 9; c++) dd[c] = rDH * (ax == 0 ? Dx(c) : ax == 1 ? Dy(c) : Dz(c));
+            const float e0 = ax == 0 ? xix : ax == 1 ? etx : ztx;
+            const float e1 = ax == 0 ? xiy : ax == 1 ? ety : zty;
+            const float e2 = ax == 0 ? xiz : ax == 1 ? etz : ztz;
+            const uint32_t pi = ax == 0 ? i - off : i, pj = ax == 1 ? j - off : j, pk = ax == 2 ? k - off : k;
+            const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
+            const size_t ps = (size_t)pni * pnj * pnk;
+            const uint32_t pa4 = (pj + pk * pnj + pi * pnj * pnk) << 2;
+            const float b1 = cold->pB[ax][li], d1 = cold->pD[ax][li], ad1 = -(cold->pA[ax][li] + d1);
+            const float rbb = 1.0f / b1;
+            const float c0 = d1 * b1;
+            float* Y = cold->auxY[f];
+            float* SA = cold->auxS[f];
+            float* TA = cold->auxT[f];
+            const bool fsx = ax < 2 && a.freenode && k == nk - 1;
+            float z1[3] = {0.0f, 0.0f, 0.0f};
+            if (fsx) {
+              const float* mat = ax == 0 ? cold->mVx : cold->mVy;
+              const uint32_t ij = ((uint32_t)(j + 3) + (uint32_t)(i + 3) * (uint32_t)ny) * 9;
+#pragma unroll
+              for (int r = 0; r < 3; r++) z1[r] = mat[ij + 3 * r] * dd[0] + mat[ij + 3 * r + 1] * dd[1] + mat[ij + 3 * r + 2] * dd[2];
+            }
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+              const float y = ldo(Y + c * ps, pa4);
+              h[c] += -y * rbb;
+              if (!a.rhs_only) {
+                float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
+                if (fsx && c >= 3) ah += pml_fs_term(c, z1, ztx, zty, ztz, lam, l2m, mu, c0);
+                if (a.irk == 0) sto(SA + c * ps, pa4, y);
+                sto(Y + c * ps, pa4, rk_combine_o(a.irk, a.rka, a.rkb, y, ah, SA + c * ps, TA + c * ps, pa4));
+              }
+            }
+          }
+        }
+      }
+
+      if (a.rhs_only) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) bst(plane_rsrc(a.rhs, i + 3, 9, nyz), col4, c * pb, h[c]);
+      } else if (a.fault && i >= a.ii0 - 3 && i <= a.ii0 + 3 && i != a.ii0) {
+        const Rsrc r = plane_rsrc(a.slabH, i - (a.ii0 - 3), 9, nyz);  // slab planes share the layout
+#pragma unroll
+        for (int c = 0; c < 9; c++) bst(r, col4, c * pb, h[c]);
+      } else {
+        const float damp = cerjan ? fminf(cold->cj[0][i + 3], cjk) : 1.0f;
+        const Rsrc rs = plane_rsrc(a.S, i + 3, 9, nyz), rt = plane_rsrc(a.T, i + 3, 9, nyz);
+        const Rsrc ro = plane_rsrc(a.Wout, i + 3, 9, nyz);
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          const float w = rk_combine_b(a.irk, a.rka, a.rkb, q[CTR][c], h[c], rs, rt, col4, c * pb);
+          bst(ro, col4, c * pb, w * damp);
+        }
+      }
+    }
+
+    if (more) {  // stage plane i+1 into the other LDS buffer
+#pragma unroll
+      for (int c = 0; c < 9; c++) lds[b ^ 1][c][tk + HKL][tj + HJL] = q[CTR + 1][c];
+      if (tid < NH) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) lds[b ^ 1][c][hlk][hlj] = hv[c];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int c = 0; c < 9; c++) q[m][c] = q[m + 1][c];
+#pragma unroll
+    for (int c = 0; c < 9; c++) q[4][c] = nq[c];
+    __syncthreads();
+  }
+}
+
+template <int FX, int FY, int FZ, class MT>
+void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
+  if (c.wave_nwork_int > 0) {
+    a.work = (const int4*)c.wave_work;
+    a.nwork = c.wave_nwork_int;
+    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, false>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+  }
+  if (c.wave_nwork > c.wave_nwork_int) {
+    a.work = (const int4*)c.wave_work + 2 * c.wave_nwork_int;
+    a.nwork = c.wave_nwork - c.wave_nwork_int;
+    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, true>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+  }
+}
+
+template <class MT>
+void launch_wave_dispatch(const WaveArgs<MT>& a, const Ctx& c, int fx, int fy, int fz, hipStream_t s) {
+  int key = (fx > 0) * 4 + (fy > 0) * 2 + (fz > 0);
+  switch (key) {
+    case 0: launch_wave_t<-1, -1, -1, MT>(a, c, s); break;
+    case 1: launch_wave_t<-1, -1, 1, MT>(a, c, s); break;
+    case 2: launch_wave_t<-1, 1, -1, MT>(a, c, s); break;
+    case 3: launch_wave_t<-1, 1, 1, MT>(a, c, s); break;
+    case 4: launch_wave_t<1, -1, -1, MT>(a, c, s); break;
+    case 5: launch_wave_t<1, -1, 1, MT>(a, c, s); break;
+    case 6: launch_wave_t<1, 1, -1, MT>(a, c, s); break;
+    default: launch_wave_t<1, 1, 1, MT>(a, c, s); break;
+  }
+}
+
+template <class MT>
+WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float* rhs_out) {
+  WaveArgs<MT> a{};
+  a.Win = b.Win; a.Wout = b.Wout; a.S = b.S; a.T = b.T;
+  a.met = (const MT*)c.met;
+  a.slabH = c.slabH;
+  a.rhs = rhs_out;
+  a.cold = (const WaveCold*)c.wave_cold;
+  bool any = false;
+  for (int f = 0; f < 6; f++) any |= c.pml.on[f] != 0;
+  a.pml = any ? 1 : 0;
+  a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.ny = c.d.ny; a.ii0 = c.d.i0 - 3;
+  a.nyz = (unsigned)c.d.nyz;
+  a.rDH = (float)(1.0 / c.p.DH);
+  const double RA[4] = {0.5, 0.5, 1.0, 1e10}, RB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  a.rka = (float)(RA[irk] * c.p.DT);
+  a.rkb = (float)(RB[irk] * c.p.DT);
+  a.irk = irk;
+  a.freenode = c.p.FreeSurface ? 1 : 0;
+  a.cerjan = c.p.useCerjan ? 1 : 0;
+  a.fault = c.p.Rupture ? 1 : 0;
+  a.rhs_only = rhs_only ? 1 : 0;
+  return a;
+}
+
+// Tiles [lo, hi) of one axis: the edge band below, tiles of length t
+// starting at the interior's first index, the edge band above.  Each
+// segment is (start, length, interior?).
+struct Seg { int s, n; bool in; };
+std::vector<Seg> axis_segments(int n, int lo, int hi, int t) {
+  std::vector<Seg> v;
+  const int a = std::min(lo, n), b = std::max(a, n - hi);
+  for (int x = 0; x < a; x += t) v.push_back({x, std::min(t, a - x), false});
+  for (int x = a; x < b; x += t) v.push_back({x, std::min(t, b - x), true});
+  for (int x = b; x < n; x += t) v.push_back({x, std::min(t, n - x), false});
+  return v;
+}
+
+}  // namespace
+
+// Uploads the off-hot-path pointer table and builds the stage-kernel work
+// lists.  The interior box excludes every face band where a PML profile or
+// the Cerjan sponge is non-trivial and the three free-surface rows; its
+// tiles start at the box edge so that no interior tile straddles a band.
+// Interior items come first (chunks of about 2048 blocks in total, so 256
+// CUs stay balanced), then the edge items; each list is chunk-major so that
+// one XCD's contiguous share covers neighbouring tiles.
+int upload_wave_cold(Ctx& c) {
+  WaveCold w{};
+  for (int x = 0; x < 3; x++) { w.pA[x] = c.pml.A[x]; w.pB[x] = c.pml.B[x]; w.pD[x] = c.pml.D[x]; w.cj[x] = c.cj[x]; }
+  for (int f = 0; f < 6; f++) { w.auxY[f] = c.pml.Y[f]; w.auxS[f] = c.pml.S[f]; w.auxT[f] = c.pml.T[f]; w.on[f] = c.pml.on[f]; }
+  w.mVx = c.matVx2Vz; w.mVy = c.matVy2Vz;
+  w.N = c.pml.N;
+  if (!c.wave_cold) {
+    if (hipMalloc(&c.wave_cold, sizeof(WaveCold)) != hipSuccess) return 1;
+    c.allocations.push_back(c.wave_cold);
+  }
+  if (hipMemcpy(c.wave_cold, &w, sizeof w, hipMemcpyHostToDevice) != hipSuccess) return 1;
+
+  const Params& p = c.p;
+  const int n3[3] = {c.d.ni, c.d.nj, c.d.nk};
+  const bool freenode = p.FreeSurface != 0;
+  int lo[3], hi[3];
+  for (int ax = 0; ax < 3; ax++) {
+    lo[ax] = std::max(c.pml.on[2 * ax] ? c.pml.N : 0, p.useCerjan ? p.DAMP_N : 0);
+    hi[ax] = std::max(c.pml.on[2 * ax + 1] ? c.pml.N : 0, p.useCerjan && (ax < 2 || !freenode) ? p.DAMP_N : 0);
+  }
+  if (freenode) hi[2] = std::max(hi[2], 3);
+  const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
+  const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
+  int tiles_in = 0;
+  for (const Seg& b : sk) for (const Seg& a : sj) tiles_in += a.in && b.in;
+  const int ni = n3[0], ia = std::min(lo[0], ni), ib = std::max(ia, ni - hi[0]);
+  int ich = p.WaveIChunk;
+  if (ich <= 0) {
+    const int span = std::max(1, ib - ia), chunks = std::max(1, std::min(span, (2048 + std::max(tiles_in, 1) - 1) / std::max(tiles_in, 1)));
+    ich = std::max((span + chunks - 1) / chunks, std::min(8, span));
+  }
+  std::vector<int4> in, edge;
+  auto push = [](std::vector<int4>& v, const Seg& a, const Seg& b, int i0, int i1) {
+    v.push_back(make_int4(a.s, b.s, i0, i1));
+    v.push_back(make_int4(a.n, b.n, 0, 0));
+  };
+  // interior tiles: interior i-range in chunks to 'in', the i bands to 'edge'
+  for (int i0 = ia; i0 < ib; i0 += ich)
+    for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(in, a, b, i0, std::min(ib, i0 + ich));
+  for (int i0 = 0; i0 < ni; i0 += ich)
+    for (const Seg& b : sk)
+      for (const Seg& a : sj) {
+        const int i1 = std::min(ni, i0 + ich);
+        if (!(a.in && b.in)) { push(edge, a, b, i0, i1); continue; }
+        // interior tile: only the parts of [i0, i1) outside [ia, ib)
+        if (i0 < ia) push(edge, a, b, i0, std::min(i1, ia));
+        if (i1 > ib) push(edge, a, b, std::max(i0, ib), i1);
+      }
+  std::vector<int4> all(in);
+  all.insert(all.end(), edge.begin(), edge.end());
+  if (c.wave_work) { hipFree(c.wave_work); c.wave_work = nullptr; }
+  if (hipMalloc(&c.wave_work, all.size() * sizeof(int4)) != hipSuccess) return 1;
+  if (hipMemcpy(c.wave_work, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) return 1;
+  c.wave_nwork_int = (int)in.size() / 2;
+  c.wave_nwork = (int)all.size() / 2;
+  return 0;
+}
+
+void launch_wave_stage(Ctx& c, const StageBufs& b, int irk, int fx, int fy, int fz, bool rhs_only, float* rhs_out) {
+  if (c.p.MetricBF16) {
+    WaveArgs<uint16_t> a = make_args<uint16_t>(c, b, irk, rhs_only, rhs_out);
+    launch_wave_dispatch(a, c, fx, fy, fz, c.stream);
+  } else {
+    WaveArgs<float> a = make_args<float>(c, b, irk, rhs_only, rhs_out);
+    launch_wave_dispatch(a, c, fx, fy, fz, c.stream);
+  }
+}
+
+}  // namespace cgfdm
