@@ -58,7 +58,8 @@ struct FieldRef {
   void* ptr = nullptr;
   size_t count = 0;
   int dtype = 0;   // 0 f32, 1 f64, 2 i32, 3 bf16
-  int nf = 0;      // >0: plane-interleaved volume array with nf fields per plane on the device
+  int nf = 0;      // >0: device volume array, nf fields per plane, padded rows (count is the caller's size)
+  int planes = 0;  // planes of such an array
 };
 
 FieldRef field(Ctx& c, const std::string& n) {
@@ -68,10 +69,11 @@ FieldRef field(Ctx& c, const std::string& n) {
   auto f32 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 0; };
   auto f64 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 1; };
   auto i32 = [&](void* p, size_t cnt) { r.ptr = p; r.count = cnt; r.dtype = 2; };
-  if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; }
-  else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; }
-  else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; }
-  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; r.nf = 13; }
+  const int nx = c.d.nx;
+  if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; r.planes = nx; }
+  else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; r.planes = nx; }
+  else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; r.planes = nx; }
+  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; r.nf = 13; r.planes = nx; }
   else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "matVy2Vz") f32(c.matVy2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "Bx") f32(c.pml.B[0], c.d.ni);
@@ -103,7 +105,7 @@ FieldRef field(Ctx& c, const std::string& n) {
     else if (n == "hT11") f32(F.hT11, nyz);
     else if (n == "hT12") f32(F.hT12, nyz);
     else if (n == "hT13") f32(F.hT13, nyz);
-    else if (n == "slabH") f32(c.slabH, 7 * 9 * nyz);
+    else if (n == "slabH") { f32(c.slabH, 7 * 9 * nyz); r.nf = 9; r.planes = 7; }
     else if (n == "Tn") f32(F.Tn, n2);
     else if (n == "Ts1") f32(F.Ts1, n2);
     else if (n == "Ts2") f32(F.Ts2, n2);
@@ -348,24 +350,31 @@ int cgfdm_wave_rhs(cgfdm_ctx* h, int fx, int fy, int fz) {
   Ctx& c = h->c;
   if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
   if (!c.hW) {
-    if (hipMalloc(&c.hW, 9 * c.d.nxyz * sizeof(float)) != hipSuccess) return fail(CGFDM_EHIP, "hipMalloc hW");
-    launch_fill(c, c.hW, 9 * c.d.nxyz, 0.0f);
+    const long nw = (long)c.d.nx * 9 * c.d.pyz;
+    if (hipMalloc(&c.hW, nw * sizeof(float)) != hipSuccess) return fail(CGFDM_EHIP, "hipMalloc hW");
+    launch_fill(c, c.hW, nw, 0.0f);
   }
   StageBufs b{c.cur, nullptr, c.S, c.Tacc};
   launch_wave_stage(c, b, 0, fx, fy, fz, true, c.hW);
   return check_hip("wave_rhs launch");
 }
 
-// Between the caller's layout [field][plane][jk] and the device's
-// plane-interleaved [plane][field][jk]; to_dev selects the direction.
+// Between the caller's layout (the reference's: [field][plane][k][j], rows
+// of ny) and the device's plane-interleaved [plane][field][k][jpad + j] with
+// rows of py; to_dev selects the direction.  The slab buffer is the one
+// exception on the caller's side: it is [plane][field][k][j] there too.
+static size_t dev_words(const Ctx& c, const FieldRef& r) { return (size_t)r.planes * r.nf * c.d.pyz; }
 static void interleave_copy(const Ctx& c, const FieldRef& r, const char* from, char* to, bool to_dev) {
-  const size_t es = dsize(r.dtype), nyz = (size_t)c.d.nyz, nx = (size_t)c.d.nx, nf = (size_t)r.nf;
-  const size_t row = nyz * es;
-  for (size_t i = 0; i < nx; i++)
-    for (size_t f = 0; f < nf; f++) {
-      const size_t dev = (i * nf + f) * row, ref = (f * nx + i) * row;
-      std::memcpy(to + (to_dev ? dev : ref), from + (to_dev ? ref : dev), row);
-    }
+  const size_t es = dsize(r.dtype), nz = (size_t)c.d.nz, ny = (size_t)c.d.ny, py = (size_t)c.d.py;
+  const size_t np = (size_t)r.planes, nf = (size_t)r.nf, pyz = (size_t)c.d.pyz, nyz = ny * nz;
+  const bool slab = np != (size_t)c.d.nx;
+  for (size_t i = 0; i < np; i++)
+    for (size_t f = 0; f < nf; f++)
+      for (size_t k = 0; k < nz; k++) {
+        const size_t dev = ((i * nf + f) * pyz + k * py + c.d.jpad) * es;
+        const size_t ref = ((slab ? i * nf + f : f * np + i) * nyz + k * ny) * es;
+        std::memcpy(to + (to_dev ? dev : ref), from + (to_dev ? ref : dev), ny * es);
+      }
 }
 
 int cgfdm_field_info(cgfdm_ctx* h, const char* name, size_t* count, int* dtype) {
@@ -389,9 +398,8 @@ int cgfdm_get_field(cgfdm_ctx* h, const char* name, void* dst, size_t bytes) {
     if (hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
     return 0;
   }
-  // device [plane][field][jk] -> caller [field][plane][jk] (the reference's layout)
-  std::vector<char> tmp(bytes);
-  if (hipMemcpy(tmp.data(), r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  std::vector<char> tmp(dev_words(c, r) * dsize(r.dtype));
+  if (hipMemcpy(tmp.data(), r.ptr, tmp.size(), hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
   interleave_copy(c, r, tmp.data(), (char*)dst, false);
   return 0;
 }
@@ -412,9 +420,11 @@ int cgfdm_put_field(cgfdm_ctx* h, const char* name, const void* src, size_t byte
     if (hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
     return 0;
   }
-  std::vector<char> tmp(bytes);
+  // read-modify-write keeps the row padding of the device array as it is
+  std::vector<char> tmp(dev_words(c, r) * dsize(r.dtype));
+  if (hipMemcpy(tmp.data(), r.ptr, tmp.size(), hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
   interleave_copy(c, r, (const char*)src, tmp.data(), true);
-  if (hipMemcpy(r.ptr, tmp.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
+  if (hipMemcpy(r.ptr, tmp.data(), tmp.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
   return 0;
 }
 
@@ -535,7 +545,9 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
   NcOut* o = c.out;
   hipStreamSynchronize(c.stream);
   const int nj = c.d.nj, nk = c.d.nk, ni = c.d.ni, ny = c.d.ny;
-  const long nxyz = c.d.nxyz, nyz = c.d.nyz;
+  const long nxyz = c.d.nxyz, nyz = c.d.nyz, pyz = c.d.pyz;
+  const int py = c.d.py, jpad = c.d.jpad;
+  (void)nxyz; (void)nyz;
   if (c.p.Rupture) {
     long n2 = (long)nj * nk;
     std::vector<float> buf(n2);
@@ -573,20 +585,20 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
       if (axis == 2) {  // xy: rows of ny for each i
         line.resize(ny);
         for (int a = 0; a < d1n; a++) {
-          hipMemcpy(line.data(), c.cur + ((long)(a + 3) * 9 + comp) * nyz + (long)(idx + 3) * ny, ny * 4, hipMemcpyDeviceToHost);
+          hipMemcpy(line.data(), c.cur + ((long)(a + 3) * 9 + comp) * pyz + jpad + (long)(idx + 3) * py, ny * 4, hipMemcpyDeviceToHost);
           std::memcpy(&buf[(long)a * d2n], &line[3], d2n * 4);
         }
       } else if (axis == 1) {  // xz
-        std::vector<float> plane(nyz);
+        std::vector<float> plane(pyz);
         for (int a = 0; a < d1n; a++) {
-          hipMemcpy(plane.data(), c.cur + ((long)(a + 3) * 9 + comp) * nyz, nyz * 4, hipMemcpyDeviceToHost);
-          for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[(idx + 3) + (long)(b + 3) * ny];
+          hipMemcpy(plane.data(), c.cur + ((long)(a + 3) * 9 + comp) * pyz, pyz * 4, hipMemcpyDeviceToHost);
+          for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[jpad + (idx + 3) + (long)(b + 3) * py];
         }
       } else {  // yz plane at i
-        std::vector<float> plane(nyz);
-        hipMemcpy(plane.data(), c.cur + ((long)(idx + 3) * 9 + comp) * nyz, nyz * 4, hipMemcpyDeviceToHost);
+        std::vector<float> plane(pyz);
+        hipMemcpy(plane.data(), c.cur + ((long)(idx + 3) * 9 + comp) * pyz, pyz * 4, hipMemcpyDeviceToHost);
         for (int a = 0; a < d1n; a++)
-          for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[(b + 3) + (long)(a + 3) * ny];
+          for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[jpad + (b + 3) + (long)(a + 3) * py];
       }
       f.put_record(vids[comp], rec, buf.data());
     }

@@ -56,7 +56,18 @@ int parse_params_json(const std::string& text, Params& p, std::string& err);
 struct Dims {
   int ni, nj, nk, nx, ny, nz, i0;
   long nxyz, nyz;
+  // Device volume arrays (wavefields, metric, fault slab) are plane-
+  // interleaved with a padded row: word (field c, padded plane i, padded
+  // row k, padded column j) of an array with nf fields per plane sits at
+  // (i*nf + c)*pyz + k*py + jpad + j.  py is a multiple of 32 words and jpad
+  // puts the first interior stencil tile on a 128-byte line.
+  int py, jpad;
+  long pyz;
 };
+
+// First column of the interior box of the stage kernel (the absorbing band
+// below it is handled by the edge instantiation).
+int interior_lo_j(const Params& p);
 
 // Fault state on the device (2-D arrays [nk][nj] unless noted).
 struct FaultDev {
@@ -111,7 +122,7 @@ struct Ctx {
   float *cj[3] = {nullptr, nullptr, nullptr};      // Cerjan 1-D profiles (nx, ny, nz incl. ghosts)
   void* wave_cold = nullptr;    // device copy of the off-hot-path pointer table
   void* wave_work = nullptr;    // stage-kernel work list (two int4 per block)
-  int wave_nwork = 0, wave_nwork_int = 0;  // all items; leading interior items
+  int wave_nwork = 0, wave_nwork_int = 0, wave_nwork_edge = 0;  // all items; interior, band items (the rest: free-surface tiles)
   PmlDev pml{};
   FaultDev f{};
   std::vector<void*> allocations;

@@ -29,8 +29,10 @@ struct FArgs {
   const float* Win;              // stage input for irk 0 mW
   const float* cj[3];
   int ni, nj, nk, nx, ny, nz, i0;
-  long nyz;        // words of one field of one plane (ny*nz)
-  long wps, mps;   // plane strides of the plane-interleaved wave (9 nyz) and metric (13 nyz) arrays
+  long nyz;        // words of a 2-D fault array (ny*nz)
+  long pyz;        // words of one field of one plane of a volume array (py*nz, padded rows)
+  long wps, mps;   // plane strides of the wave (9 pyz) and metric (13 pyz) volume arrays
+  int py, jpad;    // row pitch and front pad of volume rows
   int NY, NZ;
   float DH, rDH, DT;
   double DTd;
@@ -55,11 +57,13 @@ __device__ float Gt_func(float t, float T) {  // device.cu:134-143
 }
 __device__ float Fr_func(float r, float R) { return r < R ? expf(r * r / (r * r - R * R)) : 0.0f; }
 
-// Volume arrays are plane-interleaved: field c of padded plane i at column
-// jk sits at i*wps + c*nyz + jk (wavefields) or i*mps + c*nyz + jk (metric).
-// Conservative traction J*(grad xi . sigma) at plane i, column jk along metric row 'row'.
+// Volume arrays are plane-interleaved with padded rows: field c of padded
+// plane i at volume column jk = jpad + j + k*py sits at i*wps + c*pyz + jk
+// (wavefields) or i*mps + c*pyz + jk (metric).  2-D fault arrays keep the
+// reference's [nz][ny] rows and are indexed with pf = j + k*ny.
+// Conservative traction J*(grad xi . sigma) at plane i, volume column jk along metric row 'row'.
 __device__ __forceinline__ void trac(const FArgs& a, const float* W, int i, long jk, int row, float* o) {
-  const long nyz = a.nyz, pm = jk + i * a.mps, pw = jk + i * a.wps;
+  const long nyz = a.pyz, pm = jk + i * a.mps, pw = jk + i * a.wps;
   float e0 = LM(a, (3 * row) * nyz + pm), e1 = LM(a, (3 * row + 1) * nyz + pm), e2 = LM(a, (3 * row + 2) * nyz + pm);
   float J = LM(a, 9 * nyz + pm);
   float txx = W[3 * nyz + pw], tyy = W[4 * nyz + pw], tzz = W[5 * nyz + pw];
@@ -125,7 +129,8 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
   long q = j + (long)k * a.nj;
   if (F.united[q]) return;
   const int j1 = j + 3, k1 = k + 3, ny = a.ny;
-  const long nyz = a.nyz, wps = a.wps, mps = a.mps, pf = j1 + (long)k1 * ny, pv = pf * 3;
+  const long nyz = a.nyz, pyz = a.pyz, wps = a.wps, mps = a.mps, pf = j1 + (long)k1 * ny, pv = pf * 3;
+  const long pc = a.jpad + j1 + (long)k1 * a.py;  // volume column
   const int i0 = a.i0;
   const float DH = a.DH, rDH = a.rDH, DT = a.DT, DH2 = DH * DH;
   const double RA[4] = {0.5, 0.5, 1.0, 1e10}, RB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
@@ -175,9 +180,9 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     float jac = 1.f;
     for (int l = 1; l <= 3; l++) {
       const int ii = i0 + (2 * m - 1) * l;
-      jac = LM(a, 9 * nyz + pf + ii * mps);
+      jac = LM(a, 9 * pyz + pc + ii * mps);
       float o[3];
-      trac(a, a.W, ii, pf, 0, o);
+      trac(a, a.W, ii, pc, 0, o);
       long pl = (3 + (2 * m - 1) * l) * nyz + pf;
       F.T11[pl] = o[0]; F.T12[pl] = o[1]; F.T13[pl] = o[2];
     }
@@ -204,10 +209,10 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     jac_l = jac; rho_l = rho;
   }
   float t = irk == 0 ? a.it * DT : (irk == 3 ? (a.it + 1) * DT : (a.it + 0.5f) * DT);
-  long pm = pf + i0 * mps;
-  float n[3] = {LM(a, pm), LM(a, nyz + pm), LM(a, 2 * nyz + pm)};
+  long pm = pc + i0 * mps;
+  float n[3] = {LM(a, pm), LM(a, pyz + pm), LM(a, 2 * pyz + pm)};
   float n0 = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  float jacvec = LM(a, 9 * nyz + pm) * n0;
+  float jacvec = LM(a, 9 * pyz + pm) * n0;
   for (int c = 0; c < 3; c++) n[c] /= n0;
   float dV[3];
   for (int c = 0; c < 3; c++) dV[c] = F.mW[c * 2 * nyz + nyz + pf] - F.mW[c * 2 * nyz + pf];
@@ -333,11 +338,11 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
 
   // ---- fault2wave into the stage input at i0
   {
-    const long p = pf + i0 * mps, pw = pf + i0 * wps;
-    float mt[3][3] = {{LM(a, p), LM(a, 3 * nyz + p), LM(a, 6 * nyz + p)},
-                      {LM(a, nyz + p), LM(a, 4 * nyz + p), LM(a, 7 * nyz + p)},
-                      {LM(a, 2 * nyz + p), LM(a, 5 * nyz + p), LM(a, 8 * nyz + p)}};
-    float rj = 1.0f / LM(a, 9 * nyz + p);
+    const long p = pc + i0 * mps, pw = pc + i0 * wps;
+    float mt[3][3] = {{LM(a, p), LM(a, 3 * pyz + p), LM(a, 6 * pyz + p)},
+                      {LM(a, pyz + p), LM(a, 4 * pyz + p), LM(a, 7 * pyz + p)},
+                      {LM(a, 2 * pyz + p), LM(a, 5 * pyz + p), LM(a, 8 * pyz + p)}};
+    float rj = 1.0f / LM(a, 9 * pyz + p);
     float inv[3][3];
     inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
     inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
@@ -360,13 +365,13 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
       for (int cc = 0; cc < 3; cc++)
         sg[r][cc] = (tr[r][0] * inv[0][cc] + tr[r][1] * inv[1][cc] + tr[r][2] * inv[2][cc]) * det * rj;
     float* W = a.Wi0;
-    for (int c = 0; c < 3; c++) W[c * nyz + pw] = (F.W[c * 2 * nyz + pf] + F.W[c * 2 * nyz + nyz + pf]) * 0.5f;
-    W[3 * nyz + pw] = sg[0][0];
-    W[4 * nyz + pw] = sg[1][1];
-    W[5 * nyz + pw] = sg[2][2];
-    W[6 * nyz + pw] = (sg[0][1] + sg[1][0]) * 0.5f;
-    W[7 * nyz + pw] = (sg[0][2] + sg[2][0]) * 0.5f;
-    W[8 * nyz + pw] = (sg[1][2] + sg[2][1]) * 0.5f;
+    for (int c = 0; c < 3; c++) W[c * pyz + pw] = (F.W[c * 2 * nyz + pf] + F.W[c * 2 * nyz + nyz + pf]) * 0.5f;
+    W[3 * pyz + pw] = sg[0][0];
+    W[4 * pyz + pw] = sg[1][1];
+    W[5 * pyz + pw] = sg[2][2];
+    W[6 * pyz + pw] = (sg[0][1] + sg[1][0]) * 0.5f;
+    W[7 * pyz + pw] = (sg[0][2] + sg[2][0]) * 0.5f;
+    W[8 * pyz + pw] = (sg[1][2] + sg[2][1]) * 0.5f;
   }
 }
 
@@ -384,13 +389,15 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   long q = j + (long)k * a.nj;
   if (F.united[q]) return;
   const int j1 = j + 3, k1 = k + 3, ny = a.ny, i0 = a.i0;
-  const long nyz = a.nyz, wps = a.wps, mps = a.mps, pf = j1 + (long)k1 * ny;
+  const long nyz = a.nyz, pyz = a.pyz, wps = a.wps, mps = a.mps, pf = j1 + (long)k1 * ny;
+  const long pc = a.jpad + j1 + (long)k1 * a.py;  // volume column
+  const int py = a.py;
   const float rDH = a.rDH;
   const int km = a.nk - k;
   const int FX = a.FX, FY = a.FY, FZ = a.FZ;
   const float* W = a.W;
   float* slab = a.slabH;
-  auto slabx = [&](int i, int c) -> float& { return slab[(long)(i - (i0 - 3)) * 9 * nyz + c * nyz + pf]; };
+  auto slabx = [&](int i, int c) -> float& { return slab[(long)(i - (i0 - 3)) * 9 * pyz + c * pyz + pc]; };
 
   // ---- velocity rates at i0-3..-1, +1..+3 (fault_dvelo.cu:153-255)
   for (int i = i0 - 3; i <= i0 + 3; i++) {
@@ -399,11 +406,11 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
     float v1[3][7], v2[3][7], v3[3][7];
     for (int l = -3; l <= 3; l++) {
       float o[3];
-      trac(a, W, i + l, pf, 0, o);
+      trac(a, W, i + l, pc, 0, o);
       for (int c = 0; c < 3; c++) v1[c][l + 3] = o[c];
-      trac(a, W, i, pf + l, 1, o);
+      trac(a, W, i, pc + l, 1, o);
       for (int c = 0; c < 3; c++) v2[c][l + 3] = o[c];
-      trac(a, W, i, pf + (long)l * ny, 2, o);
+      trac(a, W, i, pc + (long)l * py, 2, o);
       for (int c = 0; c < 3; c++) v3[c][l + 3] = o[c];
     }
     v1[0][nn + 3] = F.T11[3 * nyz + pf];
@@ -421,8 +428,8 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
         v3[c][km + 2] = 0.0f;
         for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
       }
-    const long p = pf + i * mps;
-    float rrj = 1.0f / (LM(a, 12 * nyz + p) * LM(a, 9 * nyz + p));
+    const long p = pc + i * mps;
+    float rrj = 1.0f / (LM(a, 12 * pyz + p) * LM(a, 9 * pyz + p));
     for (int c = 0; c < 3; c++) {
       float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
       slabx(i, c) = (dxt + dyt + dzt) * rrj;
@@ -431,7 +438,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   // ---- split-node velocity rates (fault_dvelo.cu:260-385)
   for (int m = 0; m < 2; m++) {
     const float* TT[3] = {F.T11, F.T12, F.T13};
-    float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LM(a, 9 * nyz + pf + i0 * mps));
+    float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LM(a, 9 * pyz + pc + i0 * mps));
     for (int c = 0; c < 3; c++) {
       const float* T = TT[c];
       float dxt = m == 0 ? rDH * (A0M * T[3 * nyz + pf] - A1 * T[2 * nyz + pf] - A2 * T[1 * nyz + pf] - A3 * T[0 * nyz + pf])
@@ -455,7 +462,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   }
 
   // ---- stress rates (fault_dstrs_f/b)
-  auto wv = [&](int c, int i) -> const float* { return &W[c * nyz + pf + i * wps]; };
+  auto wv = [&](int c, int i) -> const float* { return &W[c * pyz + pc + i * wps]; };
   auto fv = [&](int c, int m) -> const float* { return &F.W[c * 2 * nyz + m * nyz + pf]; };
   float DxV[8][3], DyV[8][3], DzV[8][3];
   for (int c = 0; c < 3; c++) {
@@ -534,23 +541,23 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
     int l = nn < 4 ? nn - 3 : nn - 4;
     if (l == 0) continue;
     int i = i0 + l;
-    const long p = pf + i * mps, pw = pf + i * wps;
+    const long p = pc + i * mps, pw = pc + i * wps;
     float dy[3], dz[3];
     const float* dx = DxV[nn];
-    for (int c = 0; c < 3; c++) dy[c] = drp(&W[c * nyz + pw], 1, FY) * rDH;
+    for (int c = 0; c < 3; c++) dy[c] = drp(&W[c * pyz + pw], 1, FY) * rDH;
     long ij = (j1 + (long)i * ny) * 9;
     for (int c = 0; c < 3; c++) {
       if (a.freenode && km == 1)
         dz[c] = a.mVx[ij + 3 * c] * dx[0] + a.mVx[ij + 3 * c + 1] * dx[1] + a.mVx[ij + 3 * c + 2] * dx[2]
               + a.mVy[ij + 3 * c] * dy[0] + a.mVy[ij + 3 * c + 1] * dy[1] + a.mVy[ij + 3 * c + 2] * dy[2];
-      else if (a.freenode && km == 2) dz[c] = d22(&W[c * nyz + pw], ny, FZ) * rDH;
-      else if (a.freenode && km == 3) dz[c] = d24(&W[c * nyz + pw], ny, FZ) * rDH;
-      else dz[c] = drp(&W[c * nyz + pw], ny, FZ) * rDH;
+      else if (a.freenode && km == 2) dz[c] = d22(&W[c * pyz + pw], py, FZ) * rDH;
+      else if (a.freenode && km == 3) dz[c] = d24(&W[c * pyz + pw], py, FZ) * rDH;
+      else dz[c] = drp(&W[c * pyz + pw], py, FZ) * rDH;
     }
-    float lam = LM(a, 10 * nyz + p), mu = LM(a, 11 * nyz + p), l2m = lam + 2.0f * mu;
-    float xi[3] = {LM(a, p), LM(a, nyz + p), LM(a, 2 * nyz + p)};
-    float et[3] = {LM(a, 3 * nyz + p), LM(a, 4 * nyz + p), LM(a, 5 * nyz + p)};
-    float zt[3] = {LM(a, 6 * nyz + p), LM(a, 7 * nyz + p), LM(a, 8 * nyz + p)};
+    float lam = LM(a, 10 * pyz + p), mu = LM(a, 11 * pyz + p), l2m = lam + 2.0f * mu;
+    float xi[3] = {LM(a, p), LM(a, pyz + p), LM(a, 2 * pyz + p)};
+    float et[3] = {LM(a, 3 * pyz + p), LM(a, 4 * pyz + p), LM(a, 5 * pyz + p)};
+    float zt[3] = {LM(a, 6 * pyz + p), LM(a, 7 * pyz + p), LM(a, 8 * pyz + p)};
     slabx(i, 3) = l2m * xi[0] * dx[0] + lam * xi[1] * dx[1] + lam * xi[2] * dx[2] + l2m * et[0] * dy[0] + lam * et[1] * dy[1] +
                   lam * et[2] * dy[2] + l2m * zt[0] * dz[0] + lam * zt[1] * dz[1] + lam * zt[2] * dz[2];
     slabx(i, 4) = lam * xi[0] * dx[0] + l2m * xi[1] * dx[1] + lam * xi[2] * dx[2] + lam * et[0] * dy[0] + l2m * et[1] * dy[1] +
@@ -572,7 +579,8 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   const FaultDev& F = a.F;
   long q = j + (long)k * a.nj;
   const int j1 = j + 3, k1 = k + 3, ny = a.ny, i0 = a.i0;
-  const long nyz = a.nyz, wps = a.wps, mps = a.mps, pf = j1 + (long)k1 * ny, pv = pf * 3;
+  const long nyz = a.nyz, pyz = a.pyz, wps = a.wps, mps = a.mps, pf = j1 + (long)k1 * ny, pv = pf * 3;
+  const long pc = a.jpad + j1 + (long)k1 * a.py;  // volume column
   const double RA[4] = {0.5, 0.5, 1.0, 1e10}, RB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
   const int irk = a.irk;
   const float rka = (float)(RA[irk] * a.DTd), rkb = (float)(RB[irk] * a.DTd);
@@ -581,11 +589,11 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   for (int pl = 0; pl < 7; pl++) {
     if (pl == 3) continue;
     int ip = i0 - 3 + pl;
-    const long p = pf + ip * wps;
+    const long p = pc + ip * wps;
     float damp = a.cerjan ? fminf(a.cj[0][ip], dampj) : 1.0f;
     for (int c = 0; c < 9; c++) {
-      long x = c * nyz + p;
-      float h = a.slabH[(long)pl * 9 * nyz + c * nyz + pf];
+      long x = c * pyz + p;
+      float h = a.slabH[(long)pl * 9 * pyz + c * pyz + pc];
       float w = rk_combine(irk, rka, rkb, a.Win[x], h, a.S, a.T, x);
       a.Wi0[x] = w * damp;
     }
@@ -618,11 +626,11 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   else { F.Ts1[q] += rkb1 * F.tTs1[q]; F.Ts2[q] += rkb1 * F.tTs2[q]; F.Tn[q] += rkb1 * F.tTn[q]; }
   if (uni) return;
   // fault2wave into the stage output at i0
-  const long p = pf + i0 * mps, pw = pf + i0 * wps;
-  float mt[3][3] = {{LM(a, p), LM(a, 3 * nyz + p), LM(a, 6 * nyz + p)},
-                    {LM(a, nyz + p), LM(a, 4 * nyz + p), LM(a, 7 * nyz + p)},
-                    {LM(a, 2 * nyz + p), LM(a, 5 * nyz + p), LM(a, 8 * nyz + p)}};
-  float rj = 1.0f / LM(a, 9 * nyz + p);
+  const long p = pc + i0 * mps, pw = pc + i0 * wps;
+  float mt[3][3] = {{LM(a, p), LM(a, 3 * pyz + p), LM(a, 6 * pyz + p)},
+                    {LM(a, pyz + p), LM(a, 4 * pyz + p), LM(a, 7 * pyz + p)},
+                    {LM(a, 2 * pyz + p), LM(a, 5 * pyz + p), LM(a, 8 * pyz + p)}};
+  float rj = 1.0f / LM(a, 9 * pyz + p);
   float inv[3][3];
   inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
   inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
@@ -645,13 +653,13 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
     for (int cc = 0; cc < 3; cc++) sg[r][cc] = (tr[r][0] * inv[0][cc] + tr[r][1] * inv[1][cc] + tr[r][2] * inv[2][cc]) * det * rj;
   float damp = a.cerjan ? fminf(a.cj[0][i0], dampj) : 1.0f;
   float* W = a.Wi0;
-  for (int c = 0; c < 3; c++) W[c * nyz + pw] = (F.W[c * 2 * nyz + pf] + F.W[c * 2 * nyz + nyz + pf]) * 0.5f * damp;
-  W[3 * nyz + pw] = sg[0][0] * damp;
-  W[4 * nyz + pw] = sg[1][1] * damp;
-  W[5 * nyz + pw] = sg[2][2] * damp;
-  W[6 * nyz + pw] = (sg[0][1] + sg[1][0]) * 0.5f * damp;
-  W[7 * nyz + pw] = (sg[0][2] + sg[2][0]) * 0.5f * damp;
-  W[8 * nyz + pw] = (sg[1][2] + sg[2][1]) * 0.5f * damp;
+  for (int c = 0; c < 3; c++) W[c * pyz + pw] = (F.W[c * 2 * nyz + pf] + F.W[c * 2 * nyz + nyz + pf]) * 0.5f * damp;
+  W[3 * pyz + pw] = sg[0][0] * damp;
+  W[4 * pyz + pw] = sg[1][1] * damp;
+  W[5 * pyz + pw] = sg[2][2] * damp;
+  W[6 * pyz + pw] = (sg[0][1] + sg[1][0]) * 0.5f * damp;
+  W[7 * pyz + pw] = (sg[0][2] + sg[2][0]) * 0.5f * damp;
+  W[8 * pyz + pw] = (sg[1][2] + sg[2][1]) * 0.5f * damp;
 }
 
 // cal_rup_sensor_D_cu / cal_rup_sensor_magn_cu (rup_sensor.cu:7-86).
@@ -702,7 +710,8 @@ FArgs make_fargs(Ctx& c) {
   a.slabH = c.slabH;
   for (int x = 0; x < 3; x++) a.cj[x] = c.cj[x];
   a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.nx = c.d.nx; a.ny = c.d.ny; a.nz = c.d.nz; a.i0 = c.d.i0;
-  a.nyz = c.d.nyz; a.wps = 9 * c.d.nyz; a.mps = 13 * c.d.nyz;
+  a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = 13 * c.d.pyz;
+  a.py = c.d.py; a.jpad = c.d.jpad;
   a.NY = c.p.NY; a.NZ = c.p.NZ;
   a.DH = (float)c.p.DH; a.rDH = (float)(1.0 / c.p.DH); a.DT = (float)c.p.DT; a.DTd = c.p.DT;
   a.freenode = c.p.FreeSurface ? 1 : 0;

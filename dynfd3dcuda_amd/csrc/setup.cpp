@@ -105,8 +105,9 @@ int host_setup(Ctx& c, std::string& err) {
   d.nxyz = (long)d.nx * d.ny * d.nz;
   d.nyz = (long)d.ny * d.nz;
   d.i0 = d.nx / 2;
-  // the stage kernel addresses each field with 32-bit byte offsets
-  if (d.nxyz >= (1L << 30)) { err = "grid too large: (NX+6)(NY+6)(NZ+6) must stay below 2^30 points per GPU"; return 1; }
+  d.jpad = (32 - (interior_lo_j(p) + 3) % 32) % 32;
+  d.py = (d.jpad + d.ny + 31) / 32 * 32;
+  d.pyz = (long)d.py * d.nz;
   if (p.NX < 7 || p.NY < 7 || p.NZ < 7) { err = "NX, NY and NZ must be at least 7"; return 1; }
   const int nx = d.nx, ny = d.ny, nz = d.nz, ni = d.ni, nj = d.nj, nk = d.nk;
   const long nxyz = d.nxyz, nyz = d.nyz;
@@ -348,29 +349,34 @@ int host_setup(Ctx& c, std::string& err) {
   }
 
   // ---- device allocations: 4 wavefield buffers (step start S, stage A/B, accumulator T)
-  c.S = dmalloc<float>(c, 9 * nxyz);
-  c.A = dmalloc<float>(c, 9 * nxyz);
-  c.B = dmalloc<float>(c, 9 * nxyz);
-  c.Tacc = dmalloc<float>(c, 9 * nxyz);
+  const long pyz = d.pyz, nvol = (long)nx * 9 * pyz;  // device wavefield words (padded rows)
+  c.S = dmalloc<float>(c, nvol);
+  c.A = dmalloc<float>(c, nvol);
+  c.B = dmalloc<float>(c, nvol);
+  c.Tacc = dmalloc<float>(c, nvol);
   if (!c.S || !c.A || !c.B || !c.Tacc) { err = "hipMalloc failed for the wavefield"; return 6; }
   c.cur = c.S;
-  // device metric is plane-interleaved: field m of plane i at (i*13 + m)*nyz + jk
+  // device metric: plane-interleaved with padded rows (Dims)
+  const long nmet = (long)nx * 13 * pyz;
+  auto met_dev = [&](int i, int m, int k, int j) { return ((long)i * 13 + m) * pyz + (long)k * d.py + d.jpad + j; };
   if (p.MetricBF16) {
-    std::vector<uint16_t> h(13 * nxyz);
+    std::vector<uint16_t> h(nmet, 0);
 #pragma omp parallel for
     for (int i = 0; i < nx; i++)
       for (int m = 0; m < 13; m++)
-        for (long jk = 0; jk < nyz; jk++) h[((long)i * 13 + m) * nyz + jk] = to_bf16((float)Md[m * nxyz + i * nyz + jk]);
-    uint16_t* dm = dmalloc<uint16_t>(c, 13 * nxyz);
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++) h[met_dev(i, m, k, j)] = to_bf16((float)Md[m * nxyz + P(i, j, k)]);
+    uint16_t* dm = dmalloc<uint16_t>(c, nmet);
     if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
     c.met = dm;
   } else {
-    std::vector<float> h(13 * nxyz);
+    std::vector<float> h(nmet, 0.0f);
 #pragma omp parallel for
     for (int i = 0; i < nx; i++)
       for (int m = 0; m < 13; m++)
-        for (long jk = 0; jk < nyz; jk++) h[((long)i * 13 + m) * nyz + jk] = (float)Md[m * nxyz + i * nyz + jk];
-    float* dm = dmalloc<float>(c, 13 * nxyz);
+        for (int k = 0; k < nz; k++)
+          for (int j = 0; j < ny; j++) h[met_dev(i, m, k, j)] = (float)Md[m * nxyz + P(i, j, k)];
+    float* dm = dmalloc<float>(c, nmet);
     if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
     c.met = dm;
   }
@@ -585,7 +591,7 @@ int host_setup(Ctx& c, std::string& err) {
     F.newton_stats = dmalloc<unsigned long long>(c, 4);
 #undef UPF
 #undef UPI
-    c.slabH = dmalloc<float>(c, 7 * 9 * nyz);
+    c.slabH = dmalloc<float>(c, 7 * 9 * pyz);
     if (!c.slabH || !F.newton_stats) { err = "hipMalloc failed for the fault"; return 6; }
   }
   c.hW = nullptr;

@@ -41,9 +41,9 @@ constexpr int NH = 4 * LJ + 4 * TK;  // halo cells of one plane: 4 rows + 4 side
 static_assert(NH <= NTHR, "one halo cell per thread");
 
 
-// Volume arrays are plane-interleaved: word (field c, padded plane x, column
-// jk = j + k*ny) sits at (x*nf + c)*nyz + jk, nf = 9 for wavefields, 13 for
-// the metric.  A plane of an array is addressed as one buffer resource
+// Volume arrays are plane-interleaved with padded rows: word (field c,
+// padded plane x, column jk = jpad + j + k*py) sits at (x*nf + c)*pyz + jk,
+// nf = 9 for wavefields, 13 for the metric; py is a multiple of 32 words.  A plane of an array is addressed as one buffer resource
 // (base = the plane's first word, range = the plane's bytes) with the lane's
 // column byte offset as voffset and the field's offset c*nyz*size as
 // soffset: no per-lane 64-bit address arithmetic, a handful of scalar
@@ -105,7 +105,8 @@ struct WaveArgs {
   float* __restrict__ rhs;
   const WaveCold* __restrict__ cold;
   const int4* __restrict__ work;  // two int4 per block: (j0, k0, first plane, end plane), (j extent, k extent, -, -)
-  unsigned nyz;                   // words of one field of one plane (ny*nz)
+  unsigned nyz;                   // words of one field of one plane (py*nz, padded rows)
+  int py, jpad;                   // row pitch and front pad of volume rows
   int nwork, ni, nj, nk, ny, ii0;
   float rDH, rka, rkb;
   int irk, freenode, cerjan, fault, rhs_only, pml;
@@ -140,7 +141,7 @@ __device__ void traction_image(const WaveArgs<MT>& a, int x, uint32_t col, int k
 #pragma unroll
     for (int l = 0; l < 7; l++) {
       const int xq = ax == 0 ? x + l - 3 : x;
-      const uint32_t q = ax == 1 ? col + (l - 3) : ax == 2 ? col + (l - 3) * a.ny : col;
+      const uint32_t q = ax == 1 ? col + (l - 3) : ax == 2 ? col + (l - 3) * a.py : col;
       const float e0 = mat_(M, 3 * ax, xq, q, nyz), e1 = mat_(M, 3 * ax + 1, xq, q, nyz);
       const float e2 = mat_(M, 3 * ax + 2, xq, q, nyz), J = mat_(M, 9, xq, q, nyz);
       const float txx = wat(W, 3, xq, q, nyz), tyy = wat(W, 4, xq, q, nyz), tzz = wat(W, 5, xq, q, nyz);
@@ -180,10 +181,11 @@ __device__ __forceinline__ float pml_fs_term(int c, const float* z1, float ztx, 
   return 0.0f;
 }
 
-// EDGE=false is the interior instantiation: no PML, free-surface or Cerjan
-// code, so it fits 128 VGPRs (two blocks per CU); the host gives it only
-// points where every absorbing profile is trivial and no free-surface row.
-template <int FX, int FY, int FZ, class MT, bool EDGE>
+// KIND selects the instantiation: 0 interior (no PML, free-surface or Cerjan
+// code: the host gives it only points where every absorbing profile is
+// trivial), 1 absorbing bands (PML correction, Cerjan), 2 tiles holding the
+// free-surface rows (everything).
+template <int FX, int FY, int FZ, class MT, int KIND>
 __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
   constexpr int HJL = FY > 0 ? 1 : 3;    // halo width below j (the stencil reaches HJL back, 4-HJL forward)
   constexpr int HKL = FZ > 0 ? 1 : 3;
@@ -198,12 +200,13 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
   const int ni = a.ni, nj = a.nj, nk = a.nk, ny = a.ny;
   const int j = j0 + tj, k = k0 + tk;
   const bool act = tj < we.x && tk < we.y;
-  const bool pml = EDGE && a.pml, cerjan = EDGE && a.cerjan;
+  const bool pml = KIND >= 1 && a.pml, cerjan = KIND >= 1 && a.cerjan;
   const int nxp = ni + 6;  // padded planes
   const uint32_t nyz = a.nyz;
   // Every lane loads (no predicated loads): lanes past the padded array read
   // a clamped in-bounds column whose value no active point ever uses.
-  const uint32_t col = (uint32_t)(min(j, nj + 2) + 3) + (uint32_t)(min(k, nk + 2) + 3) * (uint32_t)ny;
+  const uint32_t py = a.py, jpad = a.jpad;
+  const uint32_t col = jpad + (uint32_t)(min(j, nj + 2) + 3) + (uint32_t)(min(k, nk + 2) + 3) * py;
   const uint32_t col4 = col << 2;
   const uint32_t mcol = col * (uint32_t)sizeof(MT);
   const uint32_t pb = nyz * 4, mpb = nyz * (uint32_t)sizeof(MT);  // bytes of one field of one plane
@@ -225,7 +228,7 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
       hlj = cc < HJL ? cc : cc + TJ;
     }
     const int gj = min(max(j0 + hlj - HJL, -3), nj + 2), gk = min(max(k0 + hlk - HKL, -3), nk + 2);
-    hcol = (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * (uint32_t)ny;
+    hcol = jpad + (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * py;
   }
   const uint32_t hcol4 = hcol << 2;
 
@@ -295,7 +298,7 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
         return FZ > 0 ? drp5<1>(L[c][lk - 1][lj], L[c][lk][lj], L[c][lk + 1][lj], L[c][lk + 2][lj], L[c][lk + 3][lj])
                       : drp5<-1>(L[c][lk - 3][lj], L[c][lk - 2][lj], L[c][lk - 1][lj], L[c][lk][lj], L[c][lk + 1][lj]);
       };
-      const bool top = EDGE && a.freenode && k >= nk - 3;  // uniform per wavefront (one k-row)
+      const bool top = KIND == 2 && a.freenode && k >= nk - 3;  // uniform per wavefront (one k-row)
       // RHS accumulated derivative by derivative (wave.cu:133-276): each
       // field's derivative adds its momentum / Hooke terms, metric rows
       // pre-scaled by the PML 1/beta of their axis.
@@ -338,9 +341,10 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
         traction_image<FX, FY, FZ, MT>(a, i + 3, col, k, rb1, rb2, rb3, rrho, h);
       }
 
-      // ADE-PML (pmlx.cu:126-210 and the y/z variants): -aux/beta in the
-      // RHS and the auxiliary ODE from the plain derivative along the axis,
-      // one component at a time.
+      // ADE-PML (pmlx.cu:126-210 and the y/z variants): the RHS of a point
+      // inside the band of face f gets -aux_f/beta; the auxiliary fields
+      // themselves are advanced afterwards by pml_aux_kernel from the same
+      // stage input.
       if (pml) {
         const int N = cold->N;
 #pragma unroll
@@ -352,41 +356,14 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
             const int f = 2 * ax + side;
             const int off = side ? nn - N : 0;
             if (!cold->on[f] || li < off || li >= off + N) continue;
-            float dd[9];
-#pragma unroll
-            for (int c = 0; c < 9; c++) dd[c] = rDH * (ax == 0 ? Dx(c) : ax == 1 ? Dy(c) : Dz(c));
-            const float e0 = ax == 0 ? xix : ax == 1 ? etx : ztx;
-            const float e1 = ax == 0 ? xiy : ax == 1 ? ety : zty;
-            const float e2 = ax == 0 ? xiz : ax == 1 ? etz : ztz;
             const uint32_t pi = ax == 0 ? i - off : i, pj = ax == 1 ? j - off : j, pk = ax == 2 ? k - off : k;
             const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
             const size_t ps = (size_t)pni * pnj * pnk;
             const uint32_t pa4 = (pj + pk * pnj + pi * pnj * pnk) << 2;
-            const float b1 = cold->pB[ax][li], d1 = cold->pD[ax][li], ad1 = -(cold->pA[ax][li] + d1);
-            const float rbb = 1.0f / b1;
-            const float c0 = d1 * b1;
-            float* Y = cold->auxY[f];
-            float* SA = cold->auxS[f];
-            float* TA = cold->auxT[f];
-            const bool fsx = ax < 2 && a.freenode && k == nk - 1;
-            float z1[3] = {0.0f, 0.0f, 0.0f};
-            if (fsx) {
-              const float* mat = ax == 0 ? cold->mVx : cold->mVy;
-              const uint32_t ij = ((uint32_t)(j + 3) + (uint32_t)(i + 3) * (uint32_t)ny) * 9;
+            const float rbb = 1.0f / cold->pB[ax][li];
+            const float* Y = cold->auxY[f];
 #pragma unroll
-              for (int r = 0; r < 3; r++) z1[r] = mat[ij + 3 * r] * dd[0] + mat[ij + 3 * r + 1] * dd[1] + mat[ij + 3 * r + 2] * dd[2];
-            }
-#pragma unroll
-            for (int c = 0; c < 9; c++) {
-              const float y = ldo(Y + c * ps, pa4);
-              h[c] += -y * rbb;
-              if (!a.rhs_only) {
-                float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
-                if (fsx && c >= 3) ah += pml_fs_term(c, z1, ztx, zty, ztz, lam, l2m, mu, c0);
-                if (a.irk == 0) sto(SA + c * ps, pa4, y);
-                sto(Y + c * ps, pa4, rk_combine_o(a.irk, a.rka, a.rkb, y, ah, SA + c * ps, TA + c * ps, pa4));
-              }
-            }
+            for (int c = 0; c < 9; c++) h[c] += -ldo(Y + c * ps, pa4) * rbb;
           }
         }
       }
@@ -428,17 +405,112 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
   }
 }
 
+// ADE-PML auxiliary fields (pmlx.cu:126-210, pmly.cu, pmlz.cu): for every
+// point of the band of face f, the plain one-sided derivative along the
+// face's axis drives aux' = -(alpha + d/beta) aux + (d/beta) * RHS_axis, plus
+// the traction-free term on the free surface for the x/y faces, advanced
+// with the same RK4 stage combine as the wavefield.  Runs after the stage
+// kernel (which read the old aux values) on the unchanged stage input.  One
+// thread per band point, j fastest, so aux accesses are contiguous.
+template <class MT>
+struct PmlAuxArgs {
+  const float* __restrict__ Win;
+  const MT* __restrict__ met;
+  const WaveCold* __restrict__ cold;
+  int f, ni, nj, nk, ny, py, jpad;
+  unsigned pyz;
+  int flag;  // one-sided operator sign along the face's axis
+  float rDH, rka, rkb;
+  int irk, freenode;
+};
+
+template <class MT>
+__global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
+  const WaveCold* __restrict__ cold = a.cold;
+  const int f = a.f, ax = f >> 1, N = cold->N;
+  const int nn = ax == 0 ? a.ni : ax == 1 ? a.nj : a.nk;
+  const int off = (f & 1) ? nn - N : 0;
+  const int pnj = ax == 1 ? N : a.nj, pnk = ax == 2 ? N : a.nk, pni = ax == 0 ? N : a.ni;
+  const long ps = (long)pni * pnj * pnk;
+  const long idx = blockIdx.x * 256L + threadIdx.x;
+  if (idx >= ps) return;
+  const int pj = (int)(idx % pnj);
+  const long t = idx / pnj;
+  const int pk = (int)(t % pnk), pi = (int)(t / pnk);
+  const int i = pi + (ax == 0 ? off : 0), j = pj + (ax == 1 ? off : 0), k = pk + (ax == 2 ? off : 0);
+  const int li = ax == 0 ? i : ax == 1 ? j : k;
+  const unsigned pyz = a.pyz;
+  const long col = a.jpad + (j + 3) + (long)(k + 3) * a.py;
+  const int x = i + 3;
+  const float* W = a.Win;
+  // neighbour offset (words) along the axis, within one field's plane or across planes
+  const long st = ax == 0 ? 9L * pyz : ax == 1 ? 1L : (long)a.py;
+  const long base = (long)x * 9 * pyz + col;
+  float dd[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float* w = W + base + (long)c * pyz;
+    dd[c] = a.rDH * (a.flag > 0 ? drp5<1>(w[-st], w[0], w[st], w[2 * st], w[3 * st])
+                                : drp5<-1>(w[-3 * st], w[-2 * st], w[-st], w[0], w[st]));
+  }
+  const MT* M = a.met;
+  auto MET = [&](int m) { return mat_(M, m, x, (uint32_t)col, pyz); };
+  const float e0 = MET(3 * ax), e1 = MET(3 * ax + 1), e2 = MET(3 * ax + 2);
+  const float lam = MET(10), mu = MET(11), rrho = 1.0f / MET(12), l2m = lam + 2.0f * mu;
+  const float b1 = cold->pB[ax][li], d1 = cold->pD[ax][li], ad1 = -(cold->pA[ax][li] + d1);
+  const float c0 = d1 * b1;
+  const bool fsx = ax < 2 && a.freenode && k == a.nk - 1;
+  float z1[3] = {0.0f, 0.0f, 0.0f}, ztx = 0.0f, zty = 0.0f, ztz = 0.0f;
+  if (fsx) {
+    const float* mat = ax == 0 ? cold->mVx : cold->mVy;
+    const long ij = ((long)(j + 3) + (long)(i + 3) * a.ny) * 9;
+#pragma unroll
+    for (int r = 0; r < 3; r++) z1[r] = mat[ij + 3 * r] * dd[0] + mat[ij + 3 * r + 1] * dd[1] + mat[ij + 3 * r + 2] * dd[2];
+    ztx = MET(6); zty = MET(7); ztz = MET(8);
+  }
+  float* Y = cold->auxY[f];
+  float* S = cold->auxS[f];
+  float* T = cold->auxT[f];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const long q = c * ps + idx;
+    const float y = Y[q];
+    float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
+    if (fsx && c >= 3) ah += pml_fs_term(c, z1, ztx, zty, ztz, lam, l2m, mu, c0);
+    if (a.irk == 0) S[q] = y;
+    Y[q] = rk_combine(a.irk, a.rka, a.rkb, y, ah, S, T, q);
+  }
+}
+
 template <int FX, int FY, int FZ, class MT>
 void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
-  if (c.wave_nwork_int > 0) {
-    a.work = (const int4*)c.wave_work;
-    a.nwork = c.wave_nwork_int;
-    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, false>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+  const int n0 = c.wave_nwork_int, n1 = n0 + c.wave_nwork_edge, n2 = c.wave_nwork;
+  const int4* w = (const int4*)c.wave_work;
+  if (n0 > 0) {
+    a.work = w; a.nwork = n0;
+    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
   }
-  if (c.wave_nwork > c.wave_nwork_int) {
-    a.work = (const int4*)c.wave_work + 2 * c.wave_nwork_int;
-    a.nwork = c.wave_nwork - c.wave_nwork_int;
-    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, true>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+  if (n1 > n0) {
+    a.work = w + 2 * n0; a.nwork = n1 - n0;
+    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+  }
+  if (n2 > n1) {
+    a.work = w + 2 * n1; a.nwork = n2 - n1;
+    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+  }
+  if (a.pml && !a.rhs_only) {
+    PmlAuxArgs<MT> p{};
+    p.Win = a.Win; p.met = a.met; p.cold = a.cold;
+    p.ni = a.ni; p.nj = a.nj; p.nk = a.nk; p.ny = a.ny; p.py = a.py; p.jpad = a.jpad; p.pyz = a.nyz;
+    p.rDH = a.rDH; p.rka = a.rka; p.rkb = a.rkb; p.irk = a.irk; p.freenode = a.freenode;
+    const int flags[3] = {FX, FY, FZ};
+    for (int f = 0; f < 6; f++) {
+      if (!c.pml.on[f]) continue;
+      const int ax = f / 2;
+      const long n = (long)c.pml.N * (ax == 0 ? (long)a.nj * a.nk : ax == 1 ? (long)a.nk * a.ni : (long)a.ni * a.nj);
+      p.f = f; p.flag = flags[ax];
+      hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
+    }
   }
 }
 
@@ -469,7 +541,8 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
   for (int f = 0; f < 6; f++) any |= c.pml.on[f] != 0;
   a.pml = any ? 1 : 0;
   a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.ny = c.d.ny; a.ii0 = c.d.i0 - 3;
-  a.nyz = (unsigned)c.d.nyz;
+  a.nyz = (unsigned)c.d.pyz;
+  a.py = c.d.py; a.jpad = c.d.jpad;
   a.rDH = (float)(1.0 / c.p.DH);
   const double RA[4] = {0.5, 0.5, 1.0, 1e10}, RB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
   a.rka = (float)(RA[irk] * c.p.DT);
@@ -497,6 +570,10 @@ std::vector<Seg> axis_segments(int n, int lo, int hi, int t) {
 
 }  // namespace
 
+int interior_lo_j(const Params& p) {
+  return std::max(p.usePML ? p.PML_N : 0, p.useCerjan ? p.DAMP_N : 0);
+}
+
 // Uploads the off-hot-path pointer table and builds the stage-kernel work
 // lists.  The interior box excludes every face band where a PML profile or
 // the Cerjan sponge is non-trivial and the three free-surface rows; its
@@ -521,7 +598,7 @@ int upload_wave_cold(Ctx& c) {
   const bool freenode = p.FreeSurface != 0;
   int lo[3], hi[3];
   for (int ax = 0; ax < 3; ax++) {
-    lo[ax] = std::max(c.pml.on[2 * ax] ? c.pml.N : 0, p.useCerjan ? p.DAMP_N : 0);
+    lo[ax] = ax == 1 ? interior_lo_j(p) : std::max(c.pml.on[2 * ax] ? c.pml.N : 0, p.useCerjan ? p.DAMP_N : 0);
     hi[ax] = std::max(c.pml.on[2 * ax + 1] ? c.pml.N : 0, p.useCerjan && (ax < 2 || !freenode) ? p.DAMP_N : 0);
   }
   if (freenode) hi[2] = std::max(hi[2], 3);
@@ -535,29 +612,33 @@ int upload_wave_cold(Ctx& c) {
     const int span = std::max(1, ib - ia), chunks = std::max(1, std::min(span, (2048 + std::max(tiles_in, 1) - 1) / std::max(tiles_in, 1)));
     ich = std::max((span + chunks - 1) / chunks, std::min(8, span));
   }
-  std::vector<int4> in, edge;
-  auto push = [](std::vector<int4>& v, const Seg& a, const Seg& b, int i0, int i1) {
-    v.push_back(make_int4(a.s, b.s, i0, i1));
-    v.push_back(make_int4(a.n, b.n, 0, 0));
+  // kind 0 interior, 1 absorbing bands, 2 tiles holding free-surface rows
+  std::vector<int4> lists[3];
+  auto push = [&](int kind, const Seg& a, const Seg& b, int i0, int i1) {
+    lists[kind].push_back(make_int4(a.s, b.s, i0, i1));
+    lists[kind].push_back(make_int4(a.n, b.n, 0, 0));
   };
-  // interior tiles: interior i-range in chunks to 'in', the i bands to 'edge'
+  const int nk = n3[2];
+  auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
   for (int i0 = ia; i0 < ib; i0 += ich)
-    for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(in, a, b, i0, std::min(ib, i0 + ich));
+    for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich));
   for (int i0 = 0; i0 < ni; i0 += ich)
     for (const Seg& b : sk)
       for (const Seg& a : sj) {
         const int i1 = std::min(ni, i0 + ich);
-        if (!(a.in && b.in)) { push(edge, a, b, i0, i1); continue; }
+        if (is_top(b)) { push(2, a, b, i0, i1); continue; }
+        if (!(a.in && b.in)) { push(1, a, b, i0, i1); continue; }
         // interior tile: only the parts of [i0, i1) outside [ia, ib)
-        if (i0 < ia) push(edge, a, b, i0, std::min(i1, ia));
-        if (i1 > ib) push(edge, a, b, std::max(i0, ib), i1);
+        if (i0 < ia) push(1, a, b, i0, std::min(i1, ia));
+        if (i1 > ib) push(1, a, b, std::max(i0, ib), i1);
       }
-  std::vector<int4> all(in);
-  all.insert(all.end(), edge.begin(), edge.end());
+  std::vector<int4> all;
+  for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
   if (c.wave_work) { hipFree(c.wave_work); c.wave_work = nullptr; }
   if (hipMalloc(&c.wave_work, all.size() * sizeof(int4)) != hipSuccess) return 1;
   if (hipMemcpy(c.wave_work, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) return 1;
-  c.wave_nwork_int = (int)in.size() / 2;
+  c.wave_nwork_int = (int)lists[0].size() / 2;
+  c.wave_nwork_edge = (int)lists[1].size() / 2;
   c.wave_nwork = (int)all.size() / 2;
   return 0;
 }
