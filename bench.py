@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the CGFDM dynamic-rupture time step on one MI355X.
+
+One step = one full RK4 time step of the whole job on the GPU: rupture
+sensor, and per stage the fault trial (fault_pre), the fused wave stage
+(interior / absorbing-band / free-surface instantiations + PML auxiliary
+update), the split-node derivatives (fault_deriv) and the fault RK
+(fault_post).  Inputs are generated with the seeded synthetic generators of
+dynfd3dcuda_amd.configs and are resident in HBM before timing starts.
+
+Default workload: the curved-grid slip-weakening configuration, 400 x 300 x
+200 points at 50 m (topography, bent fault, 3-layer medium, 20-layer ADE-PML,
+FP32).  Prints one JSON line:
+  value          grid-point updates per second (pt-steps/s) of the whole step
+  roofline       the dominant stage kernel, timed with HIP events on the
+                 solver's stream; achieved = algorithmic bytes per launch
+                 (points x B_alg/4, B_alg = 768 B/pt-step FP32, 672 B with
+                 BF16 metric) / average launch time; traffic = HBM bytes per
+                 launch from the committed rocprofv3 FETCH/WRITE summary when
+                 present
+  cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, same
+                 workload, a few steps
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt updates/s + HBM roofline %; RS fault step time MLP-Newton vs plain"
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_config{cfg}.txt")
+
+
+def b_alg(params):
+    """Algorithmic bytes per grid-point update (one full RK4 step): 144
+    wavefield words + 48 coefficient words (12 per stage)."""
+    return 144 * 4 + 48 * (2 if params.get("MetricBF16") else 4)
+
+
+def workload_name(cfg, p):
+    names = {0: "plain flat grid, slip weakening, Cerjan sponge",
+             1: "curved grid + topography, bent slip-weakening fault, 3-layer medium, 20-layer ADE-PML",
+             2: "curved grid, rate-and-state aging law, plain Newton",
+             3: "curved grid, rate-and-state aging law, learned Newton guess",
+             4: "large grid, rate-and-state, BF16 metric/media"}
+    return f"config{cfg}: {p['NX']}x{p['NY']}x{p['NZ']} at {p['DH']:g} m, {names.get(cfg, '')}"
+
+
+def read_traffic(cfg, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary."""
+    path = TRAFFIC_FILE.format(cfg=cfg)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.lstrip().startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    v = vals.get(f"{kernel}_hbm_bytes_per_launch")
+    try:
+        return float(v) if v is not None else None
+    except ValueError:
+        return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(case, budget_s):
+    """Time the FP64 oracle on the same workload (same grid and inputs) for a
+    few steps on the host cores.  The oracle is the checker of the test suite
+    and is used here only as the reported CPU baseline."""
+    from oracle.binding import Oracle
+    p = case["params"]
+    orc = Oracle(p)
+    if "geom" in case:
+        orc.set_geometry(*case["geom"])
+    if "media1d" in case:
+        orc.set_media1d(*case["media1d"])
+    if case.get("topo") is not None:
+        orc.set_topography(case["topo"])
+    for k, v in case.get("stress", {}).items():
+        orc.set_stress(k, v)
+    t0 = time.perf_counter()
+    orc.setup()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    orc.step(0)  # warm-up
+    t1 = time.perf_counter() - t0
+    n = int(max(1, min(20, budget_s // max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for it in range(1, 1 + n):
+        orc.step(it)
+    dt = (time.perf_counter() - t0) / n
+    npts = p["NX"] * p["NY"] * p["NZ"]
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": npts / dt, "unit": "grid-pt updates/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, full {p['NX']}x{p['NY']}x{p['NZ']} grid of the same workload, {n} steps after 1 "
+                      f"warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), {cpu_model()}"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", type=int, default=1, help="synthetic configuration (0-4)")
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline timing budget")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the whole domain lives in one MI355X's HBM; see the sharding design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.calibrate:
+        import torch
+        n = 256 * 1024 * 1024
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "copies": 3}))
+        return
+
+    import dynfd3dcuda_amd as dyn
+    from dynfd3dcuda_amd import configs
+
+    gen = configs.CONFIGS[args.config]
+    kw = {"nt": args.steps + args.warmup + 2}
+    if args.seed is not None:
+        kw["seed"] = args.seed
+    case = gen(**kw)
+    p = case["params"]
+    solver = dyn.Solver(p)
+    configs.apply(case, solver)
+    solver.setup()
+    d = solver.dims()
+    npts = d["ni"] * d["nj"] * d["nk"]
+
+    for it in range(args.warmup):
+        solver.step(it)
+    solver.synchronize()
+    solver.profile(True)
+    t0 = time.perf_counter()
+    for it in range(args.warmup, args.warmup + args.steps):
+        solver.step(it)
+    solver.synchronize()
+    t = time.perf_counter() - t0
+    prof = solver.profile_report()
+    solver.profile(False)
+
+    value = npts * args.steps / t
+    ba = b_alg(p)
+    stage = {k: v for k, v in prof.items() if k.startswith("stage_") and v[1] > 0}
+    dom = max(stage, key=lambda k: stage[k][0])
+    ms, n, pts = stage[dom]
+    avg_s = ms / n / 1e3
+    bytes_per_launch = pts * ba / 4.0  # one launch = one RK stage of its points
+    achieved = bytes_per_launch / avg_s / 1e9
+    traffic = read_traffic(args.config, dom)
+    out = {
+        "metric": METRIC, "value": value, "unit": "grid-pt updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded generators, dynfd3dcuda_amd.configs)",
+        "config": {"workload": workload_name(args.config, p), "grid_points": npts, "B_alg_bytes_per_pt_step": ba,
+                   "step_hbm_roofline_frac": value * ba / (PEAK_HBM_GBS * 1e9),
+                   "stage_points_per_launch": {k: v[2] for k, v in stage.items()}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "points_per_launch": pts, "avg_launch_ms": avg_s * 1e3},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1] > 0},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        del solver
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

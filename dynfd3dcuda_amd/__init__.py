@@ -29,8 +29,13 @@ EXPORTED = [
     "cgfdm_set_topography", "cgfdm_set_media1d", "cgfdm_set_init_stress", "cgfdm_setup", "cgfdm_step", "cgfdm_stage",
     "cgfdm_run", "cgfdm_synchronize", "cgfdm_wave_stage", "cgfdm_wave_rhs", "cgfdm_field_info",
     "cgfdm_get_field", "cgfdm_put_field", "cgfdm_get_dims", "cgfdm_open_outputs",
-    "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats",
+    "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
+    "cgfdm_profile_get",
 ]
+
+# kernel classes timed by cgfdm_profile / cgfdm_profile_get
+PROFILE_KERNELS = ["stage_interior", "stage_band", "stage_surface", "pml_aux",
+                   "fault_pre", "fault_deriv", "fault_post", "rup_sensor"]
 
 
 class CgfdmError(RuntimeError):
@@ -85,6 +90,9 @@ def load_library(path=None):
     lib.cgfdm_write_outputs.argtypes = [c_p, ctypes.c_int]
     lib.cgfdm_close_outputs.argtypes = [c_p]
     lib.cgfdm_newton_stats.argtypes = [c_p] + [ctypes.POINTER(ctypes.c_double)] * 3
+    lib.cgfdm_profile.argtypes = [c_p, ctypes.c_int]
+    lib.cgfdm_profile_get.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     for name in EXPORTED[1:]:
         getattr(lib, name).restype = ctypes.c_int
     if path is None:
@@ -227,3 +235,17 @@ class Solver:
         v = [ctypes.c_double() for _ in range(3)]
         self._check(self.lib.cgfdm_newton_stats(self.h, *[ctypes.byref(x) for x in v]))
         return {"solves": v[0].value, "iterations": v[1].value, "fallbacks": v[2].value}
+
+    def profile(self, enable=True):
+        """Start (clearing earlier records) or stop per-kernel GPU timing."""
+        self._check(self.lib.cgfdm_profile(self.h, 1 if enable else 0))
+
+    def profile_report(self):
+        """{kernel class: (total ms, launches, grid points per launch)} since profile(True)."""
+        out = {}
+        for name in PROFILE_KERNELS:
+            ms, n, pts = ctypes.c_double(), ctypes.c_long(), ctypes.c_long()
+            self._check(self.lib.cgfdm_profile_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                                   ctypes.byref(pts)))
+            out[name] = (ms.value, n.value, pts.value)
+        return out

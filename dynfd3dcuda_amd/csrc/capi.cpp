@@ -106,6 +106,7 @@ FieldRef field(Ctx& c, const std::string& n) {
     else if (n == "hT12") f32(F.hT12, nyz);
     else if (n == "hT13") f32(F.hT13, nyz);
     else if (n == "slabH") { f32(c.slabH, 7 * 9 * nyz); r.nf = 9; r.planes = 7; }
+    else if (n == "FWs") f32(F.Ws, F.Ws ? 18 * nyz : 0);
     else if (n == "Tn") f32(F.Tn, n2);
     else if (n == "Ts1") f32(F.Ts1, n2);
     else if (n == "Ts2") f32(F.Ts2, n2);
@@ -311,6 +312,11 @@ int cgfdm_step(cgfdm_ctx* h, int it) {
     int rc = cgfdm_stage(h, it, irk);
     if (rc) return rc;
   }
+  // smooth_gauss_volume once per step after the stages (main.cu:517-523)
+  if (c.p.Rupture && c.p.FaultSmooth) {
+    launch_fault_smooth(c);
+    return check_hip("fault smoothing launch");
+  }
   return 0;
 }
 
@@ -434,6 +440,40 @@ int cgfdm_get_dims(cgfdm_ctx* h, int dims[8]) {
   p.derive();
   dims[0] = p.ni; dims[1] = p.nj; dims[2] = p.nk; dims[3] = p.nx; dims[4] = p.ny; dims[5] = p.nz;
   dims[6] = p.nx / 2; dims[7] = p.NT;
+  return 0;
+}
+
+int cgfdm_profile(cgfdm_ctx* h, int enable) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (c.stream) hipStreamSynchronize(c.stream);
+  c.prof.reset();
+  c.prof.on = enable != 0;
+  if (c.prof.on) c.prof.grow(8192);
+  return check_hip("profile");
+}
+
+int cgfdm_profile_get(cgfdm_ctx* h, const char* name, double* total_ms, long* launches, long* points) {
+  if (!h || !name) return fail(CGFDM_EPARAM, "null argument");
+  Ctx& c = h->c;
+  int k = -1;
+  for (int q = 0; q < PK_COUNT; q++)
+    if (std::strcmp(name, kProfNames[q]) == 0) k = q;
+  if (k < 0) return fail(CGFDM_EPARAM, std::string("unknown kernel class ") + name);
+  if (c.stream) hipStreamSynchronize(c.stream);
+  double ms = 0.0;
+  long n = 0;
+  for (const auto& m : c.prof.marks) {
+    if (m.first != k) continue;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, c.prof.pool[m.second], c.prof.pool[m.second + 1]) != hipSuccess)
+      return fail(CGFDM_EHIP, "event timing failed");
+    ms += t;
+    n++;
+  }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = n;
+  if (points) *points = k <= PK_STAGE_SURFACE ? c.wave_points[k] : 0;
   return 0;
 }
 

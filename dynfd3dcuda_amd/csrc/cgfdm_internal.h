@@ -33,6 +33,7 @@ struct Params {
   std::string Media1D, Media3D;
   // Makefile feature macros as runtime flags (Makefile:36-121 defaults)
   int usePML = 1, useCerjan = 0, FreeSurface = 1, NormalRefine = 1, Rupture = 1;
+  int FaultSmooth = 0;  // Makefile:104: Gaussian smoothing of the fault tractions once per step
   // extensions
   int MetricBF16 = 0;   // store metric + media in bfloat16 (FP32 compute)
   int NewtonMode = 0;   // 0: reference NRsearch; 1: residual tolerance + plain fallback
@@ -90,6 +91,8 @@ struct FaultDev {
   float *hslip, *Vs1, *Vs2, *init_t0, *friction, *rsD_y, *rsD_z, *rup_sensor;
   int *init_t0_flag, *rup_index_y, *rup_index_z, *united, *faultgrid, *flag_rup;
   unsigned long long* newton_stats;  // [3]: solves, iterations, fallbacks
+  float *Ws;                         // [9][2][nz][ny] smoothed T2*/T3* (FaultSmooth)
+  float *smooth_w;                   // [2][4][7][7] half-volume Gaussian weights (FaultSmooth)
 };
 
 struct PmlDev {
@@ -100,6 +103,39 @@ struct PmlDev {
 };
 
 // ---------------------------------------------------------------------------
+// Optional GPU timing of each kernel class with HIP events on the solver's
+// stream (off unless enabled; events come from a pool created on enable).
+enum ProfKernel {
+  PK_STAGE_INTERIOR, PK_STAGE_BAND, PK_STAGE_SURFACE, PK_PML_AUX,
+  PK_FAULT_PRE, PK_FAULT_DERIV, PK_FAULT_POST, PK_RUP_SENSOR, PK_COUNT
+};
+extern const char* const kProfNames[PK_COUNT];
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, size_t>> marks;  // (kernel class, index of its start event)
+  size_t used = 0;
+  void begin(hipStream_t s, int k) {
+    if (!on) return;
+    if (used + 2 > pool.size()) grow(used + 2048);
+    hipEventRecord(pool[used], s);
+    marks.push_back({k, used});
+    used += 2;
+  }
+  void end(hipStream_t s) {
+    if (on && !marks.empty()) hipEventRecord(pool[marks.back().second + 1], s);
+  }
+  void grow(size_t n) {
+    while (pool.size() < n) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pool.push_back(e);
+    }
+  }
+  void reset() { marks.clear(); used = 0; }
+  ~Profiler() { for (hipEvent_t e : pool) hipEventDestroy(e); }
+};
+
 struct Ctx {
   Params p;
   Dims d;
@@ -123,6 +159,8 @@ struct Ctx {
   void* wave_cold = nullptr;    // device copy of the off-hot-path pointer table
   void* wave_work = nullptr;    // stage-kernel work list (two int4 per block)
   int wave_nwork = 0, wave_nwork_int = 0, wave_nwork_edge = 0;  // all items; interior, band items (the rest: free-surface tiles)
+  long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
+  Profiler prof;
   PmlDev pml{};
   FaultDev f{};
   std::vector<void*> allocations;
@@ -145,6 +183,7 @@ void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy,
 void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz);
 void launch_fault_post(Ctx& c, const StageBufs& b, int irk);
 void launch_rup_sensor(Ctx& c);
+void launch_fault_smooth(Ctx& c);
 void launch_fill(Ctx& c, float* p, long n, float v);
 int upload_wave_cold(Ctx& c);  // nonzero on a HIP error
 

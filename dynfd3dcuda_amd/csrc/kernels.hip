@@ -38,6 +38,8 @@ struct FArgs {
   double DTd;
   int it, irk, FX, FY, FZ;
   int freenode, friction, stress_type, cerjan;
+  int G[4];        // Fault_grid
+  float Dc0;       // scalar Dc of the JSON (smooth.cu:318)
   float mu_d, viscosity, smooth_load_T, Vini;
   double f0, V0, fw;
   int newton_mode, newton_fallback;
@@ -694,6 +696,94 @@ __global__ void rup_sensor_m_kernel(FaultDev F, int nj, int nk, int g0, int g1, 
   if (F.united[q]) F.rup_sensor[q] = 0;
 }
 
+// smooth_gauss_volume0 (smooth.cu:15-62): mirror the split nodes, the
+// fault-plane T1 and the wavefield planes i0-3..i0+3 across the free
+// surface into the three ghost rows above it (one thread per j).
+__global__ void smooth_ext_kernel(FArgs a, float* __restrict__ Wv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.nj) return;
+  const FaultDev& F = a.F;
+  const long nyz = a.nyz, ny = a.ny;
+  const int j1 = j + 3, k1 = a.nk - 1 + 3;
+  for (int l = 0; l < 9; l++)
+    for (int m = 0; m < 2; m++) {
+      float* f = F.W + (long)l * 2 * nyz + m * nyz;
+      for (int n = 1; n <= 3; n++) f[(k1 + n) * ny + j1] = 2.0f * f[k1 * ny + j1] - f[(k1 - n) * ny + j1];
+    }
+  float* T[3] = {F.T11 + 3 * nyz, F.T12 + 3 * nyz, F.T13 + 3 * nyz};
+  for (int t = 0; t < 3; t++)
+    for (int n = 1; n <= 3; n++) T[t][(k1 + n) * ny + j1] = 2.0f * T[t][k1 * ny + j1] - T[t][(k1 - n) * ny + j1];
+  for (int l = 0; l < 9; l++)
+    for (int i1 = a.i0 - 3; i1 <= a.i0 + 3; i1++) {
+      float* w = Wv + i1 * a.wps + (long)l * a.pyz + a.jpad + j1;
+      for (int n = 1; n <= 3; n++) w[(long)(k1 + n) * a.py] = 2.0f * w[(long)k1 * a.py] - w[(long)(k1 - n) * a.py];
+    }
+}
+
+// smooth_gauss_volume1 (smooth.cu:64-296): T2*/T3* of each split node as a
+// half-volume Gaussian average of the node's own plane and the conservative
+// tractions J (sigma . grad eta), J (sigma . grad zeta) of the three volume
+// planes on its side.
+__global__ void __launch_bounds__(256) smooth_avg_kernel(FArgs a, const float* __restrict__ Wv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j >= a.nj || k >= a.nk) return;
+  const FaultDev& F = a.F;
+  const long nyz = a.nyz, ny = a.ny, pyz = a.pyz;
+  const int j1 = j + 3, k1 = k + 3;
+  for (int m = 0; m < 2; m++) {
+    const float* wp = F.smooth_w + m * 4 * 49;
+    float Ts[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int aa = 0; aa < 7; aa++)
+      for (int b = 0; b < 7; b++) {
+        const long q = (long)(k1 + aa - 3) * ny + (j1 + b - 3);
+        const float w = wp[aa * 7 + b];
+        for (int c = 0; c < 6; c++) Ts[c] += F.W[(3 + c) * 2 * nyz + m * nyz + q] * w;
+      }
+    for (int l = 1; l <= 3; l++) {
+      const int i = m == 0 ? a.i0 - l : a.i0 + l;
+      for (int aa = 0; aa < 7; aa++)
+        for (int b = 0; b < 7; b++) {
+          const long pc = a.jpad + (j1 + b - 3) + (long)(k1 + aa - 3) * a.py;
+          const long pm = pc + i * a.mps, pw = pc + i * a.wps;
+          float E[3][3];  // columns grad xi, grad eta, grad zeta
+          for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) E[r][cc] = LM(a, (3 * cc + r) * pyz + pm);
+          const float sxx = Wv[3 * pyz + pw], syy = Wv[4 * pyz + pw], szz = Wv[5 * pyz + pw];
+          const float sxy = Wv[6 * pyz + pw], sxz = Wv[7 * pyz + pw], syz = Wv[8 * pyz + pw];
+          const float S[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
+          const float w = LM(a, 9 * pyz + pm) * wp[(l * 7 + aa) * 7 + b];
+          for (int r = 0; r < 3; r++) {
+            const float t1 = S[r][0] * E[0][1] + S[r][1] * E[1][1] + S[r][2] * E[2][1];
+            const float t2 = S[r][0] * E[0][2] + S[r][1] * E[1][2] + S[r][2] * E[2][2];
+            Ts[r] += t1 * w;
+            Ts[3 + r] += t2 * w;
+          }
+        }
+    }
+    for (int c = 0; c < 6; c++) F.Ws[(3 + c) * 2 * nyz + m * nyz + (long)k1 * ny + j1] = Ts[c];
+  }
+}
+
+// smooth_gauss_volume2 (smooth.cu:298-345): blend the smoothed tractions in
+// by min(slip/Dc, 1) inside Fault_grid; outside it the smoothed value wins.
+__global__ void smooth_blend_kernel(FArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j >= a.nj || k >= a.nk) return;
+  const FaultDev& F = a.F;
+  const int gj = j + 1, gk = k + 1;
+  float beta = 1.0f;
+  if (gj >= a.G[0] && gj <= a.G[1] && gk >= a.G[2] && gk <= a.G[3])
+    beta = fminf((float)F.slip[(long)k * a.nj + j] / a.Dc0, 1.0f);
+  const long pf = (long)(k + 3) * a.ny + (j + 3);
+  for (int l = 3; l < 9; l++)
+    for (int m = 0; m < 2; m++) {
+      const long x = (long)l * 2 * a.nyz + m * a.nyz + pf;
+      F.W[x] = beta * F.Ws[x] + (1.0f - beta) * F.W[x];
+    }
+}
+
 __global__ void fill_kernel(float* p, long n, float v) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   long st = (long)gridDim.x * blockDim.x;
@@ -712,6 +802,8 @@ FArgs make_fargs(Ctx& c) {
   a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.nx = c.d.nx; a.ny = c.d.ny; a.nz = c.d.nz; a.i0 = c.d.i0;
   a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = 13 * c.d.pyz;
   a.py = c.d.py; a.jpad = c.d.jpad;
+  for (int q = 0; q < 4; q++) a.G[q] = c.p.Fault_grid[q];
+  a.Dc0 = (float)c.p.Dc;
   a.NY = c.p.NY; a.NZ = c.p.NZ;
   a.DH = (float)c.p.DH; a.rDH = (float)(1.0 / c.p.DH); a.DT = (float)c.p.DT; a.DTd = c.p.DT;
   a.freenode = c.p.FreeSurface ? 1 : 0;
@@ -733,28 +825,44 @@ void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy,
   FArgs a = make_fargs(c);
   a.W = Win; a.Wi0 = const_cast<float*>(Win);
   a.it = it; a.irk = irk; a.FX = fx; a.FY = fy; a.FZ = fz;
+  c.prof.begin(c.stream, PK_FAULT_PRE);
   hipLaunchKernelGGL(fault_pre_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
+  c.prof.end(c.stream);
 }
 
 void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz) {
   FArgs a = make_fargs(c);
   a.W = Win;
   a.FX = fx; a.FY = fy; a.FZ = fz;
+  c.prof.begin(c.stream, PK_FAULT_DERIV);
   hipLaunchKernelGGL(fault_deriv_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
+  c.prof.end(c.stream);
 }
 
 void launch_fault_post(Ctx& c, const StageBufs& b, int irk) {
   FArgs a = make_fargs(c);
   a.Win = b.Win; a.Wi0 = b.Wout; a.S = b.S; a.T = b.T;
   a.irk = irk;
+  c.prof.begin(c.stream, PK_FAULT_POST);
   hipLaunchKernelGGL(fault_post_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
+  c.prof.end(c.stream);
 }
 
 void launch_rup_sensor(Ctx& c) {
   dim3 g((c.d.nj + 63) / 64, (c.d.nk + 3) / 4, 1);
   const int* G = c.p.Fault_grid;
+  c.prof.begin(c.stream, PK_RUP_SENSOR);
   hipLaunchKernelGGL(rup_sensor_d_kernel, g, dim3(64, 4, 1), 0, c.stream, c.f, c.d.nj, c.d.nk, G[0], G[1], G[2], G[3]);
   hipLaunchKernelGGL(rup_sensor_m_kernel, g, dim3(64, 4, 1), 0, c.stream, c.f, c.d.nj, c.d.nk, G[0], G[1], G[2], G[3]);
+  c.prof.end(c.stream);
+}
+
+void launch_fault_smooth(Ctx& c) {
+  FArgs a = make_fargs(c);
+  if (c.p.FreeSurface)
+    hipLaunchKernelGGL(smooth_ext_kernel, dim3((c.d.nj + 255) / 256), dim3(256), 0, c.stream, a, c.cur);
+  hipLaunchKernelGGL(smooth_avg_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a, (const float*)c.cur);
+  hipLaunchKernelGGL(smooth_blend_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
 }
 
 void launch_fill(Ctx& c, float* p, long n, float v) {

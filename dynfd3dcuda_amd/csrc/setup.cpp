@@ -589,6 +589,25 @@ int host_setup(Ctx& c, std::string& err) {
     F.init_t0_flag = dmalloc<int>(c, n2); F.flag_rup = dmalloc<int>(c, n2);
     UPI(rup_index_y, riy); UPI(rup_index_z, riz); UPI(united, uni); UPI(faultgrid, fg);
     F.newton_stats = dmalloc<unsigned long long>(c, 4);
+    if (p.FaultSmooth) {
+      // half-volume Gaussian weights of smooth_gauss_volume1 (smooth.cu:70-120):
+      // exp(-r^2/0.7^2), off-plane layers x 0.4, each side normalised to 1
+      std::vector<float> w(2 * 4 * 49);
+      std::vector<double> wd(4 * 49);
+      double sum = 0;
+      for (int ii = 0; ii < 4; ii++)
+        for (int a = 0; a < 7; a++)
+          for (int b = 0; b < 7; b++) {
+            const double r2 = (double)(ii * ii + (a - 3) * (a - 3) + (b - 3) * (b - 3)) / (0.7 * 0.7);
+            wd[(ii * 7 + a) * 7 + b] = std::exp(-r2) * (ii == 0 ? 1.0 : 0.4);
+            sum += wd[(ii * 7 + a) * 7 + b];
+          }
+      for (int s = 0; s < 2; s++)
+        for (int q = 0; q < 4 * 49; q++) w[s * 4 * 49 + q] = (float)(wd[q] / sum);
+      F.smooth_w = dmalloc<float>(c, w.size());
+      F.Ws = dmalloc<float>(c, 18 * nyz);
+      if (!F.smooth_w || !F.Ws || !upload(F.smooth_w, w)) { err = "hipMalloc failed for the fault smoothing"; return 6; }
+    }
 #undef UPF
 #undef UPI
     c.slabH = dmalloc<float>(c, 7 * 9 * pyz);

@@ -56,19 +56,26 @@ __device__ __forceinline__ Rsrc plane_rsrc(const T* a, int plane, uint32_t nf, u
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(a) + (size_t)plane * nf * nyz, (short)0,
                                            (int)(nf * nyz * (uint32_t)sizeof(T)), kBufWord3);
 }
+// Cache policy of an access: kReuse for stage-input words that neighbouring
+// tiles read again (halo rows and columns, the x-queue warm-up), kStream
+// (non-temporal) for words touched once per stage, so they do not push the
+// reused rows out of the XCD's L2.
+constexpr int kReuse = 0, kStream = 2;
+template <int POL = kReuse>
 __device__ __forceinline__ float bld(Rsrc r, uint32_t voff, uint32_t soff) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, POL));
 }
+template <int POL = kStream>
 __device__ __forceinline__ void bst(Rsrc r, uint32_t voff, uint32_t soff, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, POL);
 }
 template <class MT>
 __device__ __forceinline__ float bldm(Rsrc r, uint32_t voff, uint32_t soff) {
   if constexpr (sizeof(MT) == 2) {
-    const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+    const unsigned short v = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, kStream);
     return __uint_as_float(((unsigned)v) << 16);
   } else {
-    return bld(r, voff, soff);
+    return bld<kStream>(r, voff, soff);
   }
 }
 // Word of field c at padded plane x, column jk (plain loads, off the hot path).
@@ -89,9 +96,9 @@ __device__ __forceinline__ float rk_combine_b(int irk, float rka, float rkb, flo
     bst(rt, voff, soff, w + rkb * h);
     return w + rka * h;
   }
-  if (irk == 3) return bld(rt, voff, soff) + rkb * h;
-  bst(rt, voff, soff, bld(rt, voff, soff) + rkb * h);
-  return bld(rs, voff, soff) + rka * h;
+  if (irk == 3) return bld<kStream>(rt, voff, soff) + rkb * h;
+  bst(rt, voff, soff, bld<kStream>(rt, voff, soff) + rkb * h);
+  return bld<kStream>(rs, voff, soff) + rka * h;
 }
 
 template <class MT>
@@ -411,7 +418,9 @@ __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
 // the traction-free term on the free surface for the x/y faces, advanced
 // with the same RK4 stage combine as the wavefield.  Runs after the stage
 // kernel (which read the old aux values) on the unchanged stage input.  One
-// thread per band point, j fastest, so aux accesses are contiguous.
+// thread per band point, j fastest so aux accesses are contiguous; on x
+// faces the band plane index comes next, so the five planes of an
+// x-derivative are read by neighbouring blocks and stay in L2.
 template <class MT>
 struct PmlAuxArgs {
   const float* __restrict__ Win;
@@ -436,7 +445,9 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
   if (idx >= ps) return;
   const int pj = (int)(idx % pnj);
   const long t = idx / pnj;
-  const int pk = (int)(t % pnk), pi = (int)(t / pnk);
+  const int pk = ax == 0 ? (int)(t / pni) : (int)(t % pnk);
+  const int pi = ax == 0 ? (int)(t % pni) : (int)(t / pnk);
+  const long pa = pj + (long)pk * pnj + (long)pi * pnj * pnk;  // aux layout [c][pi][pk][pj]
   const int i = pi + (ax == 0 ? off : 0), j = pj + (ax == 1 ? off : 0), k = pk + (ax == 2 ? off : 0);
   const int li = ax == 0 ? i : ax == 1 ? j : k;
   const unsigned pyz = a.pyz;
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
   float* T = cold->auxT[f];
 #pragma unroll
   for (int c = 0; c < 9; c++) {
-    const long q = c * ps + idx;
+    const long q = c * ps + pa;
     const float y = Y[q];
     float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
     if (fsx && c >= 3) ah += pml_fs_term(c, z1, ztx, zty, ztz, lam, l2m, mu, c0);
@@ -486,17 +497,24 @@ template <int FX, int FY, int FZ, class MT>
 void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   const int n0 = c.wave_nwork_int, n1 = n0 + c.wave_nwork_edge, n2 = c.wave_nwork;
   const int4* w = (const int4*)c.wave_work;
+  Profiler& pr = const_cast<Ctx&>(c).prof;
   if (n0 > 0) {
     a.work = w; a.nwork = n0;
+    pr.begin(s, PK_STAGE_INTERIOR);
     hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+    pr.end(s);
   }
   if (n1 > n0) {
     a.work = w + 2 * n0; a.nwork = n1 - n0;
+    pr.begin(s, PK_STAGE_BAND);
     hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+    pr.end(s);
   }
   if (n2 > n1) {
     a.work = w + 2 * n1; a.nwork = n2 - n1;
+    pr.begin(s, PK_STAGE_SURFACE);
     hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+    pr.end(s);
   }
   if (a.pml && !a.rhs_only) {
     PmlAuxArgs<MT> p{};
@@ -509,7 +527,9 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
       const int ax = f / 2;
       const long n = (long)c.pml.N * (ax == 0 ? (long)a.nj * a.nk : ax == 1 ? (long)a.nk * a.ni : (long)a.ni * a.nj);
       p.f = f; p.flag = flags[ax];
+      pr.begin(s, PK_PML_AUX);
       hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
+      pr.end(s);
     }
   }
 }
@@ -570,6 +590,9 @@ std::vector<Seg> axis_segments(int n, int lo, int hi, int t) {
 
 }  // namespace
 
+const char* const kProfNames[PK_COUNT] = {"stage_interior", "stage_band", "stage_surface", "pml_aux",
+                                          "fault_pre", "fault_deriv", "fault_post", "rup_sensor"};
+
 int interior_lo_j(const Params& p) {
   return std::max(p.usePML ? p.PML_N : 0, p.useCerjan ? p.DAMP_N : 0);
 }
@@ -604,34 +627,49 @@ int upload_wave_cold(Ctx& c) {
   if (freenode) hi[2] = std::max(hi[2], 3);
   const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
   const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
-  int tiles_in = 0;
-  for (const Seg& b : sk) for (const Seg& a : sj) tiles_in += a.in && b.in;
+  const int nk = n3[2];
+  auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
+  int tiles[3] = {0, 0, 0};
+  for (const Seg& b : sk)
+    for (const Seg& a : sj) tiles[is_top(b) ? 2 : (a.in && b.in ? 0 : 1)]++;
   const int ni = n3[0], ia = std::min(lo[0], ni), ib = std::max(ia, ni - hi[0]);
-  int ich = p.WaveIChunk;
-  if (ich <= 0) {
-    const int span = std::max(1, ib - ia), chunks = std::max(1, std::min(span, (2048 + std::max(tiles_in, 1) - 1) / std::max(tiles_in, 1)));
-    ich = std::max((span + chunks - 1) / chunks, std::min(8, span));
-  }
+  // planes per block of each kind: short chunks (several thousand blocks per
+  // launch) balance the CUs better than the longer chunks would save on the
+  // 4-plane warm-up of the x-queue; the few free-surface tiles get the
+  // shortest chunks so that their launch still fills the GPU
+  auto chunk = [&](int span, int ntiles, int target, int minlen) {
+    if (p.WaveIChunk > 0) return p.WaveIChunk;
+    span = std::max(span, 1);
+    const int chunks = std::max(1, std::min(span, (target + std::max(ntiles, 1) - 1) / std::max(ntiles, 1)));
+    return std::max((span + chunks - 1) / chunks, std::min(minlen, span));
+  };
+  const int ich_in = chunk(ib - ia, tiles[0], 4096, 4);
+  const int ich_band = chunk(ni, tiles[1] + tiles[0] * (ni - (ib - ia)) / std::max(ni, 1), 4096, 4);
+  const int ich_top = chunk(ni, tiles[2], 2048, 1);
   // kind 0 interior, 1 absorbing bands, 2 tiles holding free-surface rows
   std::vector<int4> lists[3];
+  for (int kd = 0; kd < 3; kd++) c.wave_points[kd] = 0;
   auto push = [&](int kind, const Seg& a, const Seg& b, int i0, int i1) {
     lists[kind].push_back(make_int4(a.s, b.s, i0, i1));
     lists[kind].push_back(make_int4(a.n, b.n, 0, 0));
+    c.wave_points[kind] += (long)a.n * b.n * (i1 - i0);
   };
-  const int nk = n3[2];
-  auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
-  for (int i0 = ia; i0 < ib; i0 += ich)
-    for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich));
-  for (int i0 = 0; i0 < ni; i0 += ich)
+  for (int i0 = ia; i0 < ib; i0 += ich_in)
+    for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
+  for (int i0 = 0; i0 < ni; i0 += ich_band)
     for (const Seg& b : sk)
       for (const Seg& a : sj) {
-        const int i1 = std::min(ni, i0 + ich);
-        if (is_top(b)) { push(2, a, b, i0, i1); continue; }
+        const int i1 = std::min(ni, i0 + ich_band);
+        if (is_top(b)) continue;
         if (!(a.in && b.in)) { push(1, a, b, i0, i1); continue; }
         // interior tile: only the parts of [i0, i1) outside [ia, ib)
         if (i0 < ia) push(1, a, b, i0, std::min(i1, ia));
         if (i1 > ib) push(1, a, b, std::max(i0, ib), i1);
       }
+  for (int i0 = 0; i0 < ni; i0 += ich_top)
+    for (const Seg& b : sk)
+      for (const Seg& a : sj)
+        if (is_top(b)) push(2, a, b, i0, std::min(ni, i0 + ich_top));
   std::vector<int4> all;
   for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
   if (c.wave_work) { hipFree(c.wave_work); c.wave_work = nullptr; }

@@ -1,0 +1,115 @@
+"""Summaries of rocprofv3 output for the solver's kernels.
+
+  python -m dynfd3dcuda_amd.profiling stats  <trace-dir>
+      per kernel class: launches, average / total GPU time (from a
+      `rocprofv3 --kernel-trace --output-format csv` run)
+  python -m dynfd3dcuda_amd.profiling traffic --fetch D1 --write D2 --cal-fetch D3 --cal-write D4 --bench-json F
+      HBM bytes per launch of each stage-kernel class from `--pmc FETCH_SIZE`
+      and `--pmc WRITE_SIZE` runs, each scaled by what the same counter reads
+      for a device-to-device copy of known size (`bench.py --calibrate`), next
+      to the algorithmic bytes per launch (points per launch from the JSON
+      line bench.py printed in the same configuration)
+
+Kernel classes: stage_interior / stage_band / stage_surface (the three
+instantiations of the wave stage kernel), pml_aux, fault_pre, fault_deriv,
+fault_post, rup_sensor; everything else is listed under its own name.
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import re
+import sys
+
+
+def kernel_class(name):
+    m = re.search(r"wave_stage_kernel<[^>]*,\s*(\d)>", name)
+    if m:
+        return ["stage_interior", "stage_band", "stage_surface"][int(m.group(1))]
+    m = re.search(r"wave_stage_kernelI.*?Li([012])EEEv", name)  # mangled form
+    if m:
+        return ["stage_interior", "stage_band", "stage_surface"][int(m.group(1))]
+    for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_deriv_kernel", "fault_post_kernel"):
+        if k in name:
+            return k[: -len("_kernel")]
+    if "rup_sensor" in name:
+        return "rup_sensor"
+    return name.split("(")[0].strip()
+
+
+def _csv(dirpath, suffix):
+    files = glob.glob(os.path.join(dirpath, "**", f"*{suffix}"), recursive=True)
+    if not files:
+        raise SystemExit(f"no *{suffix} under {dirpath}")
+    rows = []
+    for f in files:
+        rows += list(csv.DictReader(open(f)))
+    return rows
+
+
+def stats(dirpath):
+    agg = collections.defaultdict(list)
+    for r in _csv(dirpath, "kernel_trace.csv"):
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        agg[kernel_class(r["Kernel_Name"])].append(dur)
+    tot = sum(sum(v) for v in agg.values())
+    lines = [f"{'kernel class':40s} {'launches':>9s} {'avg_us':>10s} {'total_ms':>10s} {'share':>7s}"]
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"{k[:40]:40s} {len(v):9d} {sum(v) / len(v):10.1f} {sum(v) / 1e3:10.3f} {100 * sum(v) / tot:6.1f}%")
+    return "\n".join(lines)
+
+
+def _counter(dirpath, counter):
+    agg = collections.defaultdict(list)
+    for r in _csv(dirpath, "counter_collection.csv"):
+        if r["Counter_Name"] == counter:
+            agg[kernel_class(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return agg
+
+
+def traffic(a):
+    # calibration: a 1 GiB device-to-device copy reads and writes 2^30 bytes
+    cal_bytes = float(2 ** 30)
+    cf = _counter(a.cal_fetch, "FETCH_SIZE")
+    cw = _counter(a.cal_write, "WRITE_SIZE")
+    copy_k = max(cf, key=lambda k: sum(cf[k]))
+    f_fetch = cal_bytes / (max(cf[copy_k]) * 1024.0)
+    f_write = cal_bytes / (max(cw[copy_k]) * 1024.0)
+    fetch = _counter(a.fetch, "FETCH_SIZE")
+    write = _counter(a.write, "WRITE_SIZE")
+    bj = json.loads(open(a.bench_json).read().strip().splitlines()[-1])
+    pts = bj["config"].get("stage_points_per_launch", {})
+    ba = bj["config"]["B_alg_bytes_per_pt_step"]
+    out = [f"# calibration kernel: {copy_k}; FETCH_SIZE x {f_fetch:.3f}, WRITE_SIZE x {f_write:.3f} (KiB -> bytes x factor)",
+           f"fetch_scale: {f_fetch:.4f}", f"write_scale: {f_write:.4f}", f"B_alg_bytes_per_pt_step: {ba}"]
+    for k in sorted(fetch):
+        if k not in write:
+            continue
+        fb = sum(fetch[k]) / len(fetch[k]) * 1024.0 * f_fetch
+        wb = sum(write[k]) / len(write[k]) * 1024.0 * f_write
+        out.append(f"{k}_fetch_bytes_per_launch: {fb:.6g}")
+        out.append(f"{k}_write_bytes_per_launch: {wb:.6g}")
+        out.append(f"{k}_hbm_bytes_per_launch: {fb + wb:.6g}")
+        if k in pts and pts[k]:
+            alg = pts[k] * ba / 4.0
+            out.append(f"{k}_alg_bytes_per_launch: {alg:.6g}")
+            out.append(f"{k}_hbm_over_alg: {(fb + wb) / alg:.3f}")
+    return "\n".join(out)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    t = sub.add_parser("traffic")
+    for k in ("fetch", "write", "cal_fetch", "cal_write", "bench_json"):
+        t.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
+    a = ap.parse_args(argv)
+    print(stats(a.dir) if a.cmd == "stats" else traffic(a))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -121,6 +121,17 @@ int cgfdm_close_outputs(cgfdm_ctx* ctx);
  * solves, total iterations, and solves that fell back to the plain guess. */
 int cgfdm_newton_stats(cgfdm_ctx* ctx, double* solves, double* iterations, double* fallbacks);
 
+/* GPU timing per kernel class (extension, no counterpart in the reference):
+ * HIP events recorded on the solver's stream around every launch while
+ * enabled.  cgfdm_profile(ctx, 1) clears and starts recording, 0 stops.
+ * cgfdm_profile_get returns, for kernel class `name` ("stage_interior",
+ * "stage_band", "stage_surface", "pml_aux", "fault_pre", "fault_deriv",
+ * "fault_post", "rup_sensor"), the summed GPU time in ms and the launch count
+ * since recording started (it synchronizes the stream), and the grid points
+ * one launch of a stage-kernel class updates (0 for the others). */
+int cgfdm_profile(cgfdm_ctx* ctx, int enable);
+int cgfdm_profile_get(cgfdm_ctx* ctx, const char* name, double* total_ms, long* launches, long* points_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

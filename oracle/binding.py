@@ -1,6 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/cgfdm_oracle.cpp).
 
-Test infrastructure only: tests compare the HIP product path against it.
+Test infrastructure only: the tests, the smoke check and the CPU-baseline
+leg of bench.py use it as the checker; the product path never does.
 """
 import ctypes
 import os
@@ -8,8 +9,7 @@ import subprocess
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 _lib = None
@@ -39,6 +39,7 @@ def load():
     lib.orc_step.argtypes = [P, ctypes.c_int]
     lib.orc_stage.argtypes = [P] + [ctypes.c_int] * 5
     lib.orc_rup_sensor.argtypes = [P]
+    lib.orc_smooth.argtypes = [P]
     lib.orc_wave_deriv.argtypes = [P] + [ctypes.c_int] * 3
     lib.orc_pml_deriv.argtypes = [P] + [ctypes.c_int] * 3
     lib.orc_wave_rk.argtypes = [P, ctypes.c_int]
@@ -51,7 +52,7 @@ def load():
     lib.orc_get.argtypes = [P, ctypes.c_char_p, P, L]
     lib.orc_put.argtypes = [P, ctypes.c_char_p, P, L]
     for n in ["orc_set_param", "orc_set_geometry", "orc_set_coord3d", "orc_set_topography", "orc_set_media1d", "orc_set_media3d",
-              "orc_set_stress", "orc_setup", "orc_step", "orc_stage", "orc_rup_sensor", "orc_wave_deriv",
+              "orc_set_stress", "orc_setup", "orc_step", "orc_stage", "orc_rup_sensor", "orc_smooth", "orc_wave_deriv",
               "orc_pml_deriv", "orc_wave_rk", "orc_apply_cerjan", "orc_nr_search", "orc_get", "orc_put"]:
         getattr(lib, n).restype = ctypes.c_int
     _lib = lib
@@ -124,6 +125,9 @@ class Oracle:
 
     def stage(self, it, irk, fx, fy, fz):
         self.lib.orc_stage(self.h, it, irk, fx, fy, fz)
+
+    def smooth(self):
+        self.lib.orc_smooth(self.h)
 
     def wave_deriv(self, fx, fy, fz):
         self.lib.orc_wave_deriv(self.h, fx, fy, fz)

@@ -148,6 +148,7 @@ struct Oracle {
   real viscosity = 0, RupThres = 0;
   // runtime forms of the Makefile feature macros
   int usePML = 1, useCerjan = 0, FreeSurface = 1, NormalRefine = 1, Rupture = 1;
+  int FaultSmooth = 0;  // Makefile:104 (off by default)
 
   // ---- derived
   int ni, nj, nk, nx, ny, nz, i0;
@@ -175,6 +176,7 @@ struct Oracle {
 
   // fault
   std::vector<real> FW, FmW, FhW, FtW;               // 9*2*nyz
+  std::vector<real> FWs;                             // 9*2*nyz, smoothed T2*/T3* (smooth.cu)
   std::vector<real> T11, T12, T13, mT11, mT12, mT13, hT11, hT12, hT13;  // 7*nyz
   std::vector<real> rho_f, lam_f, mu_f;              // 2*nyz
   std::vector<real> D[2][3][3];                      // D(a+1)(b+1)_(m+1): [m][a][b] -> nyz*9
@@ -208,7 +210,7 @@ struct Oracle {
     SETR(mu_s) SETR(mu_d) SETR(mu_0) SETR(Dc) SETR(C0)
     SETR(smooth_load_T) SETR(bi_vp1) SETR(bi_vs1) SETR(bi_rho1) SETR(bi_vp2) SETR(bi_vs2)
     SETR(bi_rho2) SETR(viscosity) SETR(RupThres)
-    SETI(usePML) SETI(useCerjan) SETI(FreeSurface) SETI(NormalRefine) SETI(Rupture)
+    SETI(usePML) SETI(useCerjan) SETI(FreeSurface) SETI(NormalRefine) SETI(Rupture) SETI(FaultSmooth)
     // JSON key names that differ from the struct names, set_params.cu:222-247
     if (k == "RS_V0") { V0 = v; return 0; }
     if (k == "RS_Vini") { Vini = v; return 0; }
@@ -1451,6 +1453,101 @@ struct Oracle {
   }
 
   // One time step, main.cu:420-535 (FD flag table main.cu:86-94).
+  // smooth_gauss_volume (smooth.cu:13-367), run once per step after the
+  // four stages when FaultSmooth is set (main.cu:517-523).
+  void smooth_gauss() {
+    const int HW = 3;  // SMOOTH_HALF_WIDTH (smooth.cu:7)
+    FWs.resize(18 * nyz, 0.0);
+    // smooth_gauss_volume0 (smooth.cu:15-62): mirror the split nodes, the
+    // fault-plane T1 and the wavefield planes i0-3..i0+3 above the surface.
+    if (freenode) {
+      const int k1 = nk - 1 + 3;
+      for (int j = 0; j < nj; j++) {
+        const int j1 = j + 3;
+        for (int l = 0; l < 9; l++)
+          for (int m = 0; m < 2; m++)
+            for (int n = 1; n <= HW; n++) {
+              real* f = &FW[(long)l * 2 * nyz + m * nyz];
+              f[(k1 + n) * ny + j1] = 2.0 * f[k1 * ny + j1] - f[(k1 - n) * ny + j1];
+            }
+        for (int n = 1; n <= HW; n++)
+          for (std::vector<real>* T : {&T11, &T12, &T13}) {
+            real* f = &(*T)[3 * nyz];
+            f[(k1 + n) * ny + j1] = 2.0 * f[k1 * ny + j1] - f[(k1 - n) * ny + j1];
+          }
+        for (int l = 0; l < 9; l++)
+          for (int i1 = i0 - HW; i1 <= i0 + HW; i1++)
+            for (int n = 1; n <= HW; n++)
+              W[l * nxyz + P(i1, j1, k1 + n)] = 2.0 * W[l * nxyz + P(i1, j1, k1)] - W[l * nxyz + P(i1, j1, k1 - n)];
+      }
+    }
+    // smooth_gauss_volume1 (smooth.cu:64-296): half-volume Gaussian average
+    // of the T2/T3 tractions; width 0.7 and side weight 0.4 are the
+    // reference's hard-coded values (smooth.cu:72-73).
+    const real width2 = 0.7 * 0.7, pmax1 = 0.4;
+    real wp1[HW + 1][2 * HW + 1][2 * HW + 1], wp2[HW + 1][2 * HW + 1][2 * HW + 1];
+    real sum1 = 0, sum2 = 0;
+    for (int ii = 0; ii <= HW; ii++)
+      for (int a = 0; a <= 2 * HW; a++)
+        for (int b = 0; b <= 2 * HW; b++) {
+          const real r2 = (real)(ii * ii + (a - HW) * (a - HW) + (b - HW) * (b - HW)) / width2;
+          const real w = std::exp(-r2) * (ii == 0 ? 1.0 : pmax1);
+          wp1[ii][a][b] = w; wp2[ii][a][b] = w;
+          sum1 += w; sum2 += w;
+        }
+    for (int ii = 0; ii <= HW; ii++)
+      for (int a = 0; a <= 2 * HW; a++)
+        for (int b = 0; b <= 2 * HW; b++) { wp1[ii][a][b] /= sum1; wp2[ii][a][b] /= sum2; }
+#pragma omp parallel for
+    for (int k = 0; k < nk; k++)
+      for (int j = 0; j < nj; j++) {
+        const int j1 = j + 3, k1 = k + 3;
+        for (int m = 0; m < 2; m++) {
+          real Ts[6] = {0, 0, 0, 0, 0, 0};
+          auto& wp = m == 0 ? wp1 : wp2;
+          for (int a = 0; a <= 2 * HW; a++)
+            for (int b = 0; b <= 2 * HW; b++) {
+              const long q = (long)(k1 + a - HW) * ny + (j1 + b - HW);
+              for (int c = 0; c < 6; c++) Ts[c] += FW[(3 + c) * 2 * nyz + m * nyz + q] * wp[0][a][b];
+            }
+          for (int l = 1; l <= HW; l++)
+            for (int a = 0; a <= 2 * HW; a++)
+              for (int b = 0; b <= 2 * HW; b++) {
+                const long p = P(m == 0 ? i0 - l : i0 + l, j1 + b - HW, k1 + a - HW);
+                real E[3][3], S[3][3], T[3][3];
+                for (int r = 0; r < 3; r++)
+                  for (int cc = 0; cc < 3; cc++) E[r][cc] = M[(3 * cc + r) * nxyz + p];  // columns grad xi, eta, zeta
+                const real sxx = W[3 * nxyz + p], syy = W[4 * nxyz + p], szz = W[5 * nxyz + p];
+                const real sxy = W[6 * nxyz + p], sxz = W[7 * nxyz + p], syz = W[8 * nxyz + p];
+                const real Sm[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
+                for (int r = 0; r < 3; r++)
+                  for (int cc = 0; cc < 3; cc++) { S[r][cc] = Sm[r][cc]; T[r][cc] = 0; }
+                for (int r = 0; r < 3; r++)
+                  for (int cc = 0; cc < 3; cc++)
+                    for (int q = 0; q < 3; q++) T[r][cc] += S[r][q] * E[q][cc];
+                const real w = M[9 * nxyz + p] * wp[l][a][b];
+                for (int r = 0; r < 3; r++) { Ts[r] += T[r][1] * w; Ts[3 + r] += T[r][2] * w; }
+              }
+          for (int c = 0; c < 6; c++) FWs[(3 + c) * 2 * nyz + m * nyz + (long)k1 * ny + j1] = Ts[c];
+        }
+      }
+    // smooth_gauss_volume2 (smooth.cu:298-345): blend by min(slip/Dc, 1)
+    // inside Fault_grid, the smoothed value outside.
+    for (int k = 0; k < nk; k++)
+      for (int j = 0; j < nj; j++) {
+        const int gj = j + 1, gk = k + 1;
+        real beta = 1.0;
+        if (gj >= Fault_grid[0] && gj <= Fault_grid[1] && gk >= Fault_grid[2] && gk <= Fault_grid[3])
+          beta = std::min(slip[k * nj + j] / Dc, (real)1.0);
+        const long pf = (long)(k + 3) * ny + (j + 3);
+        for (int l = 3; l < 9; l++)
+          for (int m = 0; m < 2; m++) {
+            const long x = (long)l * 2 * nyz + m * nyz + pf;
+            FW[x] = beta * FWs[x] + (1.0 - beta) * FW[x];
+          }
+      }
+  }
+
   void step(int it) {
     static const int FD[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};
     int FX = FD[it % 8][0], FY = FD[it % 8][1], FZ = FD[it % 8][2];
@@ -1459,6 +1556,7 @@ struct Oracle {
       stage(it, irk, FX, FY, FZ);
       FX = -FX; FY = -FY; FZ = -FZ;
     }
+    if (Rupture && FaultSmooth) smooth_gauss();
   }
 
   int setup() {
@@ -1486,7 +1584,7 @@ struct Oracle {
     if (n == "Bx") return &Bx; if (n == "By") return &By; if (n == "Bz") return &Bz;
     if (n == "Ax") return &Ax; if (n == "Ay") return &Ay; if (n == "Az") return &Az;
     if (n == "Dx") return &Dx; if (n == "Dy") return &Dy; if (n == "Dz") return &Dz;
-    if (n == "FW") return &FW; if (n == "FhW") return &FhW; if (n == "FmW") return &FmW;
+    if (n == "FW") return &FW; if (n == "FhW") return &FhW; if (n == "FmW") return &FmW; if (n == "FWs") return &FWs;
     if (n == "T11") return &T11; if (n == "T12") return &T12; if (n == "T13") return &T13;
     if (n == "hT11") return &hT11; if (n == "hT12") return &hT12; if (n == "hT13") return &hT13;
     if (n == "Tn") return &Tn; if (n == "Ts1") return &Ts1; if (n == "Ts2") return &Ts2;
@@ -1566,6 +1664,7 @@ int orc_setup(void* h) { return ((Oracle*)h)->setup(); }
 int orc_step(void* h, int it) { ((Oracle*)h)->step(it); return 0; }
 int orc_stage(void* h, int it, int irk, int fx, int fy, int fz) { ((Oracle*)h)->stage(it, irk, fx, fy, fz); return 0; }
 int orc_rup_sensor(void* h) { ((Oracle*)h)->rup_sensor_calc(); return 0; }
+int orc_smooth(void* h) { ((Oracle*)h)->smooth_gauss(); return 0; }
 // kernel-level entry points for unit parity tests
 int orc_wave_deriv(void* h, int fx, int fy, int fz) { ((Oracle*)h)->wave_deriv(fx, fy, fz); return 0; }
 int orc_pml_deriv(void* h, int axis, int face, int flag) { ((Oracle*)h)->pml_deriv(axis, face, flag); return 0; }

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import cases
-from oracle_ref import Oracle
+from oracle.binding import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -142,6 +142,28 @@ def test_fault_sw_steps(curved):
     t_h, t_o = s.get("init_t0"), o.get("init_t0")
     assert np.array_equal(t_h > -1, t_o > -1)
     assert np.max(np.abs(t_h - t_o)) <= case["params"]["DT"] + 1e-6
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_fault_smooth_steps(curved):
+    """FaultSmooth (smooth_gauss_volume0/1/2, smooth.cu:13-345, once per step
+    after the stages): the surface mirror, the half-volume Gaussian average of
+    the split-node tractions and the slip-weighted blend, on a rupturing
+    slip-weakening fault.  The smoothed tractions FWs after the first step and
+    the split nodes and slip rate after 12 steps are compared with the oracle."""
+    case = cases.sw_case(curved=curved, FaultSmooth=1)
+    s, o = make_pair(case)
+    s.step(0)
+    o.step(0)
+    for name in ("FWs", "FW"):
+        e = relerr(s.get(name), o.get(name))
+        assert e < 2e-5, (name, e)
+    for it in range(1, 12):
+        s.step(it)
+        o.step(it)
+    for name, tol in (("FW", 1e-4), ("Vs1", 1e-3), ("slip", 1e-4), ("Ts1", 1e-4)):
+        e = relerr(s.get(name), o.get(name))
+        assert e < tol, (name, e)
 
 
 def test_fault_rs_steps():

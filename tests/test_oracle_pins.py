@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import cases
-from oracle_ref import Oracle
+from oracle.binding import Oracle
 
 # DRP coefficients as printed in the reference (macdrp.h:23-27)
 C = np.array([0.29164704779069994, 0.66674514216519998, 1.2501177132452999,
@@ -405,3 +405,78 @@ def test_locked_fault_stays_locked():
     assert np.max(np.abs(o.get("slip"))) == 0.0
     assert np.max(np.abs(o.get("FW"))) == 0.0
     assert np.max(np.abs(o.get("W"))) == 0.0
+
+
+def _smooth_oracle(R):
+    NY, NZ = 40, 36
+    p = cases.base_params(NX=20, NY=NY, NZ=NZ, usePML=0, FreeSurface=0, NormalRefine=0, Rupture=1,
+                          INPORT_STRESS_TYPE=1, FaultSmooth=1, Fault_grid=[4, NY - 3, 4, NZ - 3])
+    o = Oracle(p)
+    o.set_coord3d(_rotated_coords(p, R))
+    z = np.zeros((NZ, NY))
+    for k, v in {"Tx": -1e8 + z, "Ty": 5e7 + z, "Tz": z, "dTx": z, "dTy": z, "dTz": z,
+                 "mu_s": 0.6 + z, "mu_d": 0.5 + z, "Dc": 0.4 + z, "C0": z}.items():
+        o.set_stress(k, v)
+    o.setup()
+    return o
+
+
+def test_smooth_uniform_stress_is_fixed_point():
+    """smooth_gauss_volume1 (smooth.cu:64-296) on a rotated affine grid: with
+    a uniform stress in the volume and split-node tractions equal to its
+    conservative tractions J sigma.grad(eta), J sigma.grad(zeta), the
+    normalised half-volume average returns the same tractions on both sides
+    (weights of each side sum to 1; traction = S . (grad eta) column, which a
+    transposed product E.S would break), and the slip-weighted blend leaves
+    them unchanged."""
+    R = _rotation(0.35, -0.2, 0.5)
+    o = _smooth_oracle(R)
+    d = o.dims()
+    nx, ny, nz = d["nx"], d["ny"], d["nz"]
+    sig = np.array([[3.0e7, 1.1e7, -0.7e7], [1.1e7, -2.0e7, 0.4e7], [-0.7e7, 0.4e7, 1.5e7]])
+    W = np.zeros((9, nx, nz, ny))
+    for c, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+        W[3 + c] = sig[a, b]
+    o.put("W", W)
+    Ri = np.linalg.inv(R)  # rows: grad xi, grad eta, grad zeta (metric in grid-spacing units)
+    J = np.linalg.det(R)
+    t2 = J * sig @ Ri[1]
+    t3 = J * sig @ Ri[2]
+    FW = np.zeros((9, 2, nz, ny))
+    for r in range(3):
+        FW[3 + r] = t2[r]
+        FW[6 + r] = t3[r]
+    o.put("FW", FW)
+    M = o.get("M").reshape(13, nx, nz, ny)
+    np.testing.assert_allclose(M[9][3:-3, 3:-3, 3:-3], J, rtol=1e-10)
+    o.smooth()
+    FWs = o.get("FWs").reshape(9, 2, nz, ny)[:, :, 3:-3, 3:-3]
+    for r in range(3):
+        np.testing.assert_allclose(FWs[3 + r], t2[r], rtol=1e-10, atol=1e-6 * abs(t2).max())
+        np.testing.assert_allclose(FWs[6 + r], t3[r], rtol=1e-10, atol=1e-6 * abs(t3).max())
+    FW2 = o.get("FW").reshape(9, 2, nz, ny)[:, :, 3:-3, 3:-3]
+    np.testing.assert_allclose(FW2[3:], FW[3:, :, 3:-3, 3:-3], rtol=1e-10, atol=1e-6 * abs(t2).max())
+
+
+def test_smooth_sides_read_their_own_planes():
+    """The minus-side average reads planes i0-1..-3 only and the plus side
+    i0+1..+3 only (smooth.cu:198, 268): changing the stress on one side moves
+    only that side's smoothed tractions."""
+    o = _smooth_oracle(np.eye(3))
+    d = o.dims()
+    nx, ny, nz, i0 = d["nx"], d["ny"], d["nz"], d["i0"]
+    rng = np.random.default_rng(3)
+    W = rng.standard_normal((9, nx, nz, ny)) * 1e6
+    o.put("W", W)
+    o.put("FW", rng.standard_normal((9, 2, nz, ny)) * 1e5)
+    fw0 = o.get("FW")
+    o.smooth()
+    base = o.get("FWs").reshape(9, 2, nz, ny).copy()
+    W2 = W.copy()
+    W2[3:, i0 + 1:] += 5e6  # perturb the plus side only
+    o.put("W", W2)
+    o.put("FW", fw0)
+    o.smooth()
+    new = o.get("FWs").reshape(9, 2, nz, ny)
+    np.testing.assert_array_equal(new[:, 0], base[:, 0])
+    assert np.abs(new[3:, 1, 3:-3, 3:-3] - base[3:, 1, 3:-3, 3:-3]).max() > 1.0
