@@ -33,12 +33,11 @@ using namespace fd;
 
 namespace {
 
-constexpr int TJ = 64;  // one wavefront per k-row of the tile, j contiguous
-constexpr int TK = 8;
-constexpr int NTHR = TJ * TK;
-constexpr int LJ = TJ + 4, LK = TK + 4;
-constexpr int NH = 4 * LJ + 4 * TK;  // halo cells of one plane: 4 rows + 4 side cells per row
-static_assert(NH <= NTHR, "one halo cell per thread");
+constexpr int NTHR = 512;  // threads per block
+constexpr int TJ = 64;     // default tile: one wavefront per k-row, j contiguous
+constexpr int TK = NTHR / TJ;
+constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j x 16 k)
+constexpr int TKN = NTHR / TJN;
 
 
 // Volume arrays are plane-interleaved with padded rows: word (field c,
@@ -192,8 +191,12 @@ __device__ __forceinline__ float pml_fs_term(int c, const float* z1, float ztx, 
 // code: the host gives it only points where every absorbing profile is
 // trivial), 1 absorbing bands (PML correction, Cerjan), 2 tiles holding the
 // free-surface rows (everything).
-template <int FX, int FY, int FZ, class MT, int KIND>
+template <int FX, int FY, int FZ, class MT, int KIND, int TJ_ = TJ>
 __global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
+  constexpr int TJ = TJ_, TK = NTHR / TJ_;  // tile of TJ (j) x TK (k) columns
+  constexpr int LJ = TJ + 4, LK = TK + 4;   // LDS tile with halos
+  constexpr int NH = 4 * LJ + 4 * TK;       // halo cells of one plane: 4 rows + 4 side cells per row
+  static_assert(NH <= NTHR, "one halo cell per thread");
   constexpr int HJL = FY > 0 ? 1 : 3;    // halo width below j (the stencil reaches HJL back, 4-HJL forward)
   constexpr int HKL = FZ > 0 ? 1 : 3;
   constexpr int QLO = FX > 0 ? -1 : -3;  // queue holds planes i+QLO .. i+QLO+4
@@ -495,23 +498,31 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
 
 template <int FX, int FY, int FZ, class MT>
 void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
-  const int n0 = c.wave_nwork_int, n1 = n0 + c.wave_nwork_edge, n2 = c.wave_nwork;
+  // work list order: interior, bands (64-wide tiles), bands (32-wide tiles), free-surface tiles
+  const int* n = c.wave_nkind;
+  const int o1 = n[0], o2 = o1 + n[1], o3 = o2 + n[2];
   const int4* w = (const int4*)c.wave_work;
   Profiler& pr = const_cast<Ctx&>(c).prof;
-  if (n0 > 0) {
-    a.work = w; a.nwork = n0;
+  if (n[0] > 0) {
+    a.work = w; a.nwork = n[0];
     pr.begin(s, PK_STAGE_INTERIOR);
     hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
     pr.end(s);
   }
-  if (n1 > n0) {
-    a.work = w + 2 * n0; a.nwork = n1 - n0;
+  if (n[1] + n[2] > 0) {
     pr.begin(s, PK_STAGE_BAND);
-    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+    if (n[1] > 0) {
+      a.work = w + 2 * o1; a.nwork = n[1];
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+    }
+    if (n[2] > 0) {
+      a.work = w + 2 * o2; a.nwork = n[2];
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1, TJN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, s, a);
+    }
     pr.end(s);
   }
-  if (n2 > n1) {
-    a.work = w + 2 * n1; a.nwork = n2 - n1;
+  if (n[3] > 0) {
+    a.work = w + 2 * o3; a.nwork = n[3];
     pr.begin(s, PK_STAGE_SURFACE);
     hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
     pr.end(s);
@@ -646,37 +657,48 @@ int upload_wave_cold(Ctx& c) {
   const int ich_in = chunk(ib - ia, tiles[0], 4096, 4);
   const int ich_band = chunk(ni, tiles[1] + tiles[0] * (ni - (ib - ia)) / std::max(ni, 1), 4096, 4);
   const int ich_top = chunk(ni, tiles[2], 2048, 1);
-  // kind 0 interior, 1 absorbing bands, 2 tiles holding free-surface rows
-  std::vector<int4> lists[3];
+  // lists: 0 interior, 1 absorbing bands (64-wide tiles), 2 absorbing bands
+  // of at most 32 columns along j (32 x 16 tiles, half the idle lanes),
+  // 3 tiles holding free-surface rows
+  std::vector<int4> lists[4];
   for (int kd = 0; kd < 3; kd++) c.wave_points[kd] = 0;
-  auto push = [&](int kind, const Seg& a, const Seg& b, int i0, int i1) {
-    lists[kind].push_back(make_int4(a.s, b.s, i0, i1));
-    lists[kind].push_back(make_int4(a.n, b.n, 0, 0));
-    c.wave_points[kind] += (long)a.n * b.n * (i1 - i0);
+  auto push = [&](int list, const Seg& a, const Seg& b, int i0, int i1) {
+    lists[list].push_back(make_int4(a.s, b.s, i0, i1));
+    lists[list].push_back(make_int4(a.n, b.n, 0, 0));
+    c.wave_points[list == 0 ? 0 : list == 3 ? 2 : 1] += (long)a.n * b.n * (i1 - i0);
   };
+  const auto narrow = [&](const Seg& a) { return !a.in && a.n <= TJN; };
+  int ktop = nk;  // first row of the free-surface tiles
+  for (const Seg& b : sk)
+    if (is_top(b)) ktop = std::min(ktop, b.s);
   for (int i0 = ia; i0 < ib; i0 += ich_in)
     for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
   for (int i0 = 0; i0 < ni; i0 += ich_band)
     for (const Seg& b : sk)
       for (const Seg& a : sj) {
         const int i1 = std::min(ni, i0 + ich_band);
-        if (is_top(b)) continue;
+        if (is_top(b) || narrow(a)) continue;
         if (!(a.in && b.in)) { push(1, a, b, i0, i1); continue; }
         // interior tile: only the parts of [i0, i1) outside [ia, ib)
         if (i0 < ia) push(1, a, b, i0, std::min(i1, ia));
         if (i1 > ib) push(1, a, b, std::max(i0, ib), i1);
       }
+  for (int i0 = 0; i0 < ni; i0 += ich_band)
+    for (int k0 = 0; k0 < ktop; k0 += TKN)
+      for (const Seg& a : sj)
+        if (narrow(a)) push(2, a, Seg{k0, std::min(TKN, ktop - k0), false}, i0, std::min(ni, i0 + ich_band));
   for (int i0 = 0; i0 < ni; i0 += ich_top)
     for (const Seg& b : sk)
       for (const Seg& a : sj)
-        if (is_top(b)) push(2, a, b, i0, std::min(ni, i0 + ich_top));
+        if (is_top(b)) push(3, a, b, i0, std::min(ni, i0 + ich_top));
   std::vector<int4> all;
-  for (auto& l : lists) all.insert(all.end(), l.begin(), l.end());
+  for (int l = 0; l < 4; l++) {
+    all.insert(all.end(), lists[l].begin(), lists[l].end());
+    c.wave_nkind[l] = (int)lists[l].size() / 2;
+  }
   if (c.wave_work) { hipFree(c.wave_work); c.wave_work = nullptr; }
   if (hipMalloc(&c.wave_work, all.size() * sizeof(int4)) != hipSuccess) return 1;
   if (hipMemcpy(c.wave_work, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) return 1;
-  c.wave_nwork_int = (int)lists[0].size() / 2;
-  c.wave_nwork_edge = (int)lists[1].size() / 2;
   c.wave_nwork = (int)all.size() / 2;
   return 0;
 }
