@@ -53,7 +53,8 @@ def workload_name(cfg, p):
 
 
 def read_traffic(cfg, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary."""
+    """HBM bytes of `kernel` per RK stage (one launch of the interior
+    class) from the committed rocprofv3 summary."""
     path = TRAFFIC_FILE.format(cfg=cfg)
     if not os.path.exists(path):
         return None
@@ -62,7 +63,7 @@ def read_traffic(cfg, kernel):
         if ":" in line and not line.lstrip().startswith("#"):
             k, v = line.split(":", 1)
             vals[k.strip()] = v.strip()
-    v = vals.get(f"{kernel}_hbm_bytes_per_launch")
+    v = vals.get(f"{kernel}_hbm_bytes_per_stage")
     try:
         return float(v) if v is not None else None
     except ValueError:

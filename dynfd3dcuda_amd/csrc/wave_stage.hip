@@ -444,7 +444,9 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
   const int off = (f & 1) ? nn - N : 0;
   const int pnj = ax == 1 ? N : a.nj, pnk = ax == 2 ? N : a.nk, pni = ax == 0 ? N : a.ni;
   const long ps = (long)pni * pnj * pnk;
-  const long idx = blockIdx.x * 256L + threadIdx.x;
+  // blocks numbered XCD-contiguously: the blocks sharing stencil planes
+  // (neighbouring band planes on x faces, rows on z faces) share one L2
+  const long idx = (long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 256L + threadIdx.x;
   if (idx >= ps) return;
   const int pj = (int)(idx % pnj);
   const long t = idx / pnj;

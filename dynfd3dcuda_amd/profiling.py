@@ -4,11 +4,13 @@
       per kernel class: launches, average / total GPU time (from a
       `rocprofv3 --kernel-trace --output-format csv` run)
   python -m dynfd3dcuda_amd.profiling traffic --fetch D1 --write D2 --cal-fetch D3 --cal-write D4 --bench-json F
-      HBM bytes per launch of each stage-kernel class from `--pmc FETCH_SIZE`
-      and `--pmc WRITE_SIZE` runs, each scaled by what the same counter reads
-      for a device-to-device copy of known size (`bench.py --calibrate`), next
-      to the algorithmic bytes per launch (points per launch from the JSON
-      line bench.py printed in the same configuration)
+      HBM bytes per RK stage of each kernel class (all its dispatches in one
+      stage: the band class has two, the PML class one per absorbing face)
+      from `--pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs, each scaled by what
+      the same counter reads for a device-to-device copy of known size
+      (`bench.py --calibrate`), next to the algorithmic bytes of the stage
+      kernels (points per stage from the JSON line bench.py printed in the
+      same configuration)
 
 Kernel classes: stage_interior / stage_band / stage_surface (the three
 instantiations of the wave stage kernel), pml_aux, fault_pre, fault_deriv,
@@ -24,13 +26,16 @@ import re
 import sys
 
 
+STAGE_CLASSES = ["stage_interior", "stage_band", "stage_surface"]
+
+
 def kernel_class(name):
-    m = re.search(r"wave_stage_kernel<[^>]*,\s*(\d)>", name)
+    m = re.search(r"wave_stage_kernel<([^>]*)>", name)
+    if m:  # template arguments FX, FY, FZ, metric type, KIND[, tile width]
+        return STAGE_CLASSES[int(m.group(1).split(",")[4])]
+    m = re.search(r"wave_stage_kernelI(?:Li-?\d+E|Lin1E){3}[ft]Li([012])E", name)  # mangled form
     if m:
-        return ["stage_interior", "stage_band", "stage_surface"][int(m.group(1))]
-    m = re.search(r"wave_stage_kernelI.*?Li([012])EEEv", name)  # mangled form
-    if m:
-        return ["stage_interior", "stage_band", "stage_surface"][int(m.group(1))]
+        return STAGE_CLASSES[int(m.group(1))]
     for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_deriv_kernel", "fault_post_kernel"):
         if k in name:
             return k[: -len("_kernel")]
@@ -79,22 +84,26 @@ def traffic(a):
     f_write = cal_bytes / (max(cw[copy_k]) * 1024.0)
     fetch = _counter(a.fetch, "FETCH_SIZE")
     write = _counter(a.write, "WRITE_SIZE")
-    bj = json.loads(open(a.bench_json).read().strip().splitlines()[-1])
+    lines = [ln for ln in open(a.bench_json).read().splitlines() if ln.lstrip().startswith("{")]
+    bj = json.loads(lines[-1])
     pts = bj["config"].get("stage_points_per_launch", {})
     ba = bj["config"]["B_alg_bytes_per_pt_step"]
     out = [f"# calibration kernel: {copy_k}; FETCH_SIZE x {f_fetch:.3f}, WRITE_SIZE x {f_write:.3f} (KiB -> bytes x factor)",
            f"fetch_scale: {f_fetch:.4f}", f"write_scale: {f_write:.4f}", f"B_alg_bytes_per_pt_step: {ba}"]
+    # one fault_pre dispatch per RK stage (rupture runs); else the interior stage kernel
+    nst = len(fetch.get("fault_pre", fetch.get("stage_interior", [1])))
+    out.append(f"rk_stages_profiled: {nst}")
     for k in sorted(fetch):
-        if k not in write:
+        if k not in write or k.startswith("__"):
             continue
-        fb = sum(fetch[k]) / len(fetch[k]) * 1024.0 * f_fetch
-        wb = sum(write[k]) / len(write[k]) * 1024.0 * f_write
-        out.append(f"{k}_fetch_bytes_per_launch: {fb:.6g}")
-        out.append(f"{k}_write_bytes_per_launch: {wb:.6g}")
-        out.append(f"{k}_hbm_bytes_per_launch: {fb + wb:.6g}")
+        fb = sum(fetch[k]) / nst * 1024.0 * f_fetch
+        wb = sum(write[k]) / nst * 1024.0 * f_write
+        out.append(f"{k}_fetch_bytes_per_stage: {fb:.6g}")
+        out.append(f"{k}_write_bytes_per_stage: {wb:.6g}")
+        out.append(f"{k}_hbm_bytes_per_stage: {fb + wb:.6g}")
         if k in pts and pts[k]:
             alg = pts[k] * ba / 4.0
-            out.append(f"{k}_alg_bytes_per_launch: {alg:.6g}")
+            out.append(f"{k}_alg_bytes_per_stage: {alg:.6g}")
             out.append(f"{k}_hbm_over_alg: {(fb + wb) / alg:.3f}")
     return "\n".join(out)
 
