@@ -20,7 +20,13 @@ FP32).  Prints one JSON line:
                  present
   cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, same
                  workload, a few steps
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+  rate_state     rate-and-state fault step with plain Newton against the
+                 learned initial guess, same grid with the aging law: the
+                 plain run's Newton samples train the network on the GPU,
+                 then both runs are timed over the same window of steps
+                 inside the rupture (GPU time per RK stage of the Newton
+                 kernel and of the whole fault part, iterations per solve)
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-rs]
 """
 import argparse
 import json
@@ -113,6 +119,64 @@ def cpu_baseline(case, budget_s):
                       f"warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), {cpu_model()}"}
 
 
+def rate_state_leg(args):
+    """Plain Newton (NewtonMode 1) against the learned guess (NewtonMode 2,
+    same residual rule and fallback) on the rate-state configuration."""
+    import numpy as np
+    import dynfd3dcuda_amd as dyn
+    from dynfd3dcuda_amd import configs, mlp
+    lead, window = args.rs_lead, args.rs_steps
+    fault = ("fault_pre", "fault_rs", "fault_deriv", "fault_post")
+    res = {}
+    params = None
+    for mode in (1, 2):
+        case = configs.config2(nt=lead + window + 2)
+        case["params"].update({"NewtonMode": mode, "NewtonTol": 1e-10, "NewtonFallbackIter": 20})
+        s = dyn.Solver(case["params"])
+        configs.apply(case, s)
+        if mode == 2:
+            s.set_newton_mlp(params)
+        s.setup()
+        d = s.dims()
+        t0 = time.perf_counter()
+        if mode == 1:
+            F, T = mlp.collect(s, 0, lead, every=args.rs_sample_every, keep=args.rs_samples)
+        else:
+            for it in range(lead):
+                s.step(it)
+        s.synchronize()
+        t_lead = time.perf_counter() - t0
+        s.put("newton_counts", np.zeros(3 * d["nj"] * d["nk"], np.int32))
+        s.profile(True)
+        for it in range(lead, lead + window):
+            s.step(it)
+        s.synchronize()
+        prof = s.profile_report()
+        s.profile(False)
+        cnt = s.get("newton_counts").reshape(3, -1).astype(np.float64)
+        nst = 4 * window
+        per_node = cnt[1] / np.maximum(cnt[0], 1)
+        res[mode] = {"newton_kernel_ms_per_stage": prof["fault_rs"][0] / nst,
+                     "fault_ms_per_stage": sum(prof[k][0] for k in fault) / nst,
+                     "iters_per_solve": cnt[1].sum() / max(cnt[0].sum(), 1),
+                     "max_node_mean_iters": float(per_node.max()),
+                     "fallbacks": int(cnt[2].sum()),
+                     "sliding_nodes": int(np.sum(np.hypot(s.get("Vs1"), s.get("Vs2")) > 1e-3)),
+                     "lead_seconds": round(t_lead, 1)}
+        if mode == 1:
+            t0 = time.perf_counter()
+            params, hist = mlp.train(F, T, epochs=args.rs_epochs, batch=4096, lr=3e-3, seed=0)
+            res["train"] = {"samples": int(len(T)), "epochs": args.rs_epochs, "seconds": round(time.perf_counter() - t0, 2),
+                            "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
+            fault_nodes = d["nj"] * d["nk"]
+        s.close()
+    pl, ml = res[1], res[2]
+    return {"workload": f"config2 grid with the aging law, {fault_nodes} fault nodes; steps {lead}..{lead + window - 1}",
+            "plain": pl, "learned": ml, "train": res["train"],
+            "newton_kernel_speedup": pl["newton_kernel_ms_per_stage"] / ml["newton_kernel_ms_per_stage"],
+            "fault_step_speedup": pl["fault_ms_per_stage"] / ml["fault_ms_per_stage"]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +186,12 @@ def main():
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline timing budget")
+    ap.add_argument("--no-rs", action="store_true", help="skip the rate-state plain-vs-learned leg")
+    ap.add_argument("--rs-lead", type=int, default=600, help="steps before the rate-state timing window")
+    ap.add_argument("--rs-steps", type=int, default=100, help="rate-state timing window (steps)")
+    ap.add_argument("--rs-sample-every", type=int, default=4, help="training-sample stride (steps)")
+    ap.add_argument("--rs-samples", type=int, default=2_000_000, help="training samples kept")
+    ap.add_argument("--rs-epochs", type=int, default=8, help="training epochs")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3)")
     args = ap.parse_args()
@@ -191,9 +261,12 @@ def main():
                      "points_per_launch": pts, "avg_launch_ms": avg_s * 1e3},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1] > 0},
         "cpu_baseline": None,
+        "rate_state": None,
     }
+    solver.close()
+    if not args.no_rs:
+        out["rate_state"] = rate_state_leg(args)
     if not args.no_cpu:
-        del solver
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

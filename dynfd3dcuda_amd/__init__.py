@@ -30,12 +30,12 @@ EXPORTED = [
     "cgfdm_run", "cgfdm_synchronize", "cgfdm_wave_stage", "cgfdm_wave_rhs", "cgfdm_field_info",
     "cgfdm_get_field", "cgfdm_put_field", "cgfdm_get_dims", "cgfdm_open_outputs",
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
-    "cgfdm_profile_get",
+    "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
 PROFILE_KERNELS = ["stage_interior", "stage_band", "stage_surface", "pml_aux",
-                   "fault_pre", "fault_deriv", "fault_post", "rup_sensor"]
+                   "fault_pre", "fault_deriv", "fault_post", "rup_sensor", "fault_rs"]
 
 
 class CgfdmError(RuntimeError):
@@ -93,6 +93,11 @@ def load_library(path=None):
     lib.cgfdm_profile.argtypes = [c_p, ctypes.c_int]
     lib.cgfdm_profile_get.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+    lib.cgfdm_set_newton_mlp.argtypes = [c_p, c_p, ctypes.c_long]
+    lib.cgfdm_mlp_forward.argtypes = [c_p, c_p, ctypes.c_long, c_p, ctypes.c_int]
+    lib.cgfdm_mlp_grad.argtypes = [c_p, c_p, c_p, ctypes.c_long, c_p, c_p]
+    lib.cgfdm_mlp_train.argtypes = [c_p, c_p, c_p, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_uint, c_p]
     for name in EXPORTED[1:]:
         getattr(lib, name).restype = ctypes.c_int
     if path is None:
@@ -230,6 +235,11 @@ class Solver:
 
     def close_outputs(self):
         self._check(self.lib.cgfdm_close_outputs(self.h))
+
+    def set_newton_mlp(self, params):
+        """Network for NewtonMode 2 (float32 blob of 4555 values)."""
+        a = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._check(self.lib.cgfdm_set_newton_mlp(self.h, a.ctypes.data, a.size))
 
     def newton_stats(self):
         v = [ctypes.c_double() for _ in range(3)]

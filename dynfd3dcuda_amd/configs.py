@@ -55,6 +55,26 @@ def bent_fault_geometry(NY, NZ, DH, max_deg, rng):
     return x, y, z
 
 
+def fault_frame(geom, DH):
+    """Unit normal n and along-strike vector s1 of a vertical fault surface
+    x(j) (geometry x, y, z [NZ][NY]); returns n, s1 as [3][NZ][NY]."""
+    x = geom[0]
+    dxdy = np.gradient(x, axis=1) / DH
+    nrm = np.sqrt(1.0 + dxdy * dxdy)
+    n = np.stack([1.0 / nrm, -dxdy / nrm, np.zeros_like(x)])
+    s1 = np.stack([dxdy / nrm, 1.0 / nrm, np.zeros_like(x)])
+    return n, s1
+
+
+def local_traction(sn, tau, frame):
+    """Global traction vector (Tx, Ty, Tz), the form Fault_init_stress takes
+    (init.cu:904-1022), of normal stress sn and along-strike shear tau (the
+    sign of the reference's flat-fault convention Ty = tau) on the fault
+    frame (n, s1).  On a flat fault this is (sn, tau, 0)."""
+    n, s1 = frame
+    return [sn * n[c] + tau * s1[c] for c in range(3)]
+
+
 def sum_of_gaussians_topography(NX, NY, DH, rng, n_hills=6, hmax=800.0, wmin=2e3, wmax=6e3):
     xg = (np.arange(NX) - NX // 2) * DH
     yg = (np.arange(NY) - NY // 2) * DH
@@ -78,7 +98,7 @@ def three_layer_media():
     return depth, rho, vp, vs
 
 
-def _sw_stress(NY, NZ, DH, rng, mean, sd, corr_m, nuc_km):
+def _sw_stress(NY, NZ, DH, rng, mean, sd, corr_m, nuc_km, frame=None):
     mu_s, sn = 0.677, 120e6
     tau = mean + sd * gaussian_random_field((NZ, NY), corr_m / DH, rng)
     tau = np.minimum(tau, 0.99 * mu_s * sn)  # clipped below static strength
@@ -89,15 +109,18 @@ def _sw_stress(NY, NZ, DH, rng, mean, sd, corr_m, nuc_km):
     patch = (np.abs(jj - jc) <= half) & (np.abs(kk - kc) <= half)
     tau[patch] = 1.01 * mu_s * sn
     z = np.zeros((NZ, NY))
+    if frame is None:
+        frame = (np.stack([1 + z, z, z]), np.stack([z, 1 + z, z]))
+    Tx, Ty, Tz = local_traction(-sn * np.ones((NZ, NY)), -tau, frame)
     return {
-        "Tx": -sn * np.ones((NZ, NY)), "Ty": -tau, "Tz": z.copy(),
+        "Tx": Tx, "Ty": Ty, "Tz": Tz,
         "dTx": z.copy(), "dTy": z.copy(), "dTz": z.copy(),
         "mu_s": mu_s * np.ones((NZ, NY)), "mu_d": 0.525 * np.ones((NZ, NY)),
         "Dc": 0.4 * np.ones((NZ, NY)), "C0": z.copy(),
     }
 
 
-def _rs_stress(NY, NZ, DH, rng, nuc_km=3.0):
+def _rs_stress(NY, NZ, DH, rng, nuc_km=3.0, frame=None):
     sn, tau_mean, tau_sd = 120e6, 75e6, 2e6
     V0, Vini = 1e-6, 1e-12
     a = 0.008 + 0.008 * (0.5 + 0.5 * np.tanh(gaussian_random_field((NZ, NY), 2e3 / DH, rng)))
@@ -111,9 +134,13 @@ def _rs_stress(NY, NZ, DH, rng, nuc_km=3.0):
     R = nuc_km * 1e3
     F = np.where(r < R, np.exp(np.minimum(r * r / np.minimum(r * r - R * R, -1e-9), 0.0)), 0.0)
     z = np.zeros((NZ, NY))
+    if frame is None:
+        frame = (np.stack([1 + z, z, z]), np.stack([z, 1 + z, z]))
+    Tx, Ty, Tz = local_traction(-sn * np.ones((NZ, NY)), -tau, frame)
+    dTx, dTy, dTz = local_traction(z, -25e6 * F, frame)
     return {
-        "Tx": -sn * np.ones((NZ, NY)), "Ty": -tau, "Tz": z.copy(),
-        "dTx": z.copy(), "dTy": -25e6 * F, "dTz": z.copy(),
+        "Tx": Tx, "Ty": Ty, "Tz": Tz,
+        "dTx": dTx, "dTy": dTy, "dTz": dTz,
         "a": a, "b": b, "L": 0.02 * np.ones((NZ, NY)), "Vw": 0.1 * np.ones((NZ, NY)), "State": psi,
     }
 
@@ -154,11 +181,11 @@ def config1(seed=1, nt=4000, scale=1):
     rng = np.random.default_rng(seed)
     NX, NY, NZ, DH, DT = 400 * scale, 300 * scale, 200 * scale, 50.0 / scale, 0.003 / scale
     p = _params(NX, NY, NZ, DH, DT, nt)
-    return {"name": "config1", "params": p,
-            "geom": bent_fault_geometry(NY, NZ, DH, 20.0, rng),
+    geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
+    return {"name": "config1", "params": p, "geom": geom,
             "media1d": three_layer_media(),
             "topo": sum_of_gaussians_topography(NX, NY, DH, rng),
-            "stress": _sw_stress(NY, NZ, DH, rng, 70e6, 3e6, 1e3, 3.0)}
+            "stress": _sw_stress(NY, NZ, DH, rng, 70e6, 3e6, 1e3, 3.0, frame=fault_frame(geom, DH))}
 
 
 def config2(seed=2, nt=4000, scale=1):
@@ -168,7 +195,7 @@ def config2(seed=2, nt=4000, scale=1):
     p = c["params"]
     p.update({"Friction_type": 1, "RS_V0": 1e-6, "RS_f0": 0.6, "RS_Vini": 1e-12, "RS_L": 0.02,
               "RS_fw": 0.2, "smooth_load_T": 1.0})
-    c["stress"] = _rs_stress(p["NY"], p["NZ"], p["DH"], rng)
+    c["stress"] = _rs_stress(p["NY"], p["NZ"], p["DH"], rng, frame=fault_frame(c["geom"], p["DH"]))
     c["name"] = "config2"
     return c
 
@@ -192,11 +219,11 @@ def config4(seed=4, nt=8000):
     p = _params(NX, NY, NZ, DH, DT, nt, Friction_type=1, RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12,
                 RS_L=0.02, RS_fw=0.2, smooth_load_T=1.0, MetricBF16=1, NewtonMode=2,
                 NewtonTol=1e-10, NewtonFallbackIter=20)
-    return {"name": "config4", "params": p,
-            "geom": bent_fault_geometry(NY, NZ, DH, 20.0, rng),
+    geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
+    return {"name": "config4", "params": p, "geom": geom,
             "media1d": three_layer_media(),
             "topo": sum_of_gaussians_topography(NX, NY, DH, rng),
-            "stress": _rs_stress(NY, NZ, DH, rng)}
+            "stress": _rs_stress(NY, NZ, DH, rng, frame=fault_frame(geom, DH))}
 
 
 CONFIGS = {0: config0, 1: config1, 2: config2, 3: config3, 4: config4}

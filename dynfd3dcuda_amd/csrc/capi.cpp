@@ -3,6 +3,7 @@
 // (srcd/main.cu:420-535), field access and output files.
 #include "../../include/cgfdm.h"
 #include "cgfdm_internal.h"
+#include "mlp.h"
 #include "netcdf_classic.h"
 #include <cstdio>
 #include <cstring>
@@ -114,6 +115,10 @@ FieldRef field(Ctx& c, const std::string& n) {
     else if (n == "tTs1") f32(F.tTs1, n2);
     else if (n == "tTs2") f32(F.tTs2, n2);
     else if (n == "Vs1") f32(F.Vs1, n2);
+    // rate-state Newton samples of the last stage: features f0..f3 of the
+    // learned guess and the solved log10 slip rate, [5][nk][nj]
+    else if (n == "RS_samples") f32(F.rs_carryf ? F.rs_carryf + 4 * n2 : nullptr, 5 * n2);
+    else if (n == "newton_counts") i32(F.newton_stats, 3 * n2);  // solves, iterations, fallbacks per node
     else if (n == "Vs2") f32(F.Vs2, n2);
     else if (n == "hslip") f32(F.hslip, n2);
     else if (n == "slip") f64(F.slip, n2);
@@ -225,6 +230,7 @@ int cgfdm_destroy(cgfdm_ctx* h) {
   for (void* p : c.allocations) hipFree(p);
   if (c.hW) hipFree(c.hW);
   if (c.wave_work) hipFree(c.wave_work);
+  if (c.mlp_dev) hipFree(c.mlp_dev);
   delete h;
   return 0;
 }
@@ -293,7 +299,10 @@ int cgfdm_stage(cgfdm_ctx* h, int it, int irk) {
   int fx, fy, fz;
   stage_flags(it, irk, fx, fy, fz);
   StageBufs b = bufs_for(c, irk);
-  if (c.p.Rupture) launch_fault_pre(c, b.Win, it, irk, fx, fy, fz);
+  if (c.p.Rupture) {
+    launch_fault_pre(c, b.Win, it, irk, fx, fy, fz);
+    if (c.p.Friction_type >= 1) launch_fault_rs(c, b.Win, it, irk, fx, fy, fz);
+  }
   launch_wave_stage(c, b, irk, fx, fy, fz, false, nullptr);
   if (c.p.Rupture) {
     launch_fault_deriv(c, b.Win, fx, fy, fz);
@@ -477,13 +486,50 @@ int cgfdm_profile_get(cgfdm_ctx* h, const char* name, double* total_ms, long* la
   return 0;
 }
 
+int cgfdm_set_newton_mlp(cgfdm_ctx* h, const float* params, long n) {
+  if (!h || !params) return fail(CGFDM_EPARAM, "null argument");
+  if (n != kMlpParams) return fail(CGFDM_EPARAM, "the Newton network needs 4555 parameters");
+  Ctx& c = h->c;
+  c.mlp_blob.assign(params, params + n);
+  if (c.setup_done && mlp_upload(c, params, n)) return fail(CGFDM_EHIP, "upload of the Newton network failed");
+  return 0;
+}
+
+int cgfdm_mlp_forward(const float* params, const float* feat, long n, float* out, int mode) {
+  if (!params || n < 0 || (n > 0 && (!feat || !out)) || (mode != 0 && mode != 1)) return fail(CGFDM_EPARAM, "bad argument");
+  std::string err;
+  if (mlp_forward_host(params, feat, n, out, mode, err)) return fail(CGFDM_EHIP, err);
+  return 0;
+}
+
+int cgfdm_mlp_grad(const float* params, const float* feat, const float* target, long n, float* grad, float* loss) {
+  if (!params || !feat || !target || !grad || !loss || n <= 0) return fail(CGFDM_EPARAM, "bad argument");
+  std::string err;
+  if (mlp_grad_host(params, feat, target, n, grad, loss, err)) return fail(CGFDM_EHIP, err);
+  return 0;
+}
+
+int cgfdm_mlp_train(float* params, const float* feat, const float* target, long n, int epochs, int batch, float lr,
+                    unsigned seed, float* loss_hist) {
+  if (!params || !feat || !target || !loss_hist || n <= 0 || n > 0x7fffffffL || epochs <= 0 || batch <= 0 || !(lr > 0))
+    return fail(CGFDM_EPARAM, "bad argument");
+  std::string err;
+  if (mlp_train_host(params, feat, target, n, epochs, batch, lr, seed, loss_hist, err)) return fail(CGFDM_EHIP, err);
+  return 0;
+}
+
 int cgfdm_newton_stats(cgfdm_ctx* h, double* solves, double* iterations, double* fallbacks) {
   if (!h) return fail(CGFDM_EPARAM, "null context");
   Ctx& c = h->c;
-  unsigned long long s[4] = {0, 0, 0, 0};
+  unsigned long long s[3] = {0, 0, 0};
   if (c.setup_done && c.p.Rupture && c.f.newton_stats) {
+    const long n2 = (long)c.d.nj * c.d.nk;
+    std::vector<unsigned int> v(3 * n2);
     hipStreamSynchronize(c.stream);
-    hipMemcpy(s, c.f.newton_stats, sizeof s, hipMemcpyDeviceToHost);
+    if (hipMemcpy(v.data(), c.f.newton_stats, v.size() * sizeof(unsigned int), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(CGFDM_EHIP, "copy failed");
+    for (int t = 0; t < 3; t++)
+      for (long q = 0; q < n2; q++) s[t] += v[t * n2 + q];
   }
   if (solves) *solves = (double)s[0];
   if (iterations) *iterations = (double)s[1];

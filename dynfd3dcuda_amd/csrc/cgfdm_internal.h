@@ -90,8 +90,10 @@ struct FaultDev {
   double *slip, *mslip, *tslip, *State, *mState, *tState;
   float *hslip, *Vs1, *Vs2, *init_t0, *friction, *rsD_y, *rsD_z, *rup_sensor;
   int *init_t0_flag, *rup_index_y, *rup_index_z, *united, *faultgrid, *flag_rup;
-  unsigned long long* newton_stats;  // [3]: solves, iterations, fallbacks
+  unsigned int* newton_stats;        // [3][nk][nj] per-node counts: solves, iterations, fallbacks
   float *Ws;                         // [9][2][nz][ny] smoothed T2*/T3* (FaultSmooth)
+  double *rs_carry;                  // [9][nk][nj] rate-state Newton inputs handed from fault_pre to fault_rs
+  float *rs_carryf;                  // [9][nk][nj] normal (3), jacvec, guess features (4), solved log10 V
   float *smooth_w;                   // [2][4][7][7] half-volume Gaussian weights (FaultSmooth)
 };
 
@@ -107,7 +109,7 @@ struct PmlDev {
 // stream (off unless enabled; events come from a pool created on enable).
 enum ProfKernel {
   PK_STAGE_INTERIOR, PK_STAGE_BAND, PK_STAGE_SURFACE, PK_PML_AUX,
-  PK_FAULT_PRE, PK_FAULT_DERIV, PK_FAULT_POST, PK_RUP_SENSOR, PK_COUNT
+  PK_FAULT_PRE, PK_FAULT_DERIV, PK_FAULT_POST, PK_RUP_SENSOR, PK_FAULT_RS, PK_COUNT
 };
 extern const char* const kProfNames[PK_COUNT];
 struct Profiler {
@@ -161,6 +163,8 @@ struct Ctx {
   int wave_nwork = 0;                  // all work items
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
+  void* mlp_dev = nullptr;          // MlpDev of the learned Newton guess (NewtonMode 2)
+  std::vector<float> mlp_blob;      // its parameters, kept for setup
   Profiler prof;
   PmlDev pml{};
   FaultDev f{};
@@ -185,6 +189,14 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz);
 void launch_fault_post(Ctx& c, const StageBufs& b, int irk);
 void launch_rup_sensor(Ctx& c);
 void launch_fault_smooth(Ctx& c);
+void launch_fault_rs(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz);
+// learned Newton guess (mlp_train.hip)
+int mlp_upload(Ctx& c, const float* blob, long n);  // nonzero on error
+int mlp_forward_host(const float* blob, const float* feat, long n, float* out, int mode, std::string& err);
+int mlp_grad_host(const float* blob, const float* feat, const float* target, long n, float* grad, float* loss,
+                  std::string& err);
+int mlp_train_host(float* blob, const float* feat, const float* target, long n, int epochs, int batch, float lr,
+                   unsigned seed, float* loss_hist, std::string& err);
 void launch_fill(Ctx& c, float* p, long n, float v);
 int upload_wave_cold(Ctx& c);  // nonzero on a HIP error
 

@@ -6,6 +6,7 @@
 //   rup_sensor  = rupture-front sensor (two passes)
 // The volume stencil that precedes them in each stage is in wave_stage.hip.
 #include "fd_ops.h"
+#include "mlp.h"
 #include <hip/hip_bf16.h>
 
 namespace cgfdm {
@@ -40,6 +41,7 @@ struct FArgs {
   int freenode, friction, stress_type, cerjan;
   int G[4];        // Fault_grid
   float Dc0;       // scalar Dc of the JSON (smooth.cu:318)
+  const MlpDev* mlp;  // learned Newton guess, or null
   float mu_d, viscosity, smooth_load_T, Vini;
   double f0, V0, fw;
   int newton_mode, newton_fallback;
@@ -119,6 +121,78 @@ __device__ int nr_search(double x, double psi, double A, double Vhat, double coe
   }
   *xnew = x;
   return iter;
+}
+
+// Shared end of the friction trial (trial_slipweakening.cu:640-709,
+// trial_ratestate.cu:800-880): T1 on the fault plane with the optional
+// viscosity term, the stage traction averages, rupture time, then
+// fault2wave (transform.cu:224-403) into the stage input at i0.
+__device__ __forceinline__ void fault_tail(const FArgs& a, const FaultDev& F, long q, long pf, long pc,
+                                           const float* Tau, float Tau_n, bool ifchange, float ratecheck,
+                                           const float* Trial, float jacvec, const float* s1, const float* s2) {
+  const long nyz = a.nyz, pyz = a.pyz, wps = a.wps, mps = a.mps;
+  const int i0 = a.i0, irk = a.irk;
+  const float DT = a.DT;
+  long p1 = 3 * nyz + pf;
+  float T1[3];
+  if (ifchange) {
+    T1[0] = (Tau[0] - F.str_init_x[q]) * jacvec;
+    T1[1] = (Tau[1] - F.str_init_y[q]) * jacvec;
+    T1[2] = (Tau[2] - F.str_init_z[q]) * jacvec;
+  } else {
+    T1[0] = Trial[0]; T1[1] = Trial[1]; T1[2] = Trial[2];
+  }
+  float vis = a.viscosity * DT;
+  float h1 = (T1[0] - F.mT11[pf]) / DT, h2 = (T1[1] - F.mT12[pf]) / DT, h3 = (T1[2] - F.mT13[pf]) / DT;
+  if (irk == 3) { F.hT11[pf] = h1; F.hT12[pf] = h2; F.hT13[pf] = h3; }
+  T1[0] += vis * h1; T1[1] += vis * h2; T1[2] += vis * h3;
+  F.T11[p1] = T1[0]; F.T12[p1] = T1[1]; F.T13[p1] = T1[2];
+  F.tTs1[q] = Tau[0] * s1[0] + Tau[1] * s1[1] + Tau[2] * s1[2];
+  F.tTs2[q] = Tau[0] * s2[0] + Tau[1] * s2[1] + Tau[2] * s2[2];
+  F.tTn[q] = Tau_n;
+  if (!F.init_t0_flag[q] && ratecheck > 1e-3f) {
+    F.init_t0[q] = a.it * DT;
+    F.init_t0_flag[q] = 1;
+    F.flag_rup[q] = 1;
+  }
+
+  // ---- fault2wave into the stage input at i0
+  {
+    const long p = pc + i0 * mps, pw = pc + i0 * wps;
+    float mt[3][3] = {{LM(a, p), LM(a, 3 * pyz + p), LM(a, 6 * pyz + p)},
+                      {LM(a, pyz + p), LM(a, 4 * pyz + p), LM(a, 7 * pyz + p)},
+                      {LM(a, 2 * pyz + p), LM(a, 5 * pyz + p), LM(a, 8 * pyz + p)}};
+    float rj = 1.0f / LM(a, 9 * pyz + p);
+    float inv[3][3];
+    inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
+    inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
+    inv[0][2] = mt[0][1] * mt[1][2] - mt[0][2] * mt[1][1];
+    inv[1][0] = mt[1][2] * mt[2][0] - mt[1][0] * mt[2][2];
+    inv[1][1] = mt[0][0] * mt[2][2] - mt[2][0] * mt[0][2];
+    inv[1][2] = mt[1][0] * mt[0][2] - mt[0][0] * mt[1][2];
+    inv[2][0] = mt[1][0] * mt[2][1] - mt[1][1] * mt[2][0];
+    inv[2][1] = mt[2][0] * mt[0][1] - mt[0][0] * mt[2][1];
+    inv[2][2] = mt[0][0] * mt[1][1] - mt[0][1] * mt[1][0];
+    float det = 1.0f / (inv[0][0] * mt[0][0] + inv[0][1] * mt[1][0] + inv[0][2] * mt[2][0]);
+    float tr[3][3];
+    for (int c = 0; c < 3; c++) {
+      tr[c][0] = T1[c];
+      tr[c][1] = (F.W[(3 + c) * 2 * nyz + pf] + F.W[(3 + c) * 2 * nyz + nyz + pf]) * 0.5f;
+      tr[c][2] = (F.W[(6 + c) * 2 * nyz + pf] + F.W[(6 + c) * 2 * nyz + nyz + pf]) * 0.5f;
+    }
+    float sg[3][3];
+    for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 3; cc++)
+        sg[r][cc] = (tr[r][0] * inv[0][cc] + tr[r][1] * inv[1][cc] + tr[r][2] * inv[2][cc]) * det * rj;
+    float* W = a.Wi0;
+    for (int c = 0; c < 3; c++) W[c * pyz + pw] = (F.W[c * 2 * nyz + pf] + F.W[c * 2 * nyz + nyz + pf]) * 0.5f;
+    W[3 * pyz + pw] = sg[0][0];
+    W[4 * pyz + pw] = sg[1][1];
+    W[5 * pyz + pw] = sg[2][2];
+    W[6 * pyz + pw] = (sg[0][1] + sg[1][0]) * 0.5f;
+    W[7 * pyz + pw] = (sg[0][2] + sg[2][0]) * 0.5f;
+    W[8 * pyz + pw] = (sg[1][2] + sg[2][1]) * 0.5f;
+  }
 }
 
 // fault_pre: state_rk (rk4.cu:791-938) + trial_sw (trial_slipweakening.cu:23-732)
@@ -294,87 +368,83 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     double Ttn = sqrt(Tt[1] * Tt[1] + Tt[2] * Tt[2]);
     double Rn = sqrt(Rl[1] * Rl[1] + Rl[2] * Rl[2]);
     const double RS_V0 = 1e-6;
-    double x_plain = asinh(exp(st / RS_a) * Rn / (2.0 * RS_V0));
-    double x0 = x_plain;
-    double xnew;
-    int fb;
-    int iters = nr_search(x0, st, RS_a, Vtn, coef, a.newton_mode, a.newton_tol, a.newton_fallback, x_plain, &xnew, &fb);
-    atomicAdd(&F.newton_stats[0], 1ULL);
-    atomicAdd(&F.newton_stats[1], (unsigned long long)iters);
-    if (fb) atomicAdd(&F.newton_stats[2], 1ULL);
-    float V = (float)(2.0 * RS_V0 * exp(-st / RS_a) * sinh(xnew));
-    double Tm = RS_a * xnew * fabs(Tau_nd);
-    F.friction[q] = (float)(RS_a * xnew);
-    Ttn = fmax(Ttn, 1e-8);
-    double Ts1 = Tt[1] / Ttn * Tm, Ts2 = Tt[2] / Ttn * Tm;
-    V = fmaxf(V, 1e-30f);
-    for (int c = 0; c < 3; c++) Tau[c] = (float)(Tau_nd * dn[c] + Ts1 * ds1[c] + Ts2 * ds2[c]);
-    Tau[0] = F.str_init_x[q];  // trial_ratestate.cu:797
-    Tau_n = Tau[0] * n[0] + Tau[1] * n[1] + Tau[2] * n[2];
-    ifchange = true;
-    ratecheck = V;
+    const double x_plain = asinh(exp(st / RS_a) * Rn / (2.0 * RS_V0));
+    // The Newton solve and the rest of the trial run in fault_rs_kernel:
+    // hand over its inputs, what the trial needs after it, and the features
+    // of the learned initial guess (Newton unknown x = friction / a).
+    const long n2 = (long)a.nj * a.nk;
+    double* C = F.rs_carry;
+    C[q] = st; C[n2 + q] = RS_a; C[2 * n2 + q] = Vtn; C[3 * n2 + q] = coef; C[4 * n2 + q] = x_plain;
+    C[5 * n2 + q] = Tau_nd; C[6 * n2 + q] = Tt[1]; C[7 * n2 + q] = Tt[2]; C[8 * n2 + q] = Ttn;
+    float* Cf = F.rs_carryf;
+    Cf[q] = n[0]; Cf[n2 + q] = n[1]; Cf[2 * n2 + q] = n[2]; Cf[3 * n2 + q] = jacvec;
+    // x* of f(x) = 2 V0 e^(-psi/a) sinh x + coef x - Vt = 0 depends on the
+    // two numbers e = ln(2 V0 / coef) - psi/a and Vt/coef (features
+    // T(e + Vt/coef) and Vt/coef, T(s) = s for s <= 0, ln(1 + s) above);
+    // log10 V* adds log10 coef.  The fourth is the log previous slip rate.
+    const double s_rs = log(2.0 * RS_V0 / coef) - st / RS_a + Vtn / coef;
+    Cf[4 * n2 + q] = (float)(s_rs <= 0.0 ? s_rs : log1p(s_rs));
+    Cf[5 * n2 + q] = (float)(Vtn / coef);
+    Cf[6 * n2 + q] = (float)log10(coef);
+    Cf[7 * n2 + q] = (float)log10(fmax(Rn, 1e-30));
+    return;
   }
-  // ---- shared tail (trial_slipweakening.cu:640-709)
-  long p1 = 3 * nyz + pf;
-  float T1[3];
-  if (ifchange) {
-    T1[0] = (Tau[0] - F.str_init_x[q]) * jacvec;
-    T1[1] = (Tau[1] - F.str_init_y[q]) * jacvec;
-    T1[2] = (Tau[2] - F.str_init_z[q]) * jacvec;
-  } else {
-    T1[0] = Trial[0]; T1[1] = Trial[1]; T1[2] = Trial[2];
-  }
-  float vis = a.viscosity * DT;
-  float h1 = (T1[0] - F.mT11[pf]) / DT, h2 = (T1[1] - F.mT12[pf]) / DT, h3 = (T1[2] - F.mT13[pf]) / DT;
-  if (irk == 3) { F.hT11[pf] = h1; F.hT12[pf] = h2; F.hT13[pf] = h3; }
-  T1[0] += vis * h1; T1[1] += vis * h2; T1[2] += vis * h3;
-  F.T11[p1] = T1[0]; F.T12[p1] = T1[1]; F.T13[p1] = T1[2];
-  F.tTs1[q] = Tau[0] * s1[0] + Tau[1] * s1[1] + Tau[2] * s1[2];
-  F.tTs2[q] = Tau[0] * s2[0] + Tau[1] * s2[1] + Tau[2] * s2[2];
-  F.tTn[q] = Tau_n;
-  if (!F.init_t0_flag[q] && ratecheck > 1e-3f) {
-    F.init_t0[q] = a.it * DT;
-    F.init_t0_flag[q] = 1;
-    F.flag_rup[q] = 1;
-  }
+  fault_tail(a, F, q, pf, pc, Tau, Tau_n, ifchange, ratecheck, Trial, jacvec, s1, s2);
+}
 
-  // ---- fault2wave into the stage input at i0
-  {
-    const long p = pc + i0 * mps, pw = pc + i0 * wps;
-    float mt[3][3] = {{LM(a, p), LM(a, 3 * pyz + p), LM(a, 6 * pyz + p)},
-                      {LM(a, pyz + p), LM(a, 4 * pyz + p), LM(a, 7 * pyz + p)},
-                      {LM(a, 2 * pyz + p), LM(a, 5 * pyz + p), LM(a, 8 * pyz + p)}};
-    float rj = 1.0f / LM(a, 9 * pyz + p);
-    float inv[3][3];
-    inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
-    inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
-    inv[0][2] = mt[0][1] * mt[1][2] - mt[0][2] * mt[1][1];
-    inv[1][0] = mt[1][2] * mt[2][0] - mt[1][0] * mt[2][2];
-    inv[1][1] = mt[0][0] * mt[2][2] - mt[2][0] * mt[0][2];
-    inv[1][2] = mt[1][0] * mt[0][2] - mt[0][0] * mt[1][2];
-    inv[2][0] = mt[1][0] * mt[2][1] - mt[1][1] * mt[2][0];
-    inv[2][1] = mt[2][0] * mt[0][1] - mt[0][0] * mt[2][1];
-    inv[2][2] = mt[0][0] * mt[1][1] - mt[0][1] * mt[1][0];
-    float det = 1.0f / (inv[0][0] * mt[0][0] + inv[0][1] * mt[1][0] + inv[0][2] * mt[2][0]);
-    float tr[3][3];
-    for (int c = 0; c < 3; c++) {
-      tr[c][0] = T1[c];
-      tr[c][1] = (F.W[(3 + c) * 2 * nyz + pf] + F.W[(3 + c) * 2 * nyz + nyz + pf]) * 0.5f;
-      tr[c][2] = (F.W[(6 + c) * 2 * nyz + pf] + F.W[(6 + c) * 2 * nyz + nyz + pf]) * 0.5f;
-    }
-    float sg[3][3];
-    for (int r = 0; r < 3; r++)
-      for (int cc = 0; cc < 3; cc++)
-        sg[r][cc] = (tr[r][0] * inv[0][cc] + tr[r][1] * inv[1][cc] + tr[r][2] * inv[2][cc]) * det * rj;
-    float* W = a.Wi0;
-    for (int c = 0; c < 3; c++) W[c * pyz + pw] = (F.W[c * 2 * nyz + pf] + F.W[c * 2 * nyz + nyz + pf]) * 0.5f;
-    W[3 * pyz + pw] = sg[0][0];
-    W[4 * pyz + pw] = sg[1][1];
-    W[5 * pyz + pw] = sg[2][2];
-    W[6 * pyz + pw] = (sg[0][1] + sg[1][0]) * 0.5f;
-    W[7 * pyz + pw] = (sg[0][2] + sg[2][0]) * 0.5f;
-    W[8 * pyz + pw] = (sg[1][2] + sg[2][1]) * 0.5f;
+// Rate-and-state trial after the hand-over from fault_pre_kernel
+// (trial_ratestate.cu:580-800 and NRsearch, trial_ratestate.cu:20-57): FP64
+// Newton for x = friction / a from the plain guess asinh(e^(psi/a) V/(2 V0))
+// (or the learned guess when one is loaded), then shear traction along the
+// trial direction, Tau[0] = str_init_x (trial_ratestate.cu:797) and the
+// shared tail.  Every lane of a wavefront stays active up to the guess, which
+// evaluates the network with matrix-core instructions for the whole wave.
+__global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y * blockDim.y + threadIdx.y;
+  const bool inrange = j < a.nj && k < a.nk;
+  const int jc = min(j, a.nj - 1), kc = min(k, a.nk - 1);
+  const FaultDev& F = a.F;
+  const long q = jc + (long)kc * a.nj, n2 = (long)a.nj * a.nk;
+  const bool valid = inrange && !F.united[q];
+  const double* C = F.rs_carry;
+  const float* Cf = F.rs_carryf;
+  const double st = C[q], RS_a = C[n2 + q], Vtn = C[2 * n2 + q], coef = C[3 * n2 + q], x_plain = C[4 * n2 + q];
+  double x0 = x_plain;
+  if (a.mlp) {  // uniform: every lane of the wave takes part
+    float feat[4];
+    for (int f = 0; f < 4; f++) feat[f] = valid ? Cf[(4 + f) * n2 + q] : 0.0f;
+    const float lv = mlp_guess_wave(a.mlp, feat);  // log10 of the predicted slip rate
+    x0 = choose_guess(lv, x_plain, st, RS_a, Vtn, coef);
   }
+  if (!valid) return;
+  const double Tau_nd = C[5 * n2 + q], Tt1 = C[6 * n2 + q], Tt2 = C[7 * n2 + q];
+  double Ttn = C[8 * n2 + q];
+  const float n[3] = {Cf[q], Cf[n2 + q], Cf[2 * n2 + q]};
+  const float jacvec = Cf[3 * n2 + q];
+  const int j1 = j + 3, k1 = k + 3;
+  const long pf = j1 + (long)k1 * a.ny, pv = pf * 3, pc = a.jpad + j1 + (long)k1 * a.py;
+  const float s1[3] = {F.vec_s1[pv], F.vec_s1[pv + 1], F.vec_s1[pv + 2]};
+  const float s2[3] = {F.vec_s2[pv], F.vec_s2[pv + 1], F.vec_s2[pv + 2]};
+  double xnew;
+  int fb;
+  const int iters = nr_search(x0, st, RS_a, Vtn, coef, a.newton_mode, a.newton_tol, a.newton_fallback, x_plain, &xnew, &fb);
+  F.newton_stats[q] += 1u;  // per-node counters: no contention between nodes
+  F.newton_stats[n2 + q] += (unsigned)iters;
+  F.newton_stats[2 * n2 + q] += fb ? 1u : 0u;
+  const double RS_V0 = 1e-6;
+  float V = (float)(2.0 * RS_V0 * exp(-st / RS_a) * sinh(xnew));
+  const double Tm = RS_a * xnew * fabs(Tau_nd);
+  F.friction[q] = (float)(RS_a * xnew);
+  Ttn = fmax(Ttn, 1e-8);
+  const double Ts1 = Tt1 / Ttn * Tm, Ts2 = Tt2 / Ttn * Tm;
+  V = fmaxf(V, 1e-30f);
+  float Tau[3];
+  for (int c = 0; c < 3; c++) Tau[c] = (float)(Tau_nd * (double)n[c] + Ts1 * (double)s1[c] + Ts2 * (double)s2[c]);
+  Tau[0] = F.str_init_x[q];  // trial_ratestate.cu:797
+  const float Tau_n = Tau[0] * n[0] + Tau[1] * n[1] + Tau[2] * n[2];
+  const float Trial[3] = {0.0f, 0.0f, 0.0f};
+  fault_tail(a, F, q, pf, pc, Tau, Tau_n, true, V, Trial, jacvec, s1, s2);
 }
 
 __device__ __forceinline__ void mv3(const float* A, const float* x, float* y) {
@@ -803,6 +873,7 @@ FArgs make_fargs(Ctx& c) {
   a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = 13 * c.d.pyz;
   a.py = c.d.py; a.jpad = c.d.jpad;
   for (int q = 0; q < 4; q++) a.G[q] = c.p.Fault_grid[q];
+  a.mlp = c.p.NewtonMode == 2 ? (const MlpDev*)c.mlp_dev : nullptr;
   a.Dc0 = (float)c.p.Dc;
   a.NY = c.p.NY; a.NZ = c.p.NZ;
   a.DH = (float)c.p.DH; a.rDH = (float)(1.0 / c.p.DH); a.DT = (float)c.p.DT; a.DTd = c.p.DT;
@@ -821,12 +892,26 @@ dim3 fgrid(const Ctx& c) { return dim3((c.d.nj + 63) / 64, (c.d.nk + 3) / 4, 1);
 
 }  // namespace
 
-void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz) {
+// Arguments of the trial kernels of one stage: the stage input Win (read, and
+// written at plane i0 by fault2wave), the step and stage, the FD flags.
+static FArgs trial_args(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz) {
   FArgs a = make_fargs(c);
   a.W = Win; a.Wi0 = const_cast<float*>(Win);
   a.it = it; a.irk = irk; a.FX = fx; a.FY = fy; a.FZ = fz;
+  return a;
+}
+
+void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz) {
+  FArgs a = trial_args(c, Win, it, irk, fx, fy, fz);
   c.prof.begin(c.stream, PK_FAULT_PRE);
   hipLaunchKernelGGL(fault_pre_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
+  c.prof.end(c.stream);
+}
+
+void launch_fault_rs(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz) {
+  FArgs a = trial_args(c, Win, it, irk, fx, fy, fz);
+  c.prof.begin(c.stream, PK_FAULT_RS);
+  hipLaunchKernelGGL(fault_rs_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 

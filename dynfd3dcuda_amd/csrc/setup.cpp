@@ -7,6 +7,7 @@
 // nc_read_init_stress (init.cu:904-1022) and init_wave_free_cu
 // (fault_coef.cu:1320-1486).  Setup runs once and is not on the timed path.
 #include "cgfdm_internal.h"
+#include "mlp.h"
 #include "netcdf_classic.h"
 #include <algorithm>
 #include <cmath>
@@ -588,7 +589,26 @@ int host_setup(Ctx& c, std::string& err) {
     F.friction = dmalloc<float>(c, n2); F.rsD_y = dmalloc<float>(c, n2); F.rsD_z = dmalloc<float>(c, n2); F.rup_sensor = dmalloc<float>(c, n2);
     F.init_t0_flag = dmalloc<int>(c, n2); F.flag_rup = dmalloc<int>(c, n2);
     UPI(rup_index_y, riy); UPI(rup_index_z, riz); UPI(united, uni); UPI(faultgrid, fg);
-    F.newton_stats = dmalloc<unsigned long long>(c, 4);
+    F.newton_stats = dmalloc<unsigned int>(c, 3 * n2);
+    if (p.Friction_type >= 1) {
+      F.rs_carry = dmalloc<double>(c, 9 * n2);
+      F.rs_carryf = dmalloc<float>(c, 9 * n2);
+      if (!F.rs_carry || !F.rs_carryf) { err = "hipMalloc failed for the rate-state hand-over"; return 6; }
+    }
+    if (p.NewtonMode == 2) {
+      if (p.Friction_type < 1) { err = "NewtonMode 2 needs rate-and-state friction"; return 1; }
+      if (c.mlp_blob.empty() && !p.NewtonMLP.empty()) {
+        FILE* fp = std::fopen(p.NewtonMLP.c_str(), "rb");
+        if (!fp) { err = "cannot open NewtonMLP " + p.NewtonMLP; return 2; }
+        c.mlp_blob.resize(kMlpParams + 1);
+        const size_t got = std::fread(c.mlp_blob.data(), sizeof(float), kMlpParams + 1, fp);
+        std::fclose(fp);
+        if (got != (size_t)kMlpParams) { err = "NewtonMLP " + p.NewtonMLP + " does not hold 4555 float32 values"; return 1; }
+        c.mlp_blob.resize(kMlpParams);
+      }
+      if (c.mlp_blob.empty()) { err = "NewtonMode 2 needs network weights (NewtonMLP or cgfdm_set_newton_mlp)"; return 1; }
+      if (mlp_upload(c, c.mlp_blob.data(), (long)c.mlp_blob.size())) { err = "hipMalloc failed for the Newton network"; return 6; }
+    }
     if (p.FaultSmooth) {
       // half-volume Gaussian weights of smooth_gauss_volume1 (smooth.cu:70-120):
       // exp(-r^2/0.7^2), off-plane layers x 0.4, each side normalised to 1

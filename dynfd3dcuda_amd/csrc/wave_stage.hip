@@ -604,7 +604,7 @@ std::vector<Seg> axis_segments(int n, int lo, int hi, int t) {
 }  // namespace
 
 const char* const kProfNames[PK_COUNT] = {"stage_interior", "stage_band", "stage_surface", "pml_aux",
-                                          "fault_pre", "fault_deriv", "fault_post", "rup_sensor"};
+                                          "fault_pre", "fault_deriv", "fault_post", "rup_sensor", "fault_rs"};
 
 int interior_lo_j(const Params& p) {
   return std::max(p.usePML ? p.PML_N : 0, p.useCerjan ? p.DAMP_N : 0);

@@ -132,6 +132,34 @@ int cgfdm_newton_stats(cgfdm_ctx* ctx, double* solves, double* iterations, doubl
 int cgfdm_profile(cgfdm_ctx* ctx, int enable);
 int cgfdm_profile_get(cgfdm_ctx* ctx, const char* name, double* total_ms, long* launches, long* points_per_launch);
 
+/* Learned initial guess of the rate-and-state Newton solve (extension, no
+ * counterpart in the reference).  With NewtonMode 2 the solve of
+ * trial_ratestate.cu:686-702 starts from the better (smaller residual) of the
+ * plain guess x0 = asinh(e^(psi/a) V/(2 V0)) and a 4-64-64-1 network's
+ * prediction of log10 V, and stops on the residual rule of NewtonMode 1 with
+ * the same fallback.  The network is a float32 parameter blob of
+ * CGFDM_MLP_PARAMS values:
+ *   xmean[4] xscale[4] ymean yscale W1[64][4] b1[64] W2[64][64] b2[64] w3[64] b3
+ * Its inputs per node are the four guess features (field "RS_samples" planes
+ * 0-3), its output log10 of the slip rate (plane 4). */
+#define CGFDM_MLP_PARAMS 4555
+/* Network used by NewtonMode 2 (before or after setup; replaces the file
+ * named by the JSON key NewtonMLP). */
+int cgfdm_set_newton_mlp(cgfdm_ctx* ctx, const float* params, long n);
+/* Evaluate the network on the GPU for n feature rows feat[n][4] (host
+ * arrays): mode 0 with the wavefront matrix-instruction kernel the solver
+ * uses, mode 1 with the scalar kernel. */
+int cgfdm_mlp_forward(const float* params, const float* feat, long n, float* out, int mode);
+/* Gradient (CGFDM_MLP_PARAMS floats, zero for the standardisation entries)
+ * and value of the loss mean_s (o_s - (t_s - ymean)/yscale)^2 on the GPU. */
+int cgfdm_mlp_grad(const float* params, const float* feat, const float* target, long n, float* grad, float* loss);
+/* Train in place on the GPU with Adam (step size decaying from lr to lr/20
+ * over the epochs on a cosine, minibatches of `batch` samples in a seeded
+ * random order per epoch); loss_hist[e] is the mean loss over epoch e's
+ * batches. */
+int cgfdm_mlp_train(float* params, const float* feat, const float* target, long n, int epochs, int batch, float lr,
+                    unsigned seed, float* loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

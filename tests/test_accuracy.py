@@ -1,0 +1,72 @@
+"""Output accuracy of the HIP solver against the FP64 oracle over whole
+rupture runs (tests/golden/accuracy_<case>.npz, written by
+tests/golden/make_accuracy_fixture.py from the oracle on the same seeded
+inputs).
+
+Reported and bounded per case:
+* final slip: relative L2 error <= 1e-3;
+* rupture time (init_t0, quantised to DT by the reference's rule
+  trial_slipweakening.cu:703-709): RMS difference over the nodes ruptured in
+  both runs <= 1e-3 of the run's duration, and the ruptured sets agree on
+  >= 99.9% of the nodes;
+* peak slip rate: relative L2 <= 1e-2 (a maximum over time of a quantity with
+  steep fronts);
+* surface seismograms (Vx, Vy, Vz at five receivers): relative L2 <= 1e-2.
+FP32 rounding of the wavefield is the only difference between the runs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import accuracy_cases as ac
+import dynfd3dcuda_amd as dyn
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, float) - np.asarray(b, float)) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(ac.CASES))
+def test_rupture_outputs_match_oracle(name):
+    path = os.path.join(HERE, "golden", f"accuracy_{name}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {path} not generated")
+    ref = np.load(path)
+    case = ac.CASES[name]()
+    s = dyn.Solver(case["params"])
+    ac.apply_to_solver(case, s)
+    s.setup()
+    d = s.dims()
+    rec = [tuple(r) for r in ref["receivers"]]
+    nt = int(ref["nt"])
+    seis = np.zeros((nt, len(rec), 3))
+    peak = np.zeros(d["nj"] * d["nk"])
+    for it in range(nt):
+        s.step(it)
+        seis[it] = ac.sample_receivers(s.get("W"), d, rec)
+        peak = np.maximum(peak, np.hypot(s.get("Vs1"), s.get("Vs2")))
+    slip, t0 = s.get("slip"), s.get("init_t0")
+    dur = nt * case["params"]["DT"]
+    both = (t0 > 0) & (ref["init_t0"] > 0)
+    either = (t0 > 0) | (ref["init_t0"] > 0)
+    report = {
+        "case": name, "steps": nt,
+        "slip_rel_l2": rel(slip, ref["slip"]),
+        "rupture_time_rms_over_duration": float(np.sqrt(np.mean((t0[both] - ref["init_t0"][both]) ** 2)) / dur)
+        if both.any() else 0.0,
+        "ruptured_nodes": int(both.sum()), "ruptured_set_agreement": float(both.sum() / max(either.sum(), 1)),
+        "peak_rate_rel_l2": rel(peak, ref["peak_rate"]),
+        "seismogram_rel_l2": rel(seis, ref["seis"]),
+    }
+    print(json.dumps(report))
+    assert report["ruptured_nodes"] > 100
+    assert report["slip_rel_l2"] <= 1e-3, report
+    assert report["rupture_time_rms_over_duration"] <= 1e-3, report
+    assert report["ruptured_set_agreement"] >= 0.999, report
+    assert report["peak_rate_rel_l2"] <= 1e-2, report
+    assert report["seismogram_rel_l2"] <= 1e-2, report

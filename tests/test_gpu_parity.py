@@ -192,3 +192,26 @@ def test_fault_rs_steps():
     for name, tol in (("Vs1", 1e-3), ("Ts1", 1e-4), ("Tn", 1e-4), ("slip", 1e-3), ("State", 1e-4)):
         e = relerr(s.get(name), o.get(name))
         assert e < tol, (name, e)
+
+
+def test_metric_bf16_storage():
+    """MetricBF16=1 stores the 13 metric/media words in bfloat16 and computes
+    in FP32.  The stored words are the bfloat16 of the FP64 setup values to
+    one rounding (|rel| <= 2^-8), and a 40-step slip-weakening rupture on a
+    curved grid stays close to the FP64 oracle: the per-word rounding (up to
+    3.9e-3 relative) bounds how close; the measured errors are printed."""
+    case = cases.sw_case(curved=True, MetricBF16=1)
+    s, o = make_pair(case)
+    d = o.dims()
+    raw = s.get("M_bf16").astype(np.uint32) << 16
+    mb = raw.view(np.float32).astype(np.float64).reshape(13, d["nx"], d["nz"], d["ny"])
+    mo = o.get("M").reshape(13, d["nx"], d["nz"], d["ny"])
+    sl = (slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    rel = np.abs(mb[sl] - mo[sl]) / np.maximum(np.abs(mo[sl]), 1e-30)
+    assert rel.max() <= 2.0 ** -8 * 1.0001, rel.max()
+    for it in range(40):
+        s.step(it)
+        o.step(it)
+    errs = {n: relerr(s.get(n), o.get(n)) for n in ("slip", "Vs1", "Ts1", "Tn")}
+    print("bf16 metric, 40 steps, rel L2 vs FP64 oracle:", errs)
+    assert errs["slip"] < 2e-2 and errs["Ts1"] < 1e-2 and errs["Tn"] < 1e-2, errs
