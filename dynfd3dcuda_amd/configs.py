@@ -120,16 +120,45 @@ def _sw_stress(NY, NZ, DH, rng, mean, sd, corr_m, nuc_km, frame=None):
     }
 
 
-def _rs_stress(NY, NZ, DH, rng, nuc_km=3.0, frame=None):
+def _boxcar(x, W, w):
+    """Smooth boxcar of the SCEC rate-state benchmarks: 1 for |x| <= W,
+    0.5 (1 + tanh(w/(|x|-W-w) + w/(|x|-W))) for W < |x| < W + w, 0 beyond."""
+    x = np.abs(x)
+    out = np.zeros_like(x, dtype=np.float64)
+    out[x <= W] = 1.0
+    m = (x > W) & (x < W + w)
+    xm = x[m]
+    out[m] = 0.5 * (1.0 + np.tanh(w / (xm - W - w) + w / (xm - W)))
+    return out
+
+
+def _rs_stress(NY, NZ, DH, rng, nuc_km=3.0, frame=None, L=0.02):
+    """Aging-law setup in the manner of the SCEC TPV101 benchmark: velocity
+    weakening a = 0.008 (b = 0.012) in the middle of the frictional area,
+    rising to velocity strengthening a = 0.016 over a band (3 km, at most a
+    fifth of the extent) at its lateral, bottom and top edges, with a seeded
+    +-3% field on a; sigma_n = 120 MPa, tau = 75 MPa + 2 MPa seeded field,
+    steady state at V = 1e-12 m/s, and a 25 MPa overstress patch of radius
+    nuc_km ramped in time over smooth_load_T (via dT0).  The explicit RK4
+    update of the aging-law state is stable while DT V / L stays below about
+    2.8, which bounds the slip rate a run with characteristic slip L can
+    carry."""
     sn, tau_mean, tau_sd = 120e6, 75e6, 2e6
     V0, Vini = 1e-6, 1e-12
-    a = 0.008 + 0.008 * (0.5 + 0.5 * np.tanh(gaussian_random_field((NZ, NY), 2e3 / DH, rng)))
+    jj, kk = np.meshgrid(np.arange(NY), np.arange(NZ))
+    ly = (NY - 2 * FAULT_BORDER) * DH           # frictional extent along strike
+    lz = (NZ - FAULT_BORDER) * DH               # and down dip from the surface
+    wy, wz = min(3e3, 0.2 * ly), min(3e3, 0.2 * lz)
+    y = (jj - (NY - 1) / 2.0) * DH
+    depth = (NZ - 1 - kk) * DH
+    B = _boxcar(y, ly / 2 - wy, wy) * _boxcar(depth - lz / 2, lz / 2 - wz, wz)
+    a = 0.008 + 0.008 * (1.0 - B)
+    a *= 1.0 + 0.03 * np.tanh(gaussian_random_field((NZ, NY), 2e3 / DH, rng))
     b = 0.012 * np.ones((NZ, NY))
     tau = tau_mean + tau_sd * gaussian_random_field((NZ, NY), 1e3 / DH, rng)
     psi = a * np.log(2 * V0 / Vini * np.sinh(tau / (a * sn)))  # steady state at Vini
-    jj, kk = np.meshgrid(np.arange(NY), np.arange(NZ))
     jc = NY // 2
-    kc = NZ - 1 - int(round(0.5 * (NZ - FAULT_BORDER)))
+    kc = NZ - 1 - int(round(lz / 2 / DH))
     r = np.hypot(jj - jc, kk - kc) * DH
     R = nuc_km * 1e3
     F = np.where(r < R, np.exp(np.minimum(r * r / np.minimum(r * r - R * R, -1e-9), 0.0)), 0.0)
@@ -141,7 +170,7 @@ def _rs_stress(NY, NZ, DH, rng, nuc_km=3.0, frame=None):
     return {
         "Tx": Tx, "Ty": Ty, "Tz": Tz,
         "dTx": dTx, "dTy": dTy, "dTz": dTz,
-        "a": a, "b": b, "L": 0.02 * np.ones((NZ, NY)), "Vw": 0.1 * np.ones((NZ, NY)), "State": psi,
+        "a": a, "b": b, "L": L * np.ones((NZ, NY)), "Vw": 0.1 * np.ones((NZ, NY)), "State": psi,
     }
 
 

@@ -92,7 +92,7 @@ struct FaultDev {
   int *init_t0_flag, *rup_index_y, *rup_index_z, *united, *faultgrid, *flag_rup;
   unsigned int* newton_stats;        // [3][nk][nj] per-node counts: solves, iterations, fallbacks
   float *Ws;                         // [9][2][nz][ny] smoothed T2*/T3* (FaultSmooth)
-  double *rs_carry;                  // [9][nk][nj] rate-state Newton inputs handed from fault_pre to fault_rs
+  double *rs_carry;                  // [10][nk][nj] rate-state Newton inputs handed from fault_pre to fault_rs
   float *rs_carryf;                  // [9][nk][nj] normal (3), jacvec, guess features (4), solved log10 V
   float *smooth_w;                   // [2][4][7][7] half-volume Gaussian weights (FaultSmooth)
 };

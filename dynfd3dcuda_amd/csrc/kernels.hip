@@ -376,6 +376,7 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     double* C = F.rs_carry;
     C[q] = st; C[n2 + q] = RS_a; C[2 * n2 + q] = Vtn; C[3 * n2 + q] = coef; C[4 * n2 + q] = x_plain;
     C[5 * n2 + q] = Tau_nd; C[6 * n2 + q] = Tt[1]; C[7 * n2 + q] = Tt[2]; C[8 * n2 + q] = Ttn;
+    C[9 * n2 + q] = Rn;
     float* Cf = F.rs_carryf;
     Cf[q] = n[0]; Cf[n2 + q] = n[1]; Cf[2 * n2 + q] = n[2]; Cf[3 * n2 + q] = jacvec;
     // x* of f(x) = 2 V0 e^(-psi/a) sinh x + coef x - Vt = 0 depends on the
@@ -411,11 +412,21 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
   const float* Cf = F.rs_carryf;
   const double st = C[q], RS_a = C[n2 + q], Vtn = C[2 * n2 + q], coef = C[3 * n2 + q], x_plain = C[4 * n2 + q];
   double x0 = x_plain;
-  if (a.mlp) {  // uniform: every lane of the wave takes part
-    float feat[4];
-    for (int f = 0; f < 4; f++) feat[f] = valid ? Cf[(4 + f) * n2 + q] : 0.0f;
-    const float lv = mlp_guess_wave(a.mlp, feat);  // log10 of the predicted slip rate
-    x0 = choose_guess(lv, x_plain, st, RS_a, Vtn, coef);
+  if (a.mlp) {
+    // 2 V0 e^(-psi/a) sinh(x_plain) is the previous slip rate Rn, so the
+    // plain guess's residual costs no transcendental.  The network is
+    // consulted by the waves in which some node's plain guess is poor
+    // (relative residual above kMlpGate); the vote is wave-uniform, so all
+    // 64 lanes enter the matrix instructions together.
+    const double Rn = C[9 * n2 + q];
+    const double f_pln = Rn + coef * x_plain - Vtn;
+    const bool poor = valid && fabs(f_pln) > kMlpGate * fmax(Vtn, 1e-300);
+    if (__any(poor)) {
+      float feat[4];
+      for (int f = 0; f < 4; f++) feat[f] = valid ? Cf[(4 + f) * n2 + q] : 0.0f;
+      const float lv = mlp_guess_wave(a.mlp, feat);  // log10 of the predicted slip rate
+      if (poor) x0 = choose_guess(lv, x_plain, f_pln, st, RS_a, Vtn, coef);
+    }
   }
   if (!valid) return;
   const double Tau_nd = C[5 * n2 + q], Tt1 = C[6 * n2 + q], Tt2 = C[7 * n2 + q];

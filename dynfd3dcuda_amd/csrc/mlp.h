@@ -146,15 +146,28 @@ __device__ inline float mlp_guess_wave(const MlpDev* __restrict__ m, const float
   return (y + m->b3) * m->yscale + m->ymean;
 }
 
+// Relative residual of the plain guess above which a node asks the network.
+constexpr double kMlpGate = 0.05;
+
 // Start of the Newton solve: the learned guess (log10 slip rate lv) or the
-// plain one, whichever has the smaller residual |f|.
-__device__ inline double choose_guess(float lv, double x_plain, double st, double a, double Vt, double coef) {
-  const double V0 = 1e-6;
-  const double ea = 2.0 * V0 * exp(-st / a);
-  double v = pow(10.0, (double)fminf(fmaxf(lv, -40.0f), 3.0f));
-  const double x_mlp = asinh(v / ea);
-  const double f_mlp = ea * sinh(x_mlp) + coef * x_mlp - Vt;
-  const double f_pln = ea * sinh(x_plain) + coef * x_plain - Vt;
+// plain one (residual f_pln), whichever has the smaller residual |f| of
+// f(x) = 2 V0 e^(-psi/a) sinh x + coef x - Vt.  x of the learned guess is
+// asinh(V / (2 V0 e^(-psi/a))); for x > 20 that is ln V + psi/a - ln V0 to
+// a relative 1e-17 and its sinh term is V itself.
+__device__ inline double choose_guess(float lv, double x_plain, double f_pln, double st, double a, double Vt,
+                                      double coef) {
+  const double V0 = 1e-6, ln10 = 2.302585092994046;
+  const double lvc = (double)fminf(fmaxf(lv, -40.0f), 3.0f);
+  const double v = exp(lvc * ln10);
+  double x_mlp = lvc * ln10 + st / a - log(V0);
+  double f_mlp;
+  if (x_mlp > 20.0) {
+    f_mlp = v + coef * x_mlp - Vt;
+  } else {
+    const double ea = 2.0 * V0 * exp(-st / a);
+    x_mlp = asinh(v / ea);
+    f_mlp = ea * sinh(x_mlp) + coef * x_mlp - Vt;
+  }
   return fabs(f_mlp) < fabs(f_pln) ? x_mlp : x_plain;
 }
 

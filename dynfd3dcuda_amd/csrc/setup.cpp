@@ -591,7 +591,7 @@ int host_setup(Ctx& c, std::string& err) {
     UPI(rup_index_y, riy); UPI(rup_index_z, riz); UPI(united, uni); UPI(faultgrid, fg);
     F.newton_stats = dmalloc<unsigned int>(c, 3 * n2);
     if (p.Friction_type >= 1) {
-      F.rs_carry = dmalloc<double>(c, 9 * n2);
+      F.rs_carry = dmalloc<double>(c, 10 * n2);
       F.rs_carryf = dmalloc<float>(c, 9 * n2);
       if (!F.rs_carry || !F.rs_carryf) { err = "hipMalloc failed for the rate-state hand-over"; return 6; }
     }

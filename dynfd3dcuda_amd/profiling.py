@@ -13,7 +13,7 @@
       same configuration)
 
 Kernel classes: stage_interior / stage_band / stage_surface (the three
-instantiations of the wave stage kernel), pml_aux, fault_pre, fault_deriv,
+instantiations of the wave stage kernel), pml_aux, fault_pre, fault_rs, fault_deriv,
 fault_post, rup_sensor; everything else is listed under its own name.
 """
 import argparse
@@ -36,7 +36,7 @@ def kernel_class(name):
     m = re.search(r"wave_stage_kernelI(?:Li-?\d+E|Lin1E){3}[ft]Li([012])E", name)  # mangled form
     if m:
         return STAGE_CLASSES[int(m.group(1))]
-    for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_deriv_kernel", "fault_post_kernel"):
+    for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_rs_kernel", "fault_deriv_kernel", "fault_post_kernel"):
         if k in name:
             return k[: -len("_kernel")]
     if "rup_sensor" in name:

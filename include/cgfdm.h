@@ -125,7 +125,7 @@ int cgfdm_newton_stats(cgfdm_ctx* ctx, double* solves, double* iterations, doubl
  * HIP events recorded on the solver's stream around every launch while
  * enabled.  cgfdm_profile(ctx, 1) clears and starts recording, 0 stops.
  * cgfdm_profile_get returns, for kernel class `name` ("stage_interior",
- * "stage_band", "stage_surface", "pml_aux", "fault_pre", "fault_deriv",
+ * "stage_band", "stage_surface", "pml_aux", "fault_pre", "fault_rs", "fault_deriv",
  * "fault_post", "rup_sensor"), the summed GPU time in ms and the launch count
  * since recording started (it synchronizes the stream), and the grid points
  * one launch of a stage-kernel class updates (0 for the others). */

@@ -23,13 +23,14 @@ def sw_curved():
 
 def rs_curved():
     """Rate-and-state aging law with a Gaussian-in-time overstress on the same
-    kind of grid, 400 steps."""
+    kind of grid, 400 steps.  L = 0.1 m keeps the explicit state update
+    stable (DT V / L < 2.8) up to 46 m/s slip rate."""
     c = sw_curved()
     rng = np.random.default_rng(12)
     p = c["params"]
     p.update({"Friction_type": 1, "RS_V0": 1e-6, "RS_f0": 0.6, "RS_Vini": 1e-12, "RS_L": 0.02,
               "RS_fw": 0.2, "smooth_load_T": 0.5})
-    c["stress"] = configs._rs_stress(p["NY"], p["NZ"], p["DH"], rng, frame=configs.fault_frame(c["geom"], p["DH"]))
+    c["stress"] = configs._rs_stress(p["NY"], p["NZ"], p["DH"], rng, frame=configs.fault_frame(c["geom"], p["DH"]), L=0.1)
     return c
 
 
