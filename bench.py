@@ -20,6 +20,9 @@ FP32).  Prints one JSON line:
                  present
   cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, same
                  workload, a few steps
+  mixed          the same workload with the metric and media stored in BF16
+                 (FP32 arithmetic): throughput and the step's fraction of the
+                 672 B/pt-step roofline
   rate_state     rate-and-state fault step with plain Newton against the
                  learned initial guess, same grid with the aging law: the
                  plain run's Newton samples train the network on the GPU,
@@ -119,6 +122,30 @@ def cpu_baseline(case, budget_s):
                       f"warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), {cpu_model()}"}
 
 
+def mixed_leg(case, steps, warmup):
+    """Same workload with MetricBF16=1: throughput of the whole step."""
+    import dynfd3dcuda_amd as dyn
+    from dynfd3dcuda_amd import configs
+    p = dict(case["params"], MetricBF16=1)
+    s = dyn.Solver(p)
+    configs.apply(case, s)
+    s.setup()
+    d = s.dims()
+    for it in range(warmup):
+        s.step(it)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        s.step(it)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    s.close()
+    v = d["ni"] * d["nj"] * d["nk"] * steps / t
+    ba = b_alg(p)
+    return {"metric_storage": "bf16", "value": v, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * t / steps,
+            "B_alg_bytes_per_pt_step": ba, "step_hbm_roofline_frac": v * ba / (PEAK_HBM_GBS * 1e9)}
+
+
 def rate_state_leg(args):
     """Plain Newton (NewtonMode 1) against the learned guess (NewtonMode 2,
     same residual rule and fallback) on the rate-state configuration."""
@@ -187,6 +214,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline timing budget")
     ap.add_argument("--no-rs", action="store_true", help="skip the rate-state plain-vs-learned leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the BF16-metric leg")
     ap.add_argument("--rs-lead", type=int, default=600, help="steps before the rate-state timing window")
     ap.add_argument("--rs-steps", type=int, default=100, help="rate-state timing window (steps)")
     ap.add_argument("--rs-sample-every", type=int, default=4, help="training-sample stride (steps)")
@@ -261,9 +289,12 @@ def main():
                      "points_per_launch": pts, "avg_launch_ms": avg_s * 1e3},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1] > 0},
         "cpu_baseline": None,
+        "mixed": None,
         "rate_state": None,
     }
     solver.close()
+    if not args.no_mixed:
+        out["mixed"] = mixed_leg(case, args.steps, args.warmup)
     if not args.no_rs:
         out["rate_state"] = rate_state_leg(args)
     if not args.no_cpu:
