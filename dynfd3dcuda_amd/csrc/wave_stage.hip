@@ -192,7 +192,7 @@ __device__ __forceinline__ float pml_fs_term(int c, const float* z1, float ztx, 
 // trivial), 1 absorbing bands (PML correction, Cerjan), 2 tiles holding the
 // free-surface rows (everything).
 template <int FX, int FY, int FZ, class MT, int KIND, int TJ_ = TJ>
-__global__ void __launch_bounds__(NTHR) wave_stage_kernel(WaveArgs<MT> a) {
+__global__ void __launch_bounds__(NTHR, KIND == 2 ? 2 : 4) wave_stage_kernel(WaveArgs<MT> a) {
   constexpr int TJ = TJ_, TK = NTHR / TJ_;  // tile of TJ (j) x TK (k) columns
   constexpr int LJ = TJ + 4, LK = TK + 4;   // LDS tile with halos
   constexpr int NH = 4 * LJ + 4 * TK;       // halo cells of one plane: 4 rows + 4 side cells per row
