@@ -462,8 +462,55 @@ __device__ __forceinline__ void mv3(const float* A, const float* x, float* y) {
   for (int r = 0; r < 3; r++) y[r] = A[3 * r] * x[0] + A[3 * r + 1] * x[1] + A[3 * r + 2] * x[2];
 }
 
+// x-derivative of velocity component c on off-fault plane i0 + l of the
+// slab (nn = l + 3 for l < 0, l + 4 for l > 0), as fault_dstrs_f/_b form it
+// (fault_dstrs_f.cu:180-330): one-sided DRP, the 2-4 operator and the
+// two-point difference next to the fault, reaching the split node's own
+// side (F.W of side 0 or 1) instead of the plane i0.
+__device__ __forceinline__ float dxv_off(const FArgs& a, int nn, int c, long pc, long pf) {
+  const FaultDev& F = a.F;
+  const long wps = a.wps;
+  const float* W = a.W;
+  const int i0 = a.i0;
+  auto wv = [&](int i) -> const float* { return &W[c * a.pyz + pc + i * wps]; };
+  const float f0 = F.W[c * 2 * a.nyz + pf], f1 = F.W[c * 2 * a.nyz + a.nyz + pf];
+  float d;
+  if (a.FX > 0) {
+    if (nn == 7) d = drpF(wv(i0 + 3), wps);
+    else if (nn == 6) d = d24F(wv(i0 + 2), wps);
+    else if (nn == 5) d = wv(i0 + 1)[wps] - wv(i0 + 1)[0];
+    else if (nn == 0) {
+      const float v5[5] = {wv(i0 - 4)[0], wv(i0 - 3)[0], wv(i0 - 2)[0], wv(i0 - 1)[0], f0};
+      d = drpF(&v5[1], 1);
+    } else if (nn == 1) {
+      const float v3[3] = {wv(i0 - 2)[0], wv(i0 - 1)[0], f0};
+      d = d24F(&v3[0], 1);
+    } else {
+      d = f0 - wv(i0 - 1)[0];
+    }
+  } else {
+    if (nn == 0) d = drpB(wv(i0 - 3), wps);
+    else if (nn == 1) d = d24B(wv(i0 - 2), wps);
+    else if (nn == 2) d = wv(i0 - 1)[0] - wv(i0 - 1)[-wps];
+    else if (nn == 7) {
+      const float v5[5] = {f1, wv(i0 + 1)[0], wv(i0 + 2)[0], wv(i0 + 3)[0], wv(i0 + 4)[0]};
+      d = drpB(&v5[3], 1);
+    } else if (nn == 6) {
+      const float v3[3] = {f1, wv(i0 + 1)[0], wv(i0 + 2)[0]};
+      d = d24B(&v3[2], 1);
+    } else {
+      d = wv(i0 + 1)[0] - f1;
+    }
+  }
+  return d * a.rDH;
+}
+
 // fault_deriv: fault_dvelo_cu (fault_dvelo.cu:17-389) + fault_dstrs_f/b_cu
-// (fault_dstrs_f.cu:17-1254, fault_dstrs_b.cu) writing the slab RHS.
+// (fault_dstrs_f.cu:17-1254, fault_dstrs_b.cu) writing the slab RHS.  The
+// parts that are independent per slab plane run in parallel, one per block
+// layer (blockIdx.z): 0-5 the velocity rates of planes i0-3..-1, +1..+3,
+// 6-11 the stress rates of the same planes, 12-13 the split-node velocity
+// rates of sides 0 and 1, 14-15 their stress rates.
 __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -482,44 +529,82 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   float* slab = a.slabH;
   auto slabx = [&](int i, int c) -> float& { return slab[(long)(i - (i0 - 3)) * 9 * pyz + c * pyz + pc]; };
 
-  // ---- velocity rates at i0-3..-1, +1..+3 (fault_dvelo.cu:153-255)
-  for (int i = i0 - 3; i <= i0 + 3; i++) {
-    int nn = i0 - i;
-    if (nn == 0) continue;
-    float v1[3][7], v2[3][7], v3[3][7];
-    for (int l = -3; l <= 3; l++) {
-      float o[3];
-      trac(a, W, i + l, pc, 0, o);
-      for (int c = 0; c < 3; c++) v1[c][l + 3] = o[c];
-      trac(a, W, i, pc + l, 1, o);
-      for (int c = 0; c < 3; c++) v2[c][l + 3] = o[c];
-      trac(a, W, i, pc + (long)l * py, 2, o);
-      for (int c = 0; c < 3; c++) v3[c][l + 3] = o[c];
-    }
-    v1[0][nn + 3] = F.T11[3 * nyz + pf];
-    v1[1][nn + 3] = F.T12[3 * nyz + pf];
-    v1[2][nn + 3] = F.T13[3 * nyz + pf];
-    for (int c = 0; c < 3; c++) {
-      float* v = v1[c];
-      if (nn == 2) v[6] = 2.0f * v[5] - v[4];
-      if (nn == 1) { v[5] = 2.0f * v[4] - v[3]; v[6] = 2.0f * v[4] - v[2]; }
-      if (nn == -1) { v[0] = 2.0f * v[2] - v[4]; v[1] = 2.0f * v[2] - v[3]; }
-      if (nn == -2) v[0] = 2.0f * v[1] - v[2];
-    }
-    if (a.freenode && km <= 3)
-      for (int c = 0; c < 3; c++) {
-        v3[c][km + 2] = 0.0f;
-        for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
+  const int part = blockIdx.z;
+  if (part < 6) {  // velocity rates at i0-3..-1, +1..+3 (fault_dvelo.cu:153-255)
+    const int i = i0 + (part < 3 ? part - 3 : part - 2);
+    {
+      const int nn = i0 - i;
+      float v1[3][7], v2[3][7], v3[3][7];
+      for (int l = -3; l <= 3; l++) {
+        float o[3];
+        trac(a, W, i + l, pc, 0, o);
+        for (int c = 0; c < 3; c++) v1[c][l + 3] = o[c];
+        trac(a, W, i, pc + l, 1, o);
+        for (int c = 0; c < 3; c++) v2[c][l + 3] = o[c];
+        trac(a, W, i, pc + (long)l * py, 2, o);
+        for (int c = 0; c < 3; c++) v3[c][l + 3] = o[c];
       }
-    const long p = pc + i * mps;
-    float rrj = 1.0f / (LM(a, 12 * pyz + p) * LM(a, 9 * pyz + p));
-    for (int c = 0; c < 3; c++) {
-      float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
-      slabx(i, c) = (dxt + dyt + dzt) * rrj;
+      v1[0][nn + 3] = F.T11[3 * nyz + pf];
+      v1[1][nn + 3] = F.T12[3 * nyz + pf];
+      v1[2][nn + 3] = F.T13[3 * nyz + pf];
+      for (int c = 0; c < 3; c++) {
+        float* v = v1[c];
+        if (nn == 2) v[6] = 2.0f * v[5] - v[4];
+        if (nn == 1) { v[5] = 2.0f * v[4] - v[3]; v[6] = 2.0f * v[4] - v[2]; }
+        if (nn == -1) { v[0] = 2.0f * v[2] - v[4]; v[1] = 2.0f * v[2] - v[3]; }
+        if (nn == -2) v[0] = 2.0f * v[1] - v[2];
+      }
+      if (a.freenode && km <= 3)
+        for (int c = 0; c < 3; c++) {
+          v3[c][km + 2] = 0.0f;
+          for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
+        }
+      const long p = pc + i * mps;
+      float rrj = 1.0f / (LM(a, 12 * pyz + p) * LM(a, 9 * pyz + p));
+      for (int c = 0; c < 3; c++) {
+        float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
+        slabx(i, c) = (dxt + dyt + dzt) * rrj;
+      }
     }
+    return;
   }
-  // ---- split-node velocity rates (fault_dvelo.cu:260-385)
-  for (int m = 0; m < 2; m++) {
+  if (part < 12) {  // off-fault stress rates (fault_dstrs_f.cu:1009-1250)
+    const int nn = part < 9 ? part - 6 : part - 4;
+    {
+      const int l = nn < 4 ? nn - 3 : nn - 4;
+      const int i = i0 + l;
+      const long p = pc + i * mps, pw = pc + i * wps;
+      float dy[3], dz[3];
+      float dx[3];
+      for (int c = 0; c < 3; c++) dx[c] = dxv_off(a, nn, c, pc, pf);
+      for (int c = 0; c < 3; c++) dy[c] = drp(&W[c * pyz + pw], 1, FY) * rDH;
+      long ij = (j1 + (long)i * ny) * 9;
+      for (int c = 0; c < 3; c++) {
+        if (a.freenode && km == 1)
+          dz[c] = a.mVx[ij + 3 * c] * dx[0] + a.mVx[ij + 3 * c + 1] * dx[1] + a.mVx[ij + 3 * c + 2] * dx[2]
+                + a.mVy[ij + 3 * c] * dy[0] + a.mVy[ij + 3 * c + 1] * dy[1] + a.mVy[ij + 3 * c + 2] * dy[2];
+        else if (a.freenode && km == 2) dz[c] = d22(&W[c * pyz + pw], py, FZ) * rDH;
+        else if (a.freenode && km == 3) dz[c] = d24(&W[c * pyz + pw], py, FZ) * rDH;
+        else dz[c] = drp(&W[c * pyz + pw], py, FZ) * rDH;
+      }
+      float lam = LM(a, 10 * pyz + p), mu = LM(a, 11 * pyz + p), l2m = lam + 2.0f * mu;
+      float xi[3] = {LM(a, p), LM(a, pyz + p), LM(a, 2 * pyz + p)};
+      float et[3] = {LM(a, 3 * pyz + p), LM(a, 4 * pyz + p), LM(a, 5 * pyz + p)};
+      float zt[3] = {LM(a, 6 * pyz + p), LM(a, 7 * pyz + p), LM(a, 8 * pyz + p)};
+      slabx(i, 3) = l2m * xi[0] * dx[0] + lam * xi[1] * dx[1] + lam * xi[2] * dx[2] + l2m * et[0] * dy[0] + lam * et[1] * dy[1] +
+                    lam * et[2] * dy[2] + l2m * zt[0] * dz[0] + lam * zt[1] * dz[1] + lam * zt[2] * dz[2];
+      slabx(i, 4) = lam * xi[0] * dx[0] + l2m * xi[1] * dx[1] + lam * xi[2] * dx[2] + lam * et[0] * dy[0] + l2m * et[1] * dy[1] +
+                    lam * et[2] * dy[2] + lam * zt[0] * dz[0] + l2m * zt[1] * dz[1] + lam * zt[2] * dz[2];
+      slabx(i, 5) = lam * xi[0] * dx[0] + lam * xi[1] * dx[1] + l2m * xi[2] * dx[2] + lam * et[0] * dy[0] + lam * et[1] * dy[1] +
+                    l2m * et[2] * dy[2] + lam * zt[0] * dz[0] + lam * zt[1] * dz[1] + l2m * zt[2] * dz[2];
+      slabx(i, 6) = (xi[1] * dx[0] + xi[0] * dx[1] + et[1] * dy[0] + et[0] * dy[1] + zt[1] * dz[0] + zt[0] * dz[1]) * mu;
+      slabx(i, 7) = (xi[2] * dx[0] + xi[0] * dx[2] + et[2] * dy[0] + et[0] * dy[2] + zt[2] * dz[0] + zt[0] * dz[2]) * mu;
+      slabx(i, 8) = (xi[2] * dx[1] + xi[1] * dx[2] + et[2] * dy[1] + et[1] * dy[2] + zt[2] * dz[1] + zt[1] * dz[2]) * mu;
+    }
+    return;
+  }
+  if (part < 14) {  // split-node velocity rates of side part-12 (fault_dvelo.cu:260-385)
+    const int m = part - 12;
     const float* TT[3] = {F.T11, F.T12, F.T13};
     float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LM(a, 9 * pyz + pc + i0 * mps));
     for (int c = 0; c < 3; c++) {
@@ -542,34 +627,22 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
       float dyt = (F.rup_index_y[q] % 7 ? d22(&v2[3], 1, FY) : drp(&v2[3], 1, FY)) * rDH;
       F.hW[c * 2 * nyz + m * nyz + pf] = (dxt + dyt + dzt) * rrj;
     }
+    return;
   }
+  // split-node stress rates of side part-14; the x-derivatives of the two
+  // sides are coupled on the surface row, so each side forms both
+  const int ms = part - 14;
 
   // ---- stress rates (fault_dstrs_f/b)
   auto wv = [&](int c, int i) -> const float* { return &W[c * pyz + pc + i * wps]; };
   auto fv = [&](int c, int m) -> const float* { return &F.W[c * 2 * nyz + m * nyz + pf]; };
-  float DxV[8][3], DyV[8][3], DzV[8][3];
+  float DxV[8][3], DyV[8][3], DzV[8][3];  // entries 3 and 4: the two sides of the split node
   for (int c = 0; c < 3; c++) {
     if (FX > 0) {
-      DxV[7][c] = drpF(wv(c, i0 + 3), wps) * rDH;
-      DxV[6][c] = d24F(wv(c, i0 + 2), wps) * rDH;
-      DxV[5][c] = (wv(c, i0 + 1)[wps] - wv(c, i0 + 1)[0]) * rDH;
       DxV[4][c] = (wv(c, i0 + 1)[0] - fv(c, 1)[0]) * rDH;
-      float v5[5] = {wv(c, i0 - 4)[0], wv(c, i0 - 3)[0], wv(c, i0 - 2)[0], wv(c, i0 - 1)[0], fv(c, 0)[0]};
-      DxV[0][c] = drpF(&v5[1], 1) * rDH;
-      float v3[3] = {wv(c, i0 - 2)[0], wv(c, i0 - 1)[0], fv(c, 0)[0]};
-      DxV[1][c] = d24F(&v3[0], 1) * rDH;
-      DxV[2][c] = (fv(c, 0)[0] - wv(c, i0 - 1)[0]) * rDH;
       DxV[3][c] = 0.0f;
     } else {
-      DxV[0][c] = drpB(wv(c, i0 - 3), wps) * rDH;
-      DxV[1][c] = d24B(wv(c, i0 - 2), wps) * rDH;
-      DxV[2][c] = (wv(c, i0 - 1)[0] - wv(c, i0 - 1)[-wps]) * rDH;
       DxV[3][c] = (fv(c, 0)[0] - wv(c, i0 - 1)[0]) * rDH;
-      float v5[5] = {fv(c, 1)[0], wv(c, i0 + 1)[0], wv(c, i0 + 2)[0], wv(c, i0 + 3)[0], wv(c, i0 + 4)[0]};
-      DxV[7][c] = drpB(&v5[3], 1) * rDH;
-      float v3[3] = {fv(c, 1)[0], wv(c, i0 + 1)[0], wv(c, i0 + 2)[0]};
-      DxV[6][c] = d24B(&v3[2], 1) * rDH;
-      DxV[5][c] = (wv(c, i0 + 1)[0] - fv(c, 1)[0]) * rDH;
       DxV[4][c] = 0.0f;
     }
     for (int m = 0; m < 2; m++) DyV[3 + m][c] = (F.rup_index_y[q] % 7 ? d22(fv(c, m), 1, FY) : drp(fv(c, m), 1, FY)) * rDH;
@@ -593,7 +666,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
     for (int c = 0; c < 3; c++) DzV[4][c] = o1[c] + o2[c];
   } else {
     for (int c = 0; c < 3; c++)
-      for (int m = 0; m < 2; m++) {
+      for (int m = ms; m <= ms; m++) {  // off the surface row each side needs only its own
         const float* f = fv(c, m);
         if (a.freenode && km == 2) DzV[3 + m][c] = d22(f, ny, FZ) * rDH;
         else if (a.freenode && km == 3) DzV[3 + m][c] = d24(f, ny, FZ) * rDH;
@@ -602,7 +675,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
     // DxV_T1: D11^-1 (dT1/dt - D12 DyV - D13 DzV) on both sides
     float dtT1[3] = {0.f, 0.f, 0.f};
     if (F.faultgrid[q]) { dtT1[0] = F.hT11[pf]; dtT1[1] = F.hT12[pf]; dtT1[2] = F.hT13[pf]; }
-    for (int m = 0; m < 2; m++) {
+    for (int m = ms; m <= ms; m++) {
       float o1[3], o2[3], o3[3];
       mv3(&F.matT1toVx[(m * nyz + pf) * 9], dtT1, o1);
       mv3(&F.matVytoVx[(m * nyz + pf) * 9], DyV[3 + m], o2);
@@ -611,7 +684,8 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
     }
   }
   // split-node hT2*, hT3* (fault_dstrs_f.cu:874-1007)
-  for (int m = 0; m < 2; m++)
+  {
+    const int m = ms;
     for (int row = 1; row <= 2; row++) {
       float o1[3], o2[3], o3[3];
       mv3(&F.Dmat[((long)(m * 9 + row * 3 + 0) * nyz + pf) * 9], DxV[3 + m], o1);
@@ -619,37 +693,6 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
       mv3(&F.Dmat[((long)(m * 9 + row * 3 + 2) * nyz + pf) * 9], DzV[3 + m], o3);
       for (int c = 0; c < 3; c++) F.hW[(3 * row + c) * 2 * nyz + m * nyz + pf] = o1[c] + o2[c] + o3[c];
     }
-  // off-fault stress rates (fault_dstrs_f.cu:1009-1250)
-  for (int nn = 0; nn < 8; nn++) {
-    int l = nn < 4 ? nn - 3 : nn - 4;
-    if (l == 0) continue;
-    int i = i0 + l;
-    const long p = pc + i * mps, pw = pc + i * wps;
-    float dy[3], dz[3];
-    const float* dx = DxV[nn];
-    for (int c = 0; c < 3; c++) dy[c] = drp(&W[c * pyz + pw], 1, FY) * rDH;
-    long ij = (j1 + (long)i * ny) * 9;
-    for (int c = 0; c < 3; c++) {
-      if (a.freenode && km == 1)
-        dz[c] = a.mVx[ij + 3 * c] * dx[0] + a.mVx[ij + 3 * c + 1] * dx[1] + a.mVx[ij + 3 * c + 2] * dx[2]
-              + a.mVy[ij + 3 * c] * dy[0] + a.mVy[ij + 3 * c + 1] * dy[1] + a.mVy[ij + 3 * c + 2] * dy[2];
-      else if (a.freenode && km == 2) dz[c] = d22(&W[c * pyz + pw], py, FZ) * rDH;
-      else if (a.freenode && km == 3) dz[c] = d24(&W[c * pyz + pw], py, FZ) * rDH;
-      else dz[c] = drp(&W[c * pyz + pw], py, FZ) * rDH;
-    }
-    float lam = LM(a, 10 * pyz + p), mu = LM(a, 11 * pyz + p), l2m = lam + 2.0f * mu;
-    float xi[3] = {LM(a, p), LM(a, pyz + p), LM(a, 2 * pyz + p)};
-    float et[3] = {LM(a, 3 * pyz + p), LM(a, 4 * pyz + p), LM(a, 5 * pyz + p)};
-    float zt[3] = {LM(a, 6 * pyz + p), LM(a, 7 * pyz + p), LM(a, 8 * pyz + p)};
-    slabx(i, 3) = l2m * xi[0] * dx[0] + lam * xi[1] * dx[1] + lam * xi[2] * dx[2] + l2m * et[0] * dy[0] + lam * et[1] * dy[1] +
-                  lam * et[2] * dy[2] + l2m * zt[0] * dz[0] + lam * zt[1] * dz[1] + lam * zt[2] * dz[2];
-    slabx(i, 4) = lam * xi[0] * dx[0] + l2m * xi[1] * dx[1] + lam * xi[2] * dx[2] + lam * et[0] * dy[0] + l2m * et[1] * dy[1] +
-                  lam * et[2] * dy[2] + lam * zt[0] * dz[0] + l2m * zt[1] * dz[1] + lam * zt[2] * dz[2];
-    slabx(i, 5) = lam * xi[0] * dx[0] + lam * xi[1] * dx[1] + l2m * xi[2] * dx[2] + lam * et[0] * dy[0] + lam * et[1] * dy[1] +
-                  l2m * et[2] * dy[2] + lam * zt[0] * dz[0] + lam * zt[1] * dz[1] + l2m * zt[2] * dz[2];
-    slabx(i, 6) = (xi[1] * dx[0] + xi[0] * dx[1] + et[1] * dy[0] + et[0] * dy[1] + zt[1] * dz[0] + zt[0] * dz[1]) * mu;
-    slabx(i, 7) = (xi[2] * dx[0] + xi[0] * dx[2] + et[2] * dy[0] + et[0] * dy[2] + zt[2] * dz[0] + zt[0] * dz[2]) * mu;
-    slabx(i, 8) = (xi[2] * dx[1] + xi[1] * dx[2] + et[2] * dy[1] + et[1] * dy[2] + zt[2] * dz[1] + zt[1] * dz[2]) * mu;
   }
 }
 
@@ -668,9 +711,10 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   const int irk = a.irk;
   const float rka = (float)(RA[irk] * a.DTd), rkb = (float)(RB[irk] * a.DTd);
   float dampj = a.cerjan ? fminf(a.cj[1][j1], a.cj[2][k1]) : 1.0f;
-  // volume RK of the slab planes
-  for (int pl = 0; pl < 7; pl++) {
-    if (pl == 3) continue;
+  // volume RK of the slab planes, one plane per block layer (blockIdx.z
+  // 0-5); layer 6 does the split nodes and fault2wave
+  if (blockIdx.z < 6) {
+    const int pl = blockIdx.z < 3 ? blockIdx.z : blockIdx.z + 1;
     int ip = i0 - 3 + pl;
     const long p = pc + ip * wps;
     float damp = a.cerjan ? fminf(a.cj[0][ip], dampj) : 1.0f;
@@ -680,6 +724,7 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
       float w = rk_combine(irk, rka, rkb, a.Win[x], h, a.S, a.T, x);
       a.Wi0[x] = w * damp;
     }
+    return;
   }
   float s1[3] = {F.vec_s1[pv], F.vec_s1[pv + 1], F.vec_s1[pv + 2]};
   float s2[3] = {F.vec_s2[pv], F.vec_s2[pv + 1], F.vec_s2[pv + 2]};
@@ -931,7 +976,9 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz) {
   a.W = Win;
   a.FX = fx; a.FY = fy; a.FZ = fz;
   c.prof.begin(c.stream, PK_FAULT_DERIV);
-  hipLaunchKernelGGL(fault_deriv_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
+  dim3 g = fgrid(c);
+  g.z = 16;  // per-plane parts and the split-node parts
+  hipLaunchKernelGGL(fault_deriv_kernel, g, dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 
@@ -940,7 +987,9 @@ void launch_fault_post(Ctx& c, const StageBufs& b, int irk) {
   a.Win = b.Win; a.Wi0 = b.Wout; a.S = b.S; a.T = b.T;
   a.irk = irk;
   c.prof.begin(c.stream, PK_FAULT_POST);
-  hipLaunchKernelGGL(fault_post_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
+  dim3 g = fgrid(c);
+  g.z = 7;  // six slab planes and the split-node part
+  hipLaunchKernelGGL(fault_post_kernel, g, dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 
