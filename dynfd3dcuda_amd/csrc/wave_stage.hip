@@ -638,6 +638,12 @@ int upload_wave_cold(Ctx& c) {
     hi[ax] = std::max(c.pml.on[2 * ax + 1] ? c.pml.N : 0, p.useCerjan && (ax < 2 || !freenode) ? p.DAMP_N : 0);
   }
   if (freenode) hi[2] = std::max(hi[2], 3);
+  // the interior box holds whole tiles along j and k: a remainder joins the
+  // high-side band (or the free-surface tiles), whose kernels treat any point
+  for (int ax = 1; ax < 3; ax++) {
+    const int t = ax == 1 ? TJ : TK, e = n3[ax] - lo[ax] - hi[ax];
+    if (e > t) hi[ax] += e % t;
+  }
   const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
   const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
   const int nk = n3[2];
