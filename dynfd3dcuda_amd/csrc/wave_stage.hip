@@ -110,7 +110,7 @@ struct WaveArgs {
   float* __restrict__ slabH;
   float* __restrict__ rhs;
   const WaveCold* __restrict__ cold;
-  const int4* __restrict__ work;  // two int4 per block: (j0, k0, first plane, end plane), (j extent, k extent, -, -)
+  const int4* __restrict__ work;  // two int4 per block: (j0, k0, first plane, end plane), (j lanes, k rows, first active j lane, first active k row)
   unsigned nyz;                   // words of one field of one plane (py*nz, padded rows)
   int py, jpad;                   // row pitch and front pad of volume rows
   int nwork, ni, nj, nk, ny, ii0;
@@ -209,14 +209,14 @@ __global__ void __launch_bounds__(NTHR, KIND == 2 ? 2 : 4) wave_stage_kernel(Wav
   const int tj = threadIdx.x, tk = threadIdx.y, tid = tj + tk * TJ;
   const int ni = a.ni, nj = a.nj, nk = a.nk, ny = a.ny;
   const int j = j0 + tj, k = k0 + tk;
-  const bool act = tj < we.x && tk < we.y;
+  const bool act = tj >= we.z && tj < we.x && tk >= we.w && tk < we.y;
   const bool pml = KIND >= 1 && a.pml, cerjan = KIND >= 1 && a.cerjan;
   const int nxp = ni + 6;  // padded planes
   const uint32_t nyz = a.nyz;
   // Every lane loads (no predicated loads): lanes past the padded array read
   // a clamped in-bounds column whose value no active point ever uses.
   const uint32_t py = a.py, jpad = a.jpad;
-  const uint32_t col = jpad + (uint32_t)(min(j, nj + 2) + 3) + (uint32_t)(min(k, nk + 2) + 3) * py;
+  const uint32_t col = jpad + (uint32_t)(min(max(j, -3), nj + 2) + 3) + (uint32_t)(min(max(k, -3), nk + 2) + 3) * py;
   const uint32_t col4 = col << 2;
   const uint32_t mcol = col * (uint32_t)sizeof(MT);
   const uint32_t pb = nyz * 4, mpb = nyz * (uint32_t)sizeof(MT);  // bytes of one field of one plane
@@ -591,13 +591,21 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
 // Tiles [lo, hi) of one axis: the edge band below, tiles of length t
 // starting at the interior's first index, the edge band above.  Each
 // segment is (start, length, interior?).
-struct Seg { int s, n; bool in; };
+// A tile range along one axis: lanes [0, n) of a tile starting at index s,
+// of which [o, n) are active.  Low-side band segments end at the interior
+// edge, so a partial tile's idle lanes fall on the ghost side (their clamped
+// loads all hit the ghost column) rather than on interior points.
+struct Seg { int s, n; bool in; int o; };
 std::vector<Seg> axis_segments(int n, int lo, int hi, int t) {
   std::vector<Seg> v;
   const int a = std::min(lo, n), b = std::max(a, n - hi);
-  for (int x = 0; x < a; x += t) v.push_back({x, std::min(t, a - x), false});
-  for (int x = a; x < b; x += t) v.push_back({x, std::min(t, b - x), true});
-  for (int x = b; x < n; x += t) v.push_back({x, std::min(t, n - x), false});
+  const int m = (a + t - 1) / t;
+  for (int q = 0; q < m; q++) {
+    const int x = a - (m - q) * t;
+    v.push_back({x, t, false, std::max(0, -x)});
+  }
+  for (int x = a; x < b; x += t) v.push_back({x, std::min(t, b - x), true, 0});
+  for (int x = b; x < n; x += t) v.push_back({x, std::min(t, n - x), false, 0});
   return v;
 }
 
@@ -672,10 +680,16 @@ int upload_wave_cold(Ctx& c) {
   for (int kd = 0; kd < 3; kd++) c.wave_points[kd] = 0;
   auto push = [&](int list, const Seg& a, const Seg& b, int i0, int i1) {
     lists[list].push_back(make_int4(a.s, b.s, i0, i1));
-    lists[list].push_back(make_int4(a.n, b.n, 0, 0));
-    c.wave_points[list == 0 ? 0 : list == 3 ? 2 : 1] += (long)a.n * b.n * (i1 - i0);
+    lists[list].push_back(make_int4(a.n, b.n, a.o, b.o));
+    c.wave_points[list == 0 ? 0 : list == 3 ? 2 : 1] += (long)(a.n - a.o) * (b.n - b.o) * (i1 - i0);
   };
-  const auto narrow = [&](const Seg& a) { return !a.in && a.n <= TJN; };
+  const auto narrow = [&](const Seg& a) { return !a.in && a.n - a.o <= TJN; };
+  // the same active columns in a TJN-wide tile: anchored at the interior
+  // edge on the low side, at the band start on the high side
+  const auto narrowed = [&](const Seg& a) {
+    const int w = a.n - a.o;
+    return a.o > 0 ? Seg{a.s + a.n - TJN, TJN, false, TJN - w} : Seg{a.s, w, false, 0};
+  };
   int ktop = nk;  // first row of the free-surface tiles
   for (const Seg& b : sk)
     if (is_top(b)) ktop = std::min(ktop, b.s);
@@ -694,7 +708,7 @@ int upload_wave_cold(Ctx& c) {
   for (int i0 = 0; i0 < ni; i0 += ich_band)
     for (int k0 = 0; k0 < ktop; k0 += TKN)
       for (const Seg& a : sj)
-        if (narrow(a)) push(2, a, Seg{k0, std::min(TKN, ktop - k0), false}, i0, std::min(ni, i0 + ich_band));
+        if (narrow(a)) push(2, narrowed(a), Seg{k0, std::min(TKN, ktop - k0), false, 0}, i0, std::min(ni, i0 + ich_band));
   for (int i0 = 0; i0 < ni; i0 += ich_top)
     for (const Seg& b : sk)
       for (const Seg& a : sj)
