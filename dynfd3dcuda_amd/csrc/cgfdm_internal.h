@@ -160,6 +160,8 @@ struct Ctx {
   float *cj[3] = {nullptr, nullptr, nullptr};      // Cerjan 1-D profiles (nx, ny, nz incl. ghosts)
   void* wave_cold = nullptr;    // device copy of the off-hot-path pointer table
   void* wave_work = nullptr;    // stage-kernel work list (two int4 per block)
+  float* trac_top = nullptr;    // free surface: traction vectors of the six top rows [nx][6][9][ny]
+  float* img_top = nullptr;     // free surface: image velocity RHS of the three top rows [3][3][ni][nj]
   int wave_nwork = 0;                  // all work items
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind

@@ -109,6 +109,8 @@ struct WaveArgs {
   const MT* __restrict__ met;
   float* __restrict__ slabH;
   float* __restrict__ rhs;
+  float* ttop;  // traction vectors of the six top rows (free surface)
+  float* itop;  // image velocity RHS of the three top rows (free surface)
   const WaveCold* __restrict__ cold;
   const int4* __restrict__ work;  // two int4 per block: (j0, k0, first plane, end plane), (j lanes, k rows, first active j lane, first active k row)
   unsigned nyz;                   // words of one field of one plane (py*nz, padded rows)
@@ -128,49 +130,86 @@ __device__ __forceinline__ int xcd_remap(int L, int n) {
   return x < full ? x * per + s : full * per + (x - full) * (per - 1) + s;
 }
 
-// Traction-image hV for the top three layers (wave.cu:398-487): the
-// conservative traction vectors along each axis, zero at the surface and
-// odd-mirrored above it.  Direct (cached) loads: only 3 of nk layers.
-template <int FX, int FY, int FZ, class MT>
-__device__ void traction_image(const WaveArgs<MT>& a, int x, uint32_t col, int k, float rb1, float rb2, float rb3,
-                               float rrho, float* h) {
+// Free surface, first pass: the conservative traction vectors
+// J (grad xi_ax . sigma) (wave.cu:398-440) of the six rows nk-6..nk-1 that
+// the image stencil reaches, for every padded plane and column, from the
+// stage input: ttop[x][r][3 ax + c][jj], r = row - (nk-6), jj the padded
+// column.  One thread per point.
+template <class MT>
+__global__ void __launch_bounds__(256) traction_top_kernel(WaveArgs<MT> a, int nx) {
+  const int ny = a.ny;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nx * 6 * ny) return;
+  const int jj = (int)(idx % ny);
+  const int r = (int)((idx / ny) % 6);
+  const int x = (int)(idx / ((long)ny * 6));
+  const uint32_t nyz = a.nyz;
+  const uint32_t q = (uint32_t)a.jpad + (uint32_t)jj + (uint32_t)(a.nk - 6 + r + 3) * (uint32_t)a.py;
   const float* W = a.Win;
   const MT* M = a.met;
-  const uint32_t nyz = a.nyz;
-  const int ik = a.nk - 1 - k;
-  const float rrhojac = rrho / mat_(M, 9, x, col, nyz);
-  const int fl[3] = {FX, FY, FZ};
-  float d[3][3];  // d[axis][component]
+  const float J = mat_(M, 9, x, q, nyz);
+  const float txx = wat(W, 3, x, q, nyz), tyy = wat(W, 4, x, q, nyz), tzz = wat(W, 5, x, q, nyz);
+  const float txy = wat(W, 6, x, q, nyz), txz = wat(W, 7, x, q, nyz), tyz = wat(W, 8, x, q, nyz);
+  float* o = a.ttop + ((long)x * 6 + r) * 9 * ny + jj;
 #pragma unroll
   for (int ax = 0; ax < 3; ax++) {
+    const float e0 = mat_(M, 3 * ax, x, q, nyz), e1 = mat_(M, 3 * ax + 1, x, q, nyz), e2 = mat_(M, 3 * ax + 2, x, q, nyz);
+    o[(3 * ax + 0) * ny] = (e0 * txx + e1 * txy + e2 * txz) * J;
+    o[(3 * ax + 1) * ny] = (e0 * txy + e1 * tyy + e2 * tyz) * J;
+    o[(3 * ax + 2) * ny] = (e0 * txz + e1 * tyz + e2 * tzz) * J;
+  }
+}
+
+// Free surface, second pass: the traction-image velocity RHS of the three
+// top rows (wave.cu:398-487): DRP derivatives of the traction vectors along
+// each axis, the z stencil zero at the surface and odd-mirrored above it,
+// with the PML 1/beta of each axis; itop[c][row][i][j], row = k - (nk-3).
+template <int FX, int FY, int FZ, class MT>
+__global__ void __launch_bounds__(256) image_top_kernel(WaveArgs<MT> a) {
+  const int ni = a.ni, nj = a.nj, nk = a.nk, ny = a.ny;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)ni * 3 * nj) return;
+  const int j = (int)(idx % nj);
+  const int row = (int)((idx / nj) % 3);
+  const int i = (int)(idx / ((long)nj * 3));
+  const int k = nk - 3 + row, ik = 2 - row, x = i + 3;
+  const uint32_t col = (uint32_t)a.jpad + (uint32_t)(j + 3) + (uint32_t)(k + 3) * (uint32_t)a.py;
+  const MT* M = a.met;
+  const float rrhojac = 1.0f / (mat_(M, 12, x, col, a.nyz) * mat_(M, 9, x, col, a.nyz));
+  float rb[3] = {1.0f, 1.0f, 1.0f};
+  if (a.pml) {
+    const WaveCold* cold = a.cold;
+    rb[0] = 1.0f / cold->pB[0][i];
+    rb[1] = 1.0f / cold->pB[1][j];
+    rb[2] = 1.0f / cold->pB[2][k];
+  }
+  const int fl[3] = {FX, FY, FZ};
+  const int r0 = k - (nk - 6);  // 3, 4, 5
+  float hv[3] = {0.0f, 0.0f, 0.0f};
+  for (int ax = 0; ax < 3; ax++) {
     float v[3][7];
-#pragma unroll
     for (int l = 0; l < 7; l++) {
       const int xq = ax == 0 ? x + l - 3 : x;
-      const uint32_t q = ax == 1 ? col + (l - 3) : ax == 2 ? col + (l - 3) * a.py : col;
-      const float e0 = mat_(M, 3 * ax, xq, q, nyz), e1 = mat_(M, 3 * ax + 1, xq, q, nyz);
-      const float e2 = mat_(M, 3 * ax + 2, xq, q, nyz), J = mat_(M, 9, xq, q, nyz);
-      const float txx = wat(W, 3, xq, q, nyz), tyy = wat(W, 4, xq, q, nyz), tzz = wat(W, 5, xq, q, nyz);
-      const float txy = wat(W, 6, xq, q, nyz), txz = wat(W, 7, xq, q, nyz), tyz = wat(W, 8, xq, q, nyz);
-      v[0][l] = (e0 * txx + e1 * txy + e2 * txz) * J;
-      v[1][l] = (e0 * txy + e1 * tyy + e2 * tyz) * J;
-      v[2][l] = (e0 * txz + e1 * tyz + e2 * tzz) * J;
+      const int r = ax == 2 ? min(r0 + l - 3, 5) : r0;  // rows above the surface: replaced below
+      const int jq = j + 3 + (ax == 1 ? l - 3 : 0);
+      const float* t = a.ttop + ((long)xq * 6 + r) * 9 * ny + (long)(3 * ax) * ny + jq;
+      for (int c = 0; c < 3; c++) v[c][l] = t[(long)c * ny];
     }
     if (ax == 2) {
-#pragma unroll
       for (int c = 0; c < 3; c++) {
         if (ik == 0) { v[c][3] = 0.0f; v[c][6] = -v[c][0]; v[c][5] = -v[c][1]; v[c][4] = -v[c][2]; }
         else if (ik == 1) { v[c][4] = 0.0f; v[c][6] = -v[c][2]; v[c][5] = -v[c][3]; }
         else { v[c][5] = 0.0f; v[c][6] = -v[c][4]; }
       }
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-      d[ax][c] = a.rDH * (fl[ax] > 0 ? drp5<1>(v[c][2], v[c][3], v[c][4], v[c][5], v[c][6])
-                                     : drp5<-1>(v[c][0], v[c][1], v[c][2], v[c][3], v[c][4]));
+    for (int c = 0; c < 3; c++) {
+      const float d = a.rDH * (fl[ax] > 0 ? drp5<1>(v[c][2], v[c][3], v[c][4], v[c][5], v[c][6])
+                                          : drp5<-1>(v[c][0], v[c][1], v[c][2], v[c][3], v[c][4]));
+      hv[c] += d * rb[ax];
+    }
   }
-#pragma unroll
-  for (int c = 0; c < 3; c++) h[c] = (d[0][c] * rb1 + d[1][c] * rb2 + d[2][c] * rb3) * rrhojac;
+  const long plane = (long)ni * nj;
+  for (int c = 0; c < 3; c++) a.itop[((long)c * 3 + row) * plane + (long)i * nj + j] = hv[c] * rrhojac;
 }
 
 // Extra auxiliary-field source of the x/y PML faces on the free surface
@@ -348,7 +387,11 @@ __global__ void __launch_bounds__(NTHR, KIND == 2 ? 2 : 4) wave_stage_kernel(Wav
         h[6] += (e1 * dzv[0] + e0 * dzv[1]) * mu;
         h[7] += (e2 * dzv[0] + e0 * dzv[2]) * mu;
         h[8] += (e2 * dzv[1] + e1 * dzv[2]) * mu;
-        traction_image<FX, FY, FZ, MT>(a, i + 3, col, k, rb1, rb2, rb3, rrho, h);
+        {  // the image velocity RHS (image_top_kernel)
+          const long plane = (long)ni * nj, o = ((long)(k - (nk - 3))) * plane + (long)i * nj + j;
+#pragma unroll
+          for (int c = 0; c < 3; c++) h[c] = a.itop[(long)c * 3 * plane + o];
+        }
       }
 
       // ADE-PML (pmlx.cu:126-210 and the y/z variants): the RHS of a point
@@ -526,6 +569,11 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   if (n[3] > 0) {
     a.work = w + 2 * o3; a.nwork = n[3];
     pr.begin(s, PK_STAGE_SURFACE);
+    if (a.freenode) {
+      const long n1 = (long)c.d.nx * 6 * c.d.ny, n2 = (long)c.d.ni * 3 * c.d.nj;
+      hipLaunchKernelGGL(traction_top_kernel<MT>, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s, a, c.d.nx);
+      hipLaunchKernelGGL((image_top_kernel<FX, FY, FZ, MT>), dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, a);
+    }
     hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
     pr.end(s);
   }
@@ -569,6 +617,8 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
   a.met = (const MT*)c.met;
   a.slabH = c.slabH;
   a.rhs = rhs_out;
+  a.ttop = c.trac_top;
+  a.itop = c.img_top;
   a.cold = (const WaveCold*)c.wave_cold;
   bool any = false;
   for (int f = 0; f < 6; f++) any |= c.pml.on[f] != 0;
@@ -636,6 +686,12 @@ int upload_wave_cold(Ctx& c) {
     c.allocations.push_back(c.wave_cold);
   }
   if (hipMemcpy(c.wave_cold, &w, sizeof w, hipMemcpyHostToDevice) != hipSuccess) return 1;
+  if (c.p.FreeSurface && !c.trac_top) {  // the image stencil reaches six rows down (NZ >= 7)
+    if (hipMalloc(&c.trac_top, sizeof(float) * 9 * 6 * (size_t)c.d.nx * c.d.ny) != hipSuccess) return 1;
+    c.allocations.push_back(c.trac_top);
+    if (hipMalloc(&c.img_top, sizeof(float) * 9 * (size_t)c.d.ni * c.d.nj) != hipSuccess) return 1;
+    c.allocations.push_back(c.img_top);
+  }
 
   const Params& p = c.p;
   const int n3[3] = {c.d.ni, c.d.nj, c.d.nk};
