@@ -13,7 +13,8 @@
       same configuration)
 
 Kernel classes: stage_interior / stage_band / stage_surface (the three
-instantiations of the wave stage kernel), pml_aux, fault_pre, fault_rs, fault_deriv,
+instantiations of the wave stage kernel; stage_surface includes the two
+free-surface precompute passes), pml_aux, fault_pre, fault_rs, fault_deriv,
 fault_post, rup_sensor; everything else is listed under its own name.
 """
 import argparse
@@ -36,6 +37,8 @@ def kernel_class(name):
     m = re.search(r"wave_stage_kernelI(?:Li-?\d+E|Lin1E){3}[ft]Li([012])E", name)  # mangled form
     if m:
         return STAGE_CLASSES[int(m.group(1))]
+    if "traction_top_kernel" in name or "image_top_kernel" in name:
+        return "stage_surface"  # the free-surface precompute passes
     for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_rs_kernel", "fault_deriv_kernel", "fault_post_kernel"):
         if k in name:
             return k[: -len("_kernel")]
