@@ -249,11 +249,18 @@ def main():
         kw["seed"] = args.seed
     case = gen(**kw)
     p = case["params"]
+    note = None
+    if p.get("NewtonMode") == 2 and not p.get("NewtonMLP"):
+        # no network given: the step runs with plain Newton (same residual rule)
+        p["NewtonMode"] = 1
+        note = "NewtonMode 1 (no learned-guess network given)"
     solver = dyn.Solver(p)
     configs.apply(case, solver)
+    t_setup = time.perf_counter()
     solver.setup()
     d = solver.dims()
     npts = d["ni"] * d["nj"] * d["nk"]
+    print(f"bench: setup of {npts} points in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
 
     for it in range(args.warmup):
         solver.step(it)
@@ -281,7 +288,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (seeded generators, dynfd3dcuda_amd.configs)",
-        "config": {"workload": workload_name(args.config, p), "grid_points": npts, "B_alg_bytes_per_pt_step": ba,
+        "config": {"workload": workload_name(args.config, p) + (f"; {note}" if note else ""),
+                   "grid_points": npts, "B_alg_bytes_per_pt_step": ba,
                    "step_hbm_roofline_frac": value * ba / (PEAK_HBM_GBS * 1e9),
                    "stage_points_per_launch": {k: v[2] for k, v in stage.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
