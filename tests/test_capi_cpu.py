@@ -132,3 +132,15 @@ def test_python_wrapper_flags_table():
     s = set(dyn.FD_FLAGS)
     assert len(s) == 8
     assert all(tuple(-x for x in t) in s for t in s)
+
+
+def test_cli_missing_params_file(tmp_path):
+    """The command-line driver ends with -1 when the params file is missing
+    (set_params.cu:108-132); nothing touches a GPU before that."""
+    import subprocess
+    cli = os.path.join(ROOT, "dynfd3dcuda_amd", "cgfdm")
+    if not os.path.exists(cli):
+        pytest.skip("cgfdm not built")
+    r = subprocess.run([cli, "no_such_params.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255, (r.returncode, r.stdout)
+    assert "no_such_params.json" in r.stdout
