@@ -37,7 +37,8 @@ constexpr int NTHR = 512;  // threads per block
 constexpr int TJ = 64;     // default tile: one wavefront per k-row, j contiguous
 constexpr int TK = NTHR / TJ;
 constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j x 16 k)
-constexpr int TKN = NTHR / TJN;
+constexpr int NTHRN = 256;  // threads per block of the narrow tiles
+constexpr int TKN = NTHRN / TJN;
 
 
 // Volume arrays are plane-interleaved with padded rows: word (field c,
@@ -230,12 +231,12 @@ __device__ __forceinline__ float pml_fs_term(int c, const float* z1, float ztx, 
 // code: the host gives it only points where every absorbing profile is
 // trivial), 1 absorbing bands (PML correction, Cerjan), 2 tiles holding the
 // free-surface rows (everything).
-template <int FX, int FY, int FZ, class MT, int KIND, int TJ_ = TJ>
-__global__ void __launch_bounds__(NTHR, KIND == 2 ? 2 : 4) wave_stage_kernel(WaveArgs<MT> a) {
-  constexpr int TJ = TJ_, TK = NTHR / TJ_;  // tile of TJ (j) x TK (k) columns
+template <int FX, int FY, int FZ, class MT, int KIND, int TJ_ = TJ, int NT_ = NTHR>
+__global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(WaveArgs<MT> a) {
+  constexpr int TJ = TJ_, TK = NT_ / TJ_;  // tile of TJ (j) x TK (k) columns
   constexpr int LJ = TJ + 4, LK = TK + 4;   // LDS tile with halos
   constexpr int NH = 4 * LJ + 4 * TK;       // halo cells of one plane: 4 rows + 4 side cells per row
-  static_assert(NH <= NTHR, "one halo cell per thread");
+  static_assert(NH <= NT_, "one halo cell per thread");
   constexpr int HJL = FY > 0 ? 1 : 3;    // halo width below j (the stencil reaches HJL back, 4-HJL forward)
   constexpr int HKL = FZ > 0 ? 1 : 3;
   constexpr int QLO = FX > 0 ? -1 : -3;  // queue holds planes i+QLO .. i+QLO+4
@@ -562,7 +563,7 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     }
     if (n[2] > 0) {
       a.work = w + 2 * o2; a.nwork = n[2];
-      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1, TJN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, s, a);
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1, TJN, NTHRN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, s, a);
     }
     pr.end(s);
   }
