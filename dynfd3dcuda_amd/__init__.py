@@ -26,7 +26,7 @@ FD_FLAGS = [(-1, -1, -1), (1, 1, -1), (1, 1, 1), (-1, -1, 1),
 EXPORTED = [
     "cgfdm_last_error", "cgfdm_create", "cgfdm_create_from_string", "cgfdm_set_param",
     "cgfdm_get_param", "cgfdm_destroy", "cgfdm_set_fault_geometry", "cgfdm_set_coord3d",
-    "cgfdm_set_topography", "cgfdm_set_media1d", "cgfdm_set_init_stress", "cgfdm_setup", "cgfdm_step", "cgfdm_stage",
+    "cgfdm_set_topography", "cgfdm_set_media1d", "cgfdm_set_media3d", "cgfdm_set_init_stress", "cgfdm_setup", "cgfdm_step", "cgfdm_stage",
     "cgfdm_run", "cgfdm_synchronize", "cgfdm_wave_stage", "cgfdm_wave_rhs", "cgfdm_field_info",
     "cgfdm_get_field", "cgfdm_put_field", "cgfdm_get_dims", "cgfdm_open_outputs",
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
@@ -74,6 +74,7 @@ def load_library(path=None):
     lib.cgfdm_set_coord3d.argtypes = [c_p, c_p, ctypes.c_size_t]
     lib.cgfdm_set_topography.argtypes = [c_p, c_p, ctypes.c_size_t]
     lib.cgfdm_set_media1d.argtypes = [c_p, ctypes.c_int, c_p, c_p, c_p, c_p]
+    lib.cgfdm_set_media3d.argtypes = [c_p, c_p, c_p, c_p, ctypes.c_size_t]
     lib.cgfdm_set_init_stress.argtypes = [c_p, ctypes.c_char_p, c_p, ctypes.c_size_t]
     lib.cgfdm_setup.argtypes = [c_p]
     lib.cgfdm_step.argtypes = [c_p, ctypes.c_int]
@@ -178,6 +179,11 @@ class Solver:
     def set_media1d(self, depth, rho, vp, vs):
         arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (depth, rho, vp, vs)]
         self._check(self.lib.cgfdm_set_media1d(self.h, len(arrs[0]), *[a.ctypes.data for a in arrs]))
+
+    def set_media3d(self, vp, vs, rho):
+        """Media3D on the interior grid, [NX][NZ][NY] (media.cu:140-204)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).ravel() for a in (vp, vs, rho)]
+        self._check(self.lib.cgfdm_set_media3d(self.h, *[a.ctypes.data for a in arrs], arrs[0].size))
 
     def set_init_stress(self, name, data):
         a = np.ascontiguousarray(data, dtype=np.float32).ravel()

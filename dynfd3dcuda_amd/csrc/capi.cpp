@@ -261,6 +261,15 @@ int cgfdm_set_media1d(cgfdm_ctx* h, int n, const float* depth, const float* rho,
   return 0;
 }
 
+int cgfdm_set_media3d(cgfdm_ctx* h, const float* vp, const float* vs, const float* rho, size_t n) {
+  if (!h || !vp || !vs || !rho) return fail(CGFDM_EPARAM, "null argument");
+  Ctx& c = h->c;
+  c.p.derive();
+  if (n != (size_t)c.p.ni * c.p.nj * c.p.nk) return fail(CGFDM_ESIZE, "Media3D needs NX*NY*NZ values");
+  c.m3d_vp.assign(vp, vp + n); c.m3d_vs.assign(vs, vs + n); c.m3d_rho.assign(rho, rho + n);
+  return 0;
+}
+
 int cgfdm_set_init_stress(cgfdm_ctx* h, const char* name, const float* data, size_t n) {
   if (!h || !name || !data) return fail(CGFDM_EPARAM, "null argument");
   static const char* ok[] = {"Tx", "Ty", "Tz", "dTx", "dTy", "dTz", "mu_s", "mu_d", "Dc", "C0", "a", "b", "L", "Vw", "State"};

@@ -149,6 +149,7 @@ struct Ctx {
   std::vector<double> coord3d;
   std::vector<float> topo;       // [ni][nj] surface height (extension)
   std::vector<float> m1d_depth, m1d_rho, m1d_vp, m1d_vs;
+  std::vector<float> m3d_vp, m3d_vs, m3d_rho;  // Media3D [ni][nk][nj]
   std::map<std::string, std::vector<float>> stress_in;
   // device volume buffers
   float *S = nullptr, *A = nullptr, *B = nullptr, *Tacc = nullptr;

@@ -127,6 +127,17 @@ int host_setup(Ctx& c, std::string& err) {
   }
   if (!p.Media1D.empty() && c.m1d_depth.empty())
     if (!read_media1d(p.Media1D, c, err)) return 2;
+  if (!p.Media3D.empty() && c.m3d_vp.empty()) {  // media.cu:157-167: vp, vs, rho (NX, NZ, NY)
+    NcFile nf;
+    if (!nf.open_read(p.Media3D, err)) return 4;
+    if (!nf.read_float("vp", c.m3d_vp, err) || !nf.read_float("vs", c.m3d_vs, err) || !nf.read_float("rho", c.m3d_rho, err))
+      return 4;
+  }
+  if (!c.m3d_vp.empty() && (c.m3d_vp.size() < (size_t)ni * nj * nk || c.m3d_vs.size() < (size_t)ni * nj * nk ||
+                            c.m3d_rho.size() < (size_t)ni * nj * nk)) {
+    err = "Media3D smaller than NX*NY*NZ";
+    return 4;
+  }
   if (p.Rupture && p.INPORT_STRESS_TYPE == 1 && c.stress_in.find("Tx") == c.stress_in.end()) {
     NcFile nf;
     if (!nf.open_read(p.Fault_init_stress, err)) return 4;
@@ -252,6 +263,11 @@ int host_setup(Ctx& c, std::string& err) {
             double r = c.m1d_rho[nl - 1], va = c.m1d_vp[nl - 1], vb = c.m1d_vs[nl - 1];
             for (int l = nl - 1; l >= 0; l--)
               if (depth1 > c.m1d_depth[l]) { r = c.m1d_rho[l]; va = c.m1d_vp[l]; vb = c.m1d_vs[l]; break; }
+            lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
+          }
+          if (!c.m3d_vp.empty()) {  // init_media3d (media.cu:140-204)
+            const long q3 = (j - 3) + (long)(k - 3) * nj + (long)(i - 3) * nj * nk;
+            const double r = c.m3d_rho[q3], va = c.m3d_vp[q3], vb = c.m3d_vs[q3];
             lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
           }
           Md[10 * nxyz + q] = lam; Md[11 * nxyz + q] = miu; Md[12 * nxyz + q] = rho;

@@ -65,6 +65,10 @@ int cgfdm_set_coord3d(cgfdm_ctx* ctx, const double* xyz, size_t n);
 int cgfdm_set_topography(cgfdm_ctx* ctx, const float* h, size_t n);
 /* Media1D layers (srcd/media.cu:63-138). */
 int cgfdm_set_media1d(cgfdm_ctx* ctx, int nlayer, const float* depth, const float* rho, const float* vp, const float* vs);
+/* Media3D (srcd/media.cu:140-204): vp, vs, rho on the interior grid,
+ * [NX][NZ][NY] (the variables of the Media3D netCDF file); overrides the
+ * bimaterial and layered media. */
+int cgfdm_set_media3d(cgfdm_ctx* ctx, const float* vp, const float* vs, const float* rho, size_t n);
 /* Fault_init_stress variable by its netCDF name: Tx Ty Tz dTx dTy dTz mu_s
  * mu_d Dc C0 a b L Vw State, [nk][nj] (srcd/init.cu:904-1022). */
 int cgfdm_set_init_stress(cgfdm_ctx* ctx, const char* name, const float* data, size_t n);

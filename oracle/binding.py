@@ -94,6 +94,10 @@ class Oracle:
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (depth, rho, vp, vs)]
         assert self.lib.orc_set_media1d(self.h, len(a[0]), *[x.ctypes.data for x in a]) == 0
 
+    def set_media3d(self, vp, vs, rho):
+        a = [np.ascontiguousarray(v, dtype=np.float64).ravel() for v in (vp, vs, rho)]
+        assert self.lib.orc_set_media3d(self.h, *[x.ctypes.data for x in a], a[0].size) == 0
+
     def set_stress(self, name, v):
         a = np.ascontiguousarray(v, dtype=np.float64).ravel()
         assert self.lib.orc_set_stress(self.h, name.encode(), a.ctypes.data, a.size) == 0

@@ -215,3 +215,76 @@ def test_metric_bf16_storage():
     errs = {n: relerr(s.get(n), o.get(n)) for n in ("slip", "Vs1", "Ts1", "Tn")}
     print("bf16 metric, 40 steps, rel L2 vs FP64 oracle:", errs)
     assert errs["slip"] < 2e-2 and errs["Ts1"] < 1e-2 and errs["Tn"] < 1e-2, errs
+
+
+def test_media3d_input(tmp_path):
+    """Media3D (media.cu:140-204): a heterogeneous medium given on the
+    interior grid, as arrays through the C ABI and as the Media3D netCDF file
+    (vp, vs, rho with dims NX, NZ, NY, written by scipy), gives the oracle's
+    lambda, mu, rho (ghosts mirrored) and the same three RK4 steps: 1e-6 on
+    the coefficients, 1e-5 on the wavefield."""
+    from scipy.io import netcdf_file
+    case = cases.volume_case(curved=True)
+    p = case["params"]
+    NX, NY, NZ = p["NX"], p["NY"], p["NZ"]
+    rng = np.random.default_rng(21)
+    ii, kk, jj = np.meshgrid(np.arange(NX), np.arange(NZ), np.arange(NY), indexing="ij")
+    vp = 5000.0 + 800.0 * np.sin(ii / 5.0) * np.cos(jj / 7.0) + 300.0 * kk / NZ
+    vs = vp / np.sqrt(3.0) * (1.0 + 0.05 * rng.uniform(-1, 1, vp.shape))
+    rho = 2500.0 + 200.0 * np.cos(kk / 4.0)
+    path = str(tmp_path / "media3d.nc")
+    with netcdf_file(path, "w") as f:
+        for n, d in (("nx", NX), ("nz", NZ), ("ny", NY)):
+            f.createDimension(n, d)
+        for n, a in (("vp", vp), ("vs", vs), ("rho", rho)):
+            v = f.createVariable(n, "f", ("nx", "nz", "ny"))
+            v[:] = a.astype(np.float32)
+    o = Oracle(p)
+    cases.apply_to_oracle(case, o)
+    o.set_media3d(vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32))
+    o.setup()
+    d = o.dims()
+    m_o = o.get("M").reshape(13, d["nx"], d["nz"], d["ny"])
+    W = cases.random_wavefield(d["nx"], d["ny"], d["nz"], seed=6)
+    o.put("W", W)
+    for it in range(3):
+        o.step(it)
+    w_o = interior(o.get("W"), d, 9)
+    for via_file in (False, True):
+        q = dict(p, Media3D=path) if via_file else p
+        s = dyn.Solver(q)
+        cases.apply_to_solver(case, s)
+        if not via_file:
+            s.set_media3d(vp, vs, rho)
+        s.setup()
+        m_h = s.get("M").reshape(13, d["nx"], d["nz"], d["ny"])
+        for c in (10, 11, 12):
+            assert relerr(m_h[c], m_o[c]) < 1e-6, (via_file, c)
+        s.put("W", W)
+        for it in range(3):
+            s.step(it)
+        w_h = interior(s.get("W"), d, 9)
+        for c in range(9):
+            e = relerr(w_h[c], w_o[c])
+            assert e < 1e-5, (via_file, c, e)
+
+
+def test_fault_rs_slip_law_steps():
+    """Rate-and-state slip law with flash weakening (Friction_type 2,
+    rk4.cu:902-911): the state rate -V/L (psi - psi_ss(V)) with the flash-
+    weakened steady state.  L = 0.1 m (the 0.02 m of the aging-law case makes
+    the slip-law state equation too stiff for the RK4 step: the oracle itself
+    overflows); after 50 steps the patch is weakening dynamically (slip rate
+    ~0.25 m/s) and the bounds are those of the aging-law test."""
+    case = cases.rs_case(RS_L=0.1)
+    case["params"]["Friction_type"] = 2
+    case["stress"]["L"] = np.full_like(case["stress"]["L"], 0.1)
+    s, o = make_pair(case)
+    for it in range(50):
+        s.step(it)
+        o.step(it)
+    assert np.max(np.abs(o.get("Vs1"))) > 0.1  # nucleated
+    for name, tol in (("Vs1", 1e-3), ("Ts1", 1e-4), ("Tn", 1e-4), ("slip", 1e-3), ("State", 1e-4)):
+        e = relerr(s.get(name), o.get(name))
+        print(name, e)
+        assert e < tol, (name, e)
