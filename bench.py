@@ -29,7 +29,7 @@ FP32).  Prints one JSON line:
                  then both runs are timed over the same window of steps
                  inside the rupture (GPU time per RK stage of the Newton
                  kernel and of the whole fault part, iterations per solve)
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-rs]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload curved_sw_400x300x200] [--no-rs]
 """
 import argparse
 import json
@@ -43,7 +43,7 @@ if ROOT not in sys.path:
 
 METRIC = "grid-pt updates/s + HBM roofline %; RS fault step time MLP-Newton vs plain"
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_config{cfg}.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_{workload}.txt")
 
 
 def b_alg(params):
@@ -52,19 +52,20 @@ def b_alg(params):
     return 144 * 4 + 48 * (2 if params.get("MetricBF16") else 4)
 
 
-def workload_name(cfg, p):
-    names = {0: "plain flat grid, slip weakening, Cerjan sponge",
-             1: "curved grid + topography, bent slip-weakening fault, 3-layer medium, 20-layer ADE-PML",
-             2: "curved grid, rate-and-state aging law, plain Newton",
-             3: "curved grid, rate-and-state aging law, learned Newton guess",
-             4: "large grid, rate-and-state, BF16 metric/media"}
-    return f"config{cfg}: {p['NX']}x{p['NY']}x{p['NZ']} at {p['DH']:g} m, {names.get(cfg, '')}"
+def workload_name(name, p):
+    what = {"flat_sw_100x200x100": "plain flat grid, slip weakening, Cerjan sponge",
+            "curved_sw_400x300x200": "curved grid + topography, bent slip-weakening fault, 3-layer medium, "
+                                     "20-layer ADE-PML",
+            "curved_rs_400x300x200": "curved grid, rate-and-state aging law, plain Newton",
+            "curved_rs_mlp_400x300x200": "curved grid, rate-and-state aging law, learned Newton guess",
+            "large_rs_mlp_bf16_1600x1000x500": "large grid, rate-and-state, BF16 metric/media"}
+    return f"{name}: {p['NX']}x{p['NY']}x{p['NZ']} at {p['DH']:g} m, {what.get(name, '')}"
 
 
-def read_traffic(cfg, kernel):
+def read_traffic(workload, kernel):
     """HBM bytes of `kernel` per RK stage (one launch of the interior
     class) from the committed rocprofv3 summary."""
-    path = TRAFFIC_FILE.format(cfg=cfg)
+    path = TRAFFIC_FILE.format(workload=workload)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -157,7 +158,7 @@ def rate_state_leg(args):
     res = {}
     params = None
     for mode in (1, 2):
-        case = configs.config2(nt=lead + window + 2)
+        case = configs.curved_rs_400x300x200(nt=lead + window + 2)
         case["params"].update({"NewtonMode": mode, "NewtonTol": 1e-10, "NewtonFallbackIter": 20})
         s = dyn.Solver(case["params"])
         configs.apply(case, s)
@@ -198,7 +199,7 @@ def rate_state_leg(args):
             fault_nodes = d["nj"] * d["nk"]
         s.close()
     pl, ml = res[1], res[2]
-    return {"workload": f"config2 grid with the aging law, {fault_nodes} fault nodes; steps {lead}..{lead + window - 1}",
+    return {"workload": f"curved_rs_400x300x200: the curved 400x300x200 grid with the aging law, {fault_nodes} fault nodes; steps {lead}..{lead + window - 1}",
             "plain": pl, "learned": ml, "train": res["train"],
             "newton_kernel_speedup": pl["newton_kernel_ms_per_stage"] / ml["newton_kernel_ms_per_stage"],
             "fault_step_speedup": pl["fault_ms_per_stage"] / ml["fault_ms_per_stage"]}
@@ -209,7 +210,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=1, help="synthetic configuration (0-4)")
+    ap.add_argument("--workload", default="curved_sw_400x300x200",
+                    help="synthetic workload, one of dynfd3dcuda_amd.configs.WORKLOADS")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline timing budget")
@@ -243,7 +245,9 @@ def main():
     import dynfd3dcuda_amd as dyn
     from dynfd3dcuda_amd import configs
 
-    gen = configs.CONFIGS[args.config]
+    if args.workload not in configs.WORKLOADS:
+        ap.error(f"unknown workload {args.workload}; choose from {', '.join(configs.WORKLOADS)}")
+    gen = configs.WORKLOADS[args.workload]
     kw = {"nt": args.steps + args.warmup + 2}
     if args.seed is not None:
         kw["seed"] = args.seed
@@ -282,13 +286,13 @@ def main():
     avg_s = ms / n / 1e3
     bytes_per_launch = pts * ba / 4.0  # one launch = one RK stage of its points
     achieved = bytes_per_launch / avg_s / 1e9
-    traffic = read_traffic(args.config, dom)
+    traffic = read_traffic(args.workload, dom)
     out = {
         "metric": METRIC, "value": value, "unit": "grid-pt updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (seeded generators, dynfd3dcuda_amd.configs)",
-        "config": {"workload": workload_name(args.config, p) + (f"; {note}" if note else ""),
+        "config": {"workload": workload_name(args.workload, p) + (f"; {note}" if note else ""),
                    "grid_points": npts, "B_alg_bytes_per_pt_step": ba,
                    "step_hbm_roofline_frac": value * ba / (PEAK_HBM_GBS * 1e9),
                    "stage_points_per_launch": {k: v[2] for k, v in stage.items()}},

@@ -193,69 +193,70 @@ def _params(NX, NY, NZ, DH, DT, nt, **kw):
     return p
 
 
-def config0(seed=0, nt=1500):
+def flat_sw_100x200x100(seed=0, nt=1500):
     """Plain method, small: NX=100 (normal) x NY=200 (strike) x NZ=100 (dip)
     at 100 m (2.0 M points), flat surface with identity metric, homogeneous
     medium, slip weakening, Cerjan sponge, DT 0.008 s."""
     rng = np.random.default_rng(seed)
     NX, NY, NZ, DH, DT = 100, 200, 100, 100.0, 0.008
     p = _params(NX, NY, NZ, DH, DT, nt, usePML=0, useCerjan=1, NormalRefine=0)
-    return {"name": "config0", "params": p, "geom": flat_geometry(NY, NZ, DH),
+    return {"name": "flat_sw_100x200x100", "params": p, "geom": flat_geometry(NY, NZ, DH),
             "stress": _sw_stress(NY, NZ, DH, rng, 70e6, 3e6, 1e3, 3.0)}
 
 
-def config1(seed=1, nt=4000, scale=1):
+def curved_sw_400x300x200(seed=1, nt=4000, scale=1):
     """Curved grid: 400x300x200 at 50 m, topography, bent fault, 3-layer
     medium, slip weakening, 20-layer ADE-PML, FP32."""
     rng = np.random.default_rng(seed)
     NX, NY, NZ, DH, DT = 400 * scale, 300 * scale, 200 * scale, 50.0 / scale, 0.003 / scale
     p = _params(NX, NY, NZ, DH, DT, nt)
     geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
-    return {"name": "config1", "params": p, "geom": geom,
+    return {"name": "curved_sw_400x300x200", "params": p, "geom": geom,
             "media1d": three_layer_media(),
             "topo": sum_of_gaussians_topography(NX, NY, DH, rng),
             "stress": _sw_stress(NY, NZ, DH, rng, 70e6, 3e6, 1e3, 3.0, frame=fault_frame(geom, DH))}
 
 
-def config2(seed=2, nt=4000, scale=1):
-    """Rate-and-state aging law with plain Newton on the config-1 grid."""
-    c = config1(seed=seed, nt=nt, scale=scale)
+def curved_rs_400x300x200(seed=2, nt=4000, scale=1):
+    """Rate-and-state aging law with plain Newton on the curved 400x300x200 grid."""
+    c = curved_sw_400x300x200(seed=seed, nt=nt, scale=scale)
     rng = np.random.default_rng(seed + 100)
     p = c["params"]
     p.update({"Friction_type": 1, "RS_V0": 1e-6, "RS_f0": 0.6, "RS_Vini": 1e-12, "RS_L": 0.02,
               "RS_fw": 0.2, "smooth_load_T": 1.0})
     c["stress"] = _rs_stress(p["NY"], p["NZ"], p["DH"], rng, frame=fault_frame(c["geom"], p["DH"]))
-    c["name"] = "config2"
+    c["name"] = "curved_rs_400x300x200"
     return c
 
 
-def config3(seed=2, nt=4000, scale=1, mlp_path=None):
-    """Config 2 with the learned Newton initial guess."""
-    c = config2(seed=seed, nt=nt, scale=scale)
+def curved_rs_mlp_400x300x200(seed=2, nt=4000, scale=1, mlp_path=None):
+    """The curved-grid rate-and-state workload with the learned Newton initial guess."""
+    c = curved_rs_400x300x200(seed=seed, nt=nt, scale=scale)
     c["params"].update({"NewtonMode": 2, "NewtonTol": 1e-10, "NewtonFallbackIter": 20})
     if mlp_path:
         c["params"]["NewtonMLP"] = mlp_path
-    c["name"] = "config3"
+    c["name"] = "curved_rs_mlp_400x300x200"
     return c
 
 
-def config4(seed=4, nt=8000):
+def large_rs_mlp_bf16_1600x1000x500(seed=4, nt=8000):
     """Large, mixed precision: 1600x1000x500 at 25 m, rate-state + learned
     guess, bf16 metric/media storage."""
-    c = config3(seed=seed, nt=nt)
+    c = curved_rs_mlp_400x300x200(seed=seed, nt=nt)
     rng = np.random.default_rng(seed)
     NX, NY, NZ, DH, DT = 1600, 1000, 500, 25.0, 0.0015
     p = _params(NX, NY, NZ, DH, DT, nt, Friction_type=1, RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12,
                 RS_L=0.02, RS_fw=0.2, smooth_load_T=1.0, MetricBF16=1, NewtonMode=2,
                 NewtonTol=1e-10, NewtonFallbackIter=20)
     geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
-    return {"name": "config4", "params": p, "geom": geom,
+    return {"name": "large_rs_mlp_bf16_1600x1000x500", "params": p, "geom": geom,
             "media1d": three_layer_media(),
             "topo": sum_of_gaussians_topography(NX, NY, DH, rng),
             "stress": _rs_stress(NY, NZ, DH, rng, frame=fault_frame(geom, DH))}
 
 
-CONFIGS = {0: config0, 1: config1, 2: config2, 3: config3, 4: config4}
+WORKLOADS = {f.__name__: f for f in (flat_sw_100x200x100, curved_sw_400x300x200, curved_rs_400x300x200,
+                                      curved_rs_mlp_400x300x200, large_rs_mlp_bf16_1600x1000x500)}
 
 
 def apply(case, solver):

@@ -8,7 +8,7 @@ Samples are what the plain Newton solve produces in each rate-state trial
 (trial_ratestate.cu:686-702): the four guess features of every fault node and
 the log10 slip rate it solved for (solver field "RS_samples").
 
-    python -m dynfd3dcuda_amd.mlp train --config 2 --steps 400 --out weights.bin
+    python -m dynfd3dcuda_amd.mlp --workload curved_rs_400x300x200 --steps 400 --out weights.bin
 """
 import argparse
 import ctypes
@@ -120,7 +120,8 @@ def load(path):
 def main(argv=None):
     from . import Solver, configs
     ap = argparse.ArgumentParser(description="collect rate-state samples with plain Newton and train the guess network")
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--workload", default="curved_rs_400x300x200",
+                    help="rate-state workload of dynfd3dcuda_amd.configs.WORKLOADS")
     ap.add_argument("--seeds", type=int, nargs="+", default=[0])
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--every", type=int, default=4)
@@ -129,7 +130,7 @@ def main(argv=None):
     a = ap.parse_args(argv)
     F, T = [], []
     for seed in a.seeds:
-        case = getattr(configs, f"config{a.config}")(seed=seed)
+        case = configs.WORKLOADS[a.workload](seed=seed)
         s = Solver(case["params"], NewtonMode=1)
         configs.apply(case, s)
         s.setup()

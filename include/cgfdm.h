@@ -104,7 +104,8 @@ int cgfdm_wave_stage(cgfdm_ctx* ctx, int irk, int fx, int fy, int fz);
  * plus abs_deriv_{x,y,z} contributions when PML is on). */
 int cgfdm_wave_rhs(cgfdm_ctx* ctx, int fx, int fy, int fz);
 
-/* Host copies of named fields (see DESIGN.md for the list): "W" (current
+/* Host copies of named fields (cgfdm_field_info reports each one's element
+ * count and type, and rejects an unknown name): "W" (current
  * wavefield, 9 x nx*ny*nz, float), "hW", "M" (13 fields), "FW" (split nodes),
  * fault 2-D outputs "Vs1" "Vs2" "Tn" "Ts1" "Ts2" "slip" "State" "init_t0"
  * "rup_sensor" ..., PML aux "PMLx1".."PMLz2".  dtype: 0 float32, 1 float64,
