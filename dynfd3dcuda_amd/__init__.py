@@ -31,6 +31,8 @@ EXPORTED = [
     "cgfdm_get_field", "cgfdm_put_field", "cgfdm_get_dims", "cgfdm_open_outputs",
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
     "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
+    "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
+    "cgfdm_rup_sensor", "cgfdm_fault_smooth",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
@@ -99,6 +101,13 @@ def load_library(path=None):
     lib.cgfdm_mlp_grad.argtypes = [c_p, c_p, c_p, ctypes.c_long, c_p, c_p]
     lib.cgfdm_mlp_train.argtypes = [c_p, c_p, c_p, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_uint, c_p]
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.cgfdm_stage_flags.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip, ip]
+    lib.cgfdm_set_stream.argtypes = [c_p, c_p]
+    for name in ("cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update"):
+        getattr(lib, name).argtypes = [c_p, ctypes.c_int, ctypes.c_int]
+    lib.cgfdm_rup_sensor.argtypes = [c_p]
+    lib.cgfdm_fault_smooth.argtypes = [c_p]
     for name in EXPORTED[1:]:
         getattr(lib, name).restype = ctypes.c_int
     if path is None:
@@ -211,6 +220,44 @@ class Solver:
 
     def wave_rhs(self, fx, fy, fz):
         self._check(self.lib.cgfdm_wave_rhs(self.h, fx, fy, fz))
+
+    # ---- one stage a phase at a time (the reference's launcher set,
+    # main.cu:448-515): fault_trial, wave_stage, fault_deriv, fault_update
+    def fault_trial(self, it, irk):
+        self._check(self.lib.cgfdm_fault_trial(self.h, it, irk))
+
+    def fault_deriv(self, it, irk):
+        self._check(self.lib.cgfdm_fault_deriv(self.h, it, irk))
+
+    def fault_update(self, it, irk):
+        self._check(self.lib.cgfdm_fault_update(self.h, it, irk))
+
+    def rup_sensor(self):
+        self._check(self.lib.cgfdm_rup_sensor(self.h))
+
+    def fault_smooth(self):
+        self._check(self.lib.cgfdm_fault_smooth(self.h))
+
+    def stage_by_phases(self, it, irk):
+        """The phases of one stage in cgfdm_stage's order."""
+        fx, fy, fz = self.stage_flags_lib(it, irk)
+        if self.get_param("Rupture"):
+            self.fault_trial(it, irk)
+        self.wave_stage(irk, fx, fy, fz)
+        if self.get_param("Rupture"):
+            self.fault_deriv(it, irk)
+            self.fault_update(it, irk)
+
+    def stage_flags_lib(self, it, irk):
+        f = [ctypes.c_int() for _ in range(3)]
+        self._check(self.lib.cgfdm_stage_flags(it, irk, *[ctypes.byref(x) for x in f]))
+        return tuple(x.value for x in f)
+
+    def set_stream(self, stream):
+        """Queue on `stream`, a hipStream_t (as int) created with the HIP
+        runtime this library is linked against, or None for the solver's own
+        stream."""
+        self._check(self.lib.cgfdm_set_stream(self.h, ctypes.c_void_p(stream) if stream else None))
 
     # ---- fields
     def field_info(self, name):

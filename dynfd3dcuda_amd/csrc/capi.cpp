@@ -226,7 +226,8 @@ int cgfdm_destroy(cgfdm_ctx* h) {
   Ctx& c = h->c;
   if (c.out) { cgfdm_close_outputs(h); }
   if (c.setup_done) hipSetDevice(c.device);
-  if (c.stream) { hipStreamSynchronize(c.stream); hipStreamDestroy(c.stream); }
+  if (c.stream) hipStreamSynchronize(c.stream);
+  if (c.own_stream) { hipStreamSynchronize(c.own_stream); hipStreamDestroy(c.own_stream); }
   for (void* p : c.allocations) hipFree(p);
   if (c.hW) hipFree(c.hW);
   if (c.wave_work) hipFree(c.wave_work);
@@ -295,7 +296,9 @@ int cgfdm_setup(cgfdm_ctx* h) {
     int code = rc == 5 ? CGFDM_ECFL : rc == 4 ? CGFDM_ENETCDF : rc == 2 ? CGFDM_EFILE : rc == 6 ? CGFDM_EHIP : CGFDM_EPARAM;
     return fail(code, err);
   }
-  if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return fail(CGFDM_EHIP, "stream creation failed");
+  if (hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(CGFDM_EHIP, "stream creation failed");
+  c.stream = c.own_stream;
   c.setup_done = true;
   return 0;
 }
@@ -381,6 +384,83 @@ int cgfdm_wave_rhs(cgfdm_ctx* h, int fx, int fy, int fz) {
   StageBufs b{c.cur, nullptr, c.S, c.Tacc};
   launch_wave_stage(c, b, 0, fx, fy, fz, true, c.hW);
   return check_hip("wave_rhs launch");
+}
+
+int cgfdm_stage_flags(int it, int irk, int* fx, int* fy, int* fz) {
+  if (!fx || !fy || !fz) return fail(CGFDM_EPARAM, "null argument");
+  if (irk < 0 || irk > 3 || it < 0) return fail(CGFDM_EPARAM, "bad stage index");
+  stage_flags(it, irk, *fx, *fy, *fz);
+  return 0;
+}
+
+int cgfdm_set_stream(cgfdm_ctx* h, void* stream) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "set_stream before setup");
+  hipStream_t next = stream ? (hipStream_t)stream : c.own_stream;
+  if (next == c.stream) return 0;
+  // work already queued on the previous stream finishes before any later
+  // launch can touch the same buffers
+  hipError_t e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) return fail(CGFDM_EHIP, hipGetErrorString(e));
+  c.stream = next;
+  return 0;
+}
+
+// Fault phases of one RK stage.  Called in the order cgfdm_stage runs them
+// (trial, wave stage, derivatives, update) they queue the same launches with
+// the same arguments, so the result is the same bit for bit.
+static int fault_phase_ready(cgfdm_ctx* h, int it, int irk, int& fx, int& fy, int& fz) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  if (!c.p.Rupture) return fail(CGFDM_ESTATE, "no fault (Rupture = 0)");
+  if (irk < 0 || irk > 3 || it < 0) return fail(CGFDM_EPARAM, "bad stage index");
+  stage_flags(it, irk, fx, fy, fz);
+  return 0;
+}
+
+int cgfdm_fault_trial(cgfdm_ctx* h, int it, int irk) {
+  int fx, fy, fz;
+  if (int rc = fault_phase_ready(h, it, irk, fx, fy, fz)) return rc;
+  Ctx& c = h->c;
+  StageBufs b = bufs_for(c, irk);
+  launch_fault_pre(c, b.Win, it, irk, fx, fy, fz);
+  if (c.p.Friction_type >= 1) launch_fault_rs(c, b.Win, it, irk, fx, fy, fz);
+  return check_hip("fault trial launch");
+}
+
+int cgfdm_fault_deriv(cgfdm_ctx* h, int it, int irk) {
+  int fx, fy, fz;
+  if (int rc = fault_phase_ready(h, it, irk, fx, fy, fz)) return rc;
+  Ctx& c = h->c;
+  launch_fault_deriv(c, bufs_for(c, irk).Win, fx, fy, fz);
+  return check_hip("fault derivative launch");
+}
+
+int cgfdm_fault_update(cgfdm_ctx* h, int it, int irk) {
+  int fx, fy, fz;
+  if (int rc = fault_phase_ready(h, it, irk, fx, fy, fz)) return rc;
+  Ctx& c = h->c;
+  StageBufs b = bufs_for(c, irk);
+  launch_fault_post(c, b, irk);
+  c.cur = b.Wout;
+  return check_hip("fault update launch");
+}
+
+int cgfdm_rup_sensor(cgfdm_ctx* h) {
+  int fx, fy, fz;
+  if (int rc = fault_phase_ready(h, 0, 0, fx, fy, fz)) return rc;
+  launch_rup_sensor(h->c);
+  return check_hip("rupture sensor launch");
+}
+
+int cgfdm_fault_smooth(cgfdm_ctx* h) {
+  int fx, fy, fz;
+  if (int rc = fault_phase_ready(h, 0, 0, fx, fy, fz)) return rc;
+  if (!h->c.p.FaultSmooth) return fail(CGFDM_ESTATE, "FaultSmooth is off");
+  launch_fault_smooth(h->c);
+  return check_hip("fault smoothing launch");
 }
 
 // Between the caller's layout (the reference's: [field][plane][k][j], rows

@@ -142,7 +142,8 @@ struct Ctx {
   Params p;
   Dims d;
   bool setup_done = false;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // where every launch goes
+  hipStream_t own_stream = nullptr;  // created at setup, destroyed with the context
   int device = 0;
   // host inputs
   std::vector<float> gx, gy, gz;

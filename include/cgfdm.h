@@ -104,6 +104,41 @@ int cgfdm_wave_stage(cgfdm_ctx* ctx, int irk, int fx, int fy, int fz);
  * plus abs_deriv_{x,y,z} contributions when PML is on). */
 int cgfdm_wave_rhs(cgfdm_ctx* ctx, int fx, int fy, int fz);
 
+/* The FD flag triple of stage irk of step it: row it % 8 of the 8-row table
+ * (main.cu:86-94, 440-442), negated on odd stages (main.cu:514). */
+int cgfdm_stage_flags(int it, int irk, int* fx, int* fy, int* fz);
+
+/* Queue later launches on the caller's HIP stream (a hipStream_t passed as
+ * void*); NULL returns to the context's own stream.  Work already queued on
+ * the previous stream is waited for first.  The reference launches on the
+ * default stream with a device-wide wait per stage (main.cu:505-511). */
+int cgfdm_set_stream(cgfdm_ctx* ctx, void* stream);
+
+/* The fault phases of stage irk of step it, for callers that drive a stage a
+ * phase at a time (the reference's per-launcher interface, main.cu:49-84).
+ * cgfdm_fault_trial, cgfdm_wave_stage (with cgfdm_stage_flags' triple),
+ * cgfdm_fault_deriv and cgfdm_fault_update, called in that order, queue what
+ * cgfdm_stage queues and give the same result bit for bit.  Each returns
+ * CGFDM_ESTATE without a fault (Rupture = 0).
+ *   trial:  state_rk + trial_sw | trial_rs (Newton) + fault2wave
+ *           (main.cu:455-468; rk4.cu:791-938, trial_slipweakening.cu:23-732,
+ *           trial_ratestate.cu:20-893, transform.cu:224-403; the wave2fault
+ *           launched before them returns at once, transform.cu:18);
+ *   deriv:  fault_dvelo + fault_dstrs_{f,b} (main.cu:486-491;
+ *           fault_dvelo.cu:17-389, fault_dstrs_f.cu:17-1254);
+ *   update: RK of the planes i0-3..i0+3 diverted from the volume stage +
+ *           fault_rk + fault_rk_stage2 + fault2wave + Cerjan on those planes
+ *           (main.cu:495-508; rk4.cu:154-266, 466-789). */
+int cgfdm_fault_trial(cgfdm_ctx* ctx, int it, int irk);
+int cgfdm_fault_deriv(cgfdm_ctx* ctx, int it, int irk);
+int cgfdm_fault_update(cgfdm_ctx* ctx, int it, int irk);
+/* cal_rup_sensor (rup_sensor.cu:7-97), run before the stages of a step
+ * (main.cu:444). */
+int cgfdm_rup_sensor(cgfdm_ctx* ctx);
+/* smooth_gauss_volume (smooth.cu:13-367), after the stages of a step when
+ * FaultSmooth is set (main.cu:516-522); CGFDM_ESTATE when it is not. */
+int cgfdm_fault_smooth(cgfdm_ctx* ctx);
+
 /* Host copies of named fields (cgfdm_field_info reports each one's element
  * count and type, and rejects an unknown name): "W" (current
  * wavefield, 9 x nx*ny*nz, float), "hW", "M" (13 fields), "FW" (split nodes),

@@ -38,6 +38,8 @@ def load():
     lib.orc_setup.argtypes = [P]
     lib.orc_step.argtypes = [P, ctypes.c_int]
     lib.orc_stage.argtypes = [P] + [ctypes.c_int] * 5
+    lib.orc_fault_trial.argtypes = [P] + [ctypes.c_int] * 5
+    lib.orc_fault_deriv.argtypes = [P] + [ctypes.c_int] * 3
     lib.orc_rup_sensor.argtypes = [P]
     lib.orc_smooth.argtypes = [P]
     lib.orc_wave_deriv.argtypes = [P] + [ctypes.c_int] * 3
@@ -129,6 +131,15 @@ class Oracle:
 
     def stage(self, it, irk, fx, fy, fz):
         self.lib.orc_stage(self.h, it, irk, fx, fy, fz)
+
+    def fault_trial(self, it, irk, fx, fy, fz):
+        self.lib.orc_fault_trial(self.h, it, irk, fx, fy, fz)
+
+    def fault_deriv(self, fx, fy, fz):
+        self.lib.orc_fault_deriv(self.h, fx, fy, fz)
+
+    def rup_sensor(self):
+        self.lib.orc_rup_sensor(self.h)
 
     def smooth(self):
         self.lib.orc_smooth(self.h)

@@ -1424,13 +1424,17 @@ struct Oracle {
 
   // ---------------------------------------------------------------------------
   // One RK stage in the order of main.cu:448-516.
+  // The fault's part of a stage before the volume derivative
+  // (main.cu:455-468): state_rk, the trial, fault2wave.
+  void fault_trial(int it, int irk, int FX, int FY, int FZ) {
+    state_rk(irk);
+    if (Friction_type == 0) trial_sw(it, irk, FX, FY, FZ);
+    else trial_rs(it, irk, FX, FY, FZ);
+    fault2wave();
+  }
+
   void stage(int it, int irk, int FX, int FY, int FZ) {
-    if (Rupture) {
-      state_rk(irk);
-      if (Friction_type == 0) trial_sw(it, irk, FX, FY, FZ);
-      else trial_rs(it, irk, FX, FY, FZ);
-      fault2wave();
-    }
+    if (Rupture) fault_trial(it, irk, FX, FY, FZ);
     wave_deriv(FX, FY, FZ);
     if (usePML) {
       if (isx1) pml_deriv(0, 0, FX);
@@ -1663,6 +1667,16 @@ int orc_set_stress(void* h, const char* name, const double* v, long n) {
 int orc_setup(void* h) { return ((Oracle*)h)->setup(); }
 int orc_step(void* h, int it) { ((Oracle*)h)->step(it); return 0; }
 int orc_stage(void* h, int it, int irk, int fx, int fy, int fz) { ((Oracle*)h)->stage(it, irk, fx, fy, fz); return 0; }
+int orc_fault_trial(void* h, int it, int irk, int fx, int fy, int fz) {
+  ((Oracle*)h)->fault_trial(it, irk, fx, fy, fz);
+  return 0;
+}
+// fault_dvelo + fault_dstrs_{f,b} (main.cu:486-491)
+int orc_fault_deriv(void* h, int fx, int fy, int fz) {
+  ((Oracle*)h)->fault_dvelo(fx, fy, fz);
+  ((Oracle*)h)->fault_dstrs(fx, fy, fz);
+  return 0;
+}
 int orc_rup_sensor(void* h) { ((Oracle*)h)->rup_sensor_calc(); return 0; }
 int orc_smooth(void* h) { ((Oracle*)h)->smooth_gauss(); return 0; }
 // kernel-level entry points for unit parity tests

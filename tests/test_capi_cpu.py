@@ -144,3 +144,19 @@ def test_cli_missing_params_file(tmp_path):
     r = subprocess.run([cli, "no_such_params.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
     assert r.returncode == 255, (r.returncode, r.stdout)
     assert "no_such_params.json" in r.stdout
+
+
+def test_stage_flags_table():
+    """cgfdm_stage_flags (no GPU needed): row it % 8 of the reference's table
+    (main.cu:86-94), negated on odd stages; bad indices rejected."""
+    table = [(-1, -1, -1), (1, 1, -1), (1, 1, 1), (-1, -1, 1),
+             (-1, 1, -1), (1, -1, -1), (1, -1, 1), (-1, 1, 1)]
+    L = lib()
+    f = [ctypes.c_int() for _ in range(3)]
+    for it in range(17):
+        for irk in range(4):
+            assert L.cgfdm_stage_flags(it, irk, *[ctypes.byref(x) for x in f]) == 0
+            sign = -1 if irk % 2 else 1
+            assert tuple(x.value for x in f) == tuple(sign * v for v in table[it % 8])
+    assert L.cgfdm_stage_flags(0, 4, *[ctypes.byref(x) for x in f]) == dyn.EPARAM
+    assert L.cgfdm_stage_flags(-1, 0, *[ctypes.byref(x) for x in f]) == dyn.EPARAM

@@ -1,0 +1,115 @@
+"""A stage driven one phase at a time through the C ABI, as the reference's
+main loop drives its launchers (main.cu:444-522): cgfdm_rup_sensor, then per
+stage cgfdm_fault_trial, cgfdm_wave_stage, cgfdm_fault_deriv and
+cgfdm_fault_update, then cgfdm_fault_smooth.
+
+* composed in that order the phases give what cgfdm_step gives, bit for bit
+  (slip weakening with smoothing, and rate-and-state);
+* at the phase boundaries the fields agree with the FP64 oracle run launcher
+  by launcher: the trial tractions and the fault-plane T1 after the trial,
+  the split-node rates after the derivatives (FP32 against FP64 after 8
+  steps: 1e-4 relative, norm-wise);
+* launches can go on a caller's stream (cgfdm_set_stream) with the same
+  result."""
+import numpy as np
+import pytest
+
+import cases
+from test_gpu_parity import make_pair, relerr
+
+pytestmark = pytest.mark.gpu
+
+dyn = pytest.importorskip("dynfd3dcuda_amd")
+
+FIELDS = ("W", "FW", "Vs1", "Vs2", "Ts1", "Ts2", "Tn", "slip", "State", "init_t0", "rup_sensor")
+
+
+def step_by_phases(s, it, smooth):
+    s.rup_sensor()
+    for irk in range(4):
+        s.stage_by_phases(it, irk)
+    if smooth:
+        s.fault_smooth()
+
+
+@pytest.mark.parametrize("kind", ["sw_smooth", "rs"])
+def test_phases_compose_to_step(kind):
+    case = cases.sw_case(curved=True, FaultSmooth=1) if kind == "sw_smooth" else cases.rs_case()
+    a = dyn.Solver(case["params"])
+    b = dyn.Solver(case["params"])
+    for s in (a, b):
+        cases.apply_to_solver(case, s)
+        s.setup()
+    smooth = kind == "sw_smooth"
+    for it in range(12):
+        a.step(it)
+        step_by_phases(b, it, smooth)
+    a.synchronize()
+    b.synchronize()
+    assert np.max(np.abs(a.get("Vs1"))) > 0
+    for name in FIELDS:
+        np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
+
+
+def test_phase_boundaries_match_oracle():
+    case = cases.sw_case(curved=True)
+    s, o = make_pair(case)
+    nt = 8
+    for it in range(nt):
+        s.step(it)
+        o.step(it)
+    fx, fy, fz = s.stage_flags_lib(nt, 0)
+    assert (fx, fy, fz) == dyn.stage_flags(nt, 0)
+    s.fault_trial(nt, 0)
+    o.fault_trial(nt, 0, fx, fy, fz)
+    errs = {}
+    for name in ("tTn", "tTs1", "tTs2", "T11", "T12", "T13", "FW"):
+        errs[name] = relerr(s.get(name), o.get(name))
+    s.wave_stage(0, fx, fy, fz)
+    s.fault_deriv(nt, 0)
+    o.fault_deriv(fx, fy, fz)
+    errs["FhW"] = relerr(s.get("FhW"), o.get("FhW"))
+    print("phase boundaries, rel L2 vs oracle:", errs)
+    assert np.linalg.norm(o.get("tTs1")) > 0 and np.linalg.norm(o.get("FhW")) > 0
+    for name, e in errs.items():
+        assert e < 1e-4, (name, e)
+
+
+def hip_runtime():
+    """The HIP runtime the solver library runs on (the one already mapped
+    into this process), for creating a caller's stream."""
+    import ctypes
+    for line in open("/proc/self/maps"):
+        if "libamdhip64" in line:
+            return ctypes.CDLL(line.split()[-1])
+    pytest.fail("libamdhip64 not mapped")
+
+
+def test_caller_stream():
+    import ctypes
+    case = cases.sw_case(curved=True)
+    a = dyn.Solver(case["params"])
+    b = dyn.Solver(case["params"])
+    for s in (a, b):
+        cases.apply_to_solver(case, s)
+        s.setup()
+    hip = hip_runtime()
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    try:
+        b.set_stream(st.value)
+        for it in range(6):
+            a.step(it)
+            b.step(it)
+        assert hip.hipStreamSynchronize(st) == 0
+        b.set_stream(None)
+        for it in range(6, 8):
+            a.step(it)
+            b.step(it)
+        a.synchronize()
+        b.synchronize()
+    finally:
+        b.set_stream(None)
+        hip.hipStreamDestroy(st)
+    for name in ("W", "Vs1", "Ts1", "slip"):
+        np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
