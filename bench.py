@@ -80,6 +80,58 @@ def read_traffic(workload, kernel):
         return None
 
 
+def hip_runtime():
+    """The HIP runtime the solver library runs on (already mapped into this
+    process; a second runtime in the same process would find no device)."""
+    import ctypes
+    for line in open("/proc/self/maps"):
+        if "libamdhip64" in line:
+            return ctypes.CDLL(line.split()[-1])
+    return None
+
+
+def copy_ceiling(nbytes=1 << 30, reps=10):
+    """Device-to-device copy rate (GB/s of bytes read + written) of one
+    buffer of `nbytes`, timed with HIP events: the practical ceiling that a
+    streaming kernel on this GPU can reach."""
+    import ctypes
+    hip = hip_runtime()
+    if hip is None:
+        return None
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        return None
+    try:
+        hip.hipMemset(a, 1, nbytes)
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        for _ in range(2):
+            hip.hipMemcpyAsync(b, a, nbytes, 3, None)  # 3 = hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, nbytes, 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9 if ms.value > 0 else None
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -119,6 +171,7 @@ def cpu_baseline(case, budget_s):
     npts = p["NX"] * p["NY"] * p["NZ"]
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": npts / dt, "unit": "grid-pt updates/s", "cores": cores, "kind": "port",
+            "effective_GBps": npts / dt * b_alg(p) / 1e9,
             "sample": f"FP64 oracle, full {p['NX']}x{p['NY']}x{p['NZ']} grid of the same workload, {n} steps after 1 "
                       f"warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), {cpu_model()}"}
 
@@ -213,6 +266,8 @@ def main():
     ap.add_argument("--workload", default="curved_sw_400x300x200",
                     help="synthetic workload, one of dynfd3dcuda_amd.configs.WORKLOADS")
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="override a params.json key of the workload (repeatable)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline timing budget")
     ap.add_argument("--no-rs", action="store_true", help="skip the rate-state plain-vs-learned leg")
@@ -253,6 +308,9 @@ def main():
         kw["seed"] = args.seed
     case = gen(**kw)
     p = case["params"]
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        p[k] = float(v) if any(ch in v for ch in ".eE") else int(v)
     note = None
     if p.get("NewtonMode") == 2 and not p.get("NewtonMLP"):
         # no network given: the step runs with plain Newton (same residual rule)
@@ -305,6 +363,9 @@ def main():
         "rate_state": None,
     }
     solver.close()
+    ceil = copy_ceiling()
+    out["roofline"]["copy_ceiling_GBps"] = ceil
+    out["roofline"]["frac_of_copy_ceiling"] = achieved / ceil if ceil else None
     if not args.no_mixed:
         out["mixed"] = mixed_leg(case, args.steps, args.warmup)
     if not args.no_rs:

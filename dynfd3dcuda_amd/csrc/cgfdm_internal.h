@@ -41,6 +41,7 @@ struct Params {
   int NewtonFallbackIter = 20;
   std::string NewtonMLP; // path of trained MLP weights for the Newton guess
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
+  int WaveBandRound = 0;  // 1: low j band rounded up to whole 64-wide tiles (0: band width exactly)
 
   // derived (set_params.cu:286-295)
   int ni = 0, nj = 0, nk = 0, nx = 0, ny = 0, nz = 0, NT = 0;

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
             const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
             const size_t ps = (size_t)pni * pnj * pnk;
             const uint32_t pa4 = (pj + pk * pnj + pi * pnj * pnk) << 2;
-            const float rbb = 1.0f / cold->pB[ax][li];
+            const float rbb = ax == 0 ? rb1 : ax == 1 ? rb2 : rb3;  // 1 / beta at li
             const float* Y = cold->auxY[f];
 #pragma unroll
             for (int c = 0; c < 9; c++) h[c] += -ldo(Y + c * ps, pa4) * rbb;
@@ -665,8 +665,17 @@ std::vector<Seg> axis_segments(int n, int lo, int hi, int t) {
 const char* const kProfNames[PK_COUNT] = {"stage_interior", "stage_band", "stage_surface", "pml_aux",
                                           "fault_pre", "fault_deriv", "fault_post", "rup_sensor", "fault_rs"};
 
+// First interior column along j: the width of the low absorbing band, or
+// with WaveBandRound that width rounded up to whole TJ-wide tiles, so that
+// the band's tiles are full width (a 32-wide tile holding 20 band columns
+// pays the j halo in whole cache lines for few points, in a launch of its
+// own that cannot share those lines with the interior tile beside it).  The
+// columns between the band and this edge are then plain points, which the
+// band kernel treats like the interior one.
 int interior_lo_j(const Params& p) {
-  return std::max(p.usePML ? p.PML_N : 0, p.useCerjan ? p.DAMP_N : 0);
+  const int band = std::max(p.usePML ? p.PML_N : 0, p.useCerjan ? p.DAMP_N : 0);
+  if (p.WaveBandRound == 0) return band;
+  return (band + TJ - 1) / TJ * TJ;
 }
 
 // Uploads the off-hot-path pointer table and builds the stage-kernel work
