@@ -201,8 +201,11 @@ def mixed_leg(case, steps, warmup):
 
 
 def rate_state_leg(args):
-    """Plain Newton (NewtonMode 1) against the learned guess (NewtonMode 2,
-    same residual rule and fallback) on the rate-state configuration."""
+    """The rate-state solve three ways on the rate-state configuration: the
+    reference's stopping rule (NewtonMode 0: |dx|/|x| < 1e-12, at most 100
+    iterations, trial_ratestate.cu:22-57), plain Newton on the residual rule
+    (NewtonMode 1) and the learned guess (NewtonMode 2, same residual rule
+    and fallback)."""
     import numpy as np
     import dynfd3dcuda_amd as dyn
     from dynfd3dcuda_amd import configs, mlp
@@ -210,7 +213,7 @@ def rate_state_leg(args):
     fault = ("fault_pre", "fault_rs", "fault_deriv", "fault_post")
     res = {}
     params = None
-    for mode in (1, 2):
+    for mode in (0, 1, 2):
         case = configs.curved_rs_400x300x200(nt=lead + window + 2)
         case["params"].update({"NewtonMode": mode, "NewtonTol": 1e-10, "NewtonFallbackIter": 20})
         s = dyn.Solver(case["params"])
@@ -253,7 +256,8 @@ def rate_state_leg(args):
         s.close()
     pl, ml = res[1], res[2]
     return {"workload": f"curved_rs_400x300x200: the curved 400x300x200 grid with the aging law, {fault_nodes} fault nodes; steps {lead}..{lead + window - 1}",
-            "plain": pl, "learned": ml, "train": res["train"],
+            "reference_rule": res[0], "plain": pl, "learned": ml, "train": res["train"],
+            "fault_step_speedup_vs_reference_rule": res[0]["fault_ms_per_stage"] / ml["fault_ms_per_stage"],
             "newton_kernel_speedup": pl["newton_kernel_ms_per_stage"] / ml["newton_kernel_ms_per_stage"],
             "fault_step_speedup": pl["fault_ms_per_stage"] / ml["fault_ms_per_stage"]}
 
