@@ -228,6 +228,10 @@ int cgfdm_destroy(cgfdm_ctx* h) {
   if (c.setup_done) hipSetDevice(c.device);
   if (c.stream) hipStreamSynchronize(c.stream);
   if (c.own_stream) { hipStreamSynchronize(c.own_stream); hipStreamDestroy(c.own_stream); }
+  for (hipStream_t st : c.side)
+    if (st) { hipStreamSynchronize(st); hipStreamDestroy(st); }
+  for (hipEvent_t e : c.fence)
+    if (e) hipEventDestroy(e);
   for (void* p : c.allocations) hipFree(p);
   if (c.hW) hipFree(c.hW);
   if (c.wave_work) hipFree(c.wave_work);
@@ -299,6 +303,12 @@ int cgfdm_setup(cgfdm_ctx* h) {
   if (hipStreamCreateWithFlags(&c.own_stream, hipStreamNonBlocking) != hipSuccess)
     return fail(CGFDM_EHIP, "stream creation failed");
   c.stream = c.own_stream;
+  if (c.p.StageStreams) {  // default priority, like every stream here
+    for (hipStream_t& st : c.side)
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail(CGFDM_EHIP, "stream creation failed");
+    for (hipEvent_t& e : c.fence)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(CGFDM_EHIP, "event creation failed");
+  }
   c.setup_done = true;
   return 0;
 }

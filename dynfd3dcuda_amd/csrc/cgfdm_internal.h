@@ -41,6 +41,8 @@ struct Params {
   int NewtonFallbackIter = 20;
   std::string NewtonMLP; // path of trained MLP weights for the Newton guess
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
+  int StageStreams = 0;  // 1: interior and free-surface launches of a stage on side streams
+  int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
   int WaveBandRound = 0;  // 1: low j band rounded up to whole 64-wide tiles (0: band width exactly)
 
   // derived (set_params.cu:286-295)
@@ -145,6 +147,8 @@ struct Ctx {
   bool setup_done = false;
   hipStream_t stream = nullptr;      // where every launch goes
   hipStream_t own_stream = nullptr;  // created at setup, destroyed with the context
+  hipStream_t side[2] = {nullptr, nullptr};  // StageStreams: interior, free surface
+  hipEvent_t fence[3] = {nullptr, nullptr, nullptr};  // stage input ready, interior done, surface done
   int device = 0;
   // host inputs
   std::vector<float> gx, gy, gz;
@@ -166,6 +170,7 @@ struct Ctx {
   float* trac_top = nullptr;    // free surface: traction vectors of the six top rows [nx][6][9][ny]
   float* img_top = nullptr;     // free surface: image velocity RHS of the three top rows [3][3][ni][nj]
   int wave_nwork = 0;                  // all work items
+  bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
   void* mlp_dev = nullptr;          // MlpDev of the learned Newton guess (NewtonMode 2)

@@ -549,11 +549,41 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   const int o1 = n[0], o2 = o1 + n[1], o3 = o2 + n[2];
   const int4* w = (const int4*)c.wave_work;
   Profiler& pr = const_cast<Ctx&>(c).prof;
+  // StageStreams: the interior tiles and the free-surface passes share no
+  // words with the bands or with each other (disjoint points, read-only stage
+  // input), so they run on side streams, fenced after the work already queued
+  // on the solver's stream; the auxiliary update waits for the surface tiles
+  // (which read the aux fields) and the caller's next launch for the interior.
+  // While kernel classes are timed the launches stay serial, so that each
+  // class's events bracket its own kernels only.
+  const bool par = c.side[0] != nullptr && !c.prof.on;
+  hipStream_t si = s, ss = s;
+  if (par) {
+    hipEventRecord(c.fence[0], s);
+    si = c.side[0];
+    ss = c.side[1];
+    hipStreamWaitEvent(si, c.fence[0], 0);
+    hipStreamWaitEvent(ss, c.fence[0], 0);
+  }
   if (n[0] > 0) {
     a.work = w; a.nwork = n[0];
-    pr.begin(s, PK_STAGE_INTERIOR);
-    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
-    pr.end(s);
+    pr.begin(si, PK_STAGE_INTERIOR);
+    if (c.wave_tall)
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0, TJ, 2 * NTHR>), dim3(a.nwork), dim3(TJ, 2 * TK, 1), 0, si, a);
+    else
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, si, a);
+    pr.end(si);
+  }
+  if (n[3] > 0) {
+    a.work = w + 2 * o3; a.nwork = n[3];
+    pr.begin(ss, PK_STAGE_SURFACE);
+    if (a.freenode) {
+      const long n1 = (long)c.d.nx * 6 * c.d.ny, n2 = (long)c.d.ni * 3 * c.d.nj;
+      hipLaunchKernelGGL(traction_top_kernel<MT>, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, ss, a, c.d.nx);
+      hipLaunchKernelGGL((image_top_kernel<FX, FY, FZ, MT>), dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, ss, a);
+    }
+    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, ss, a);
+    pr.end(ss);
   }
   if (n[1] + n[2] > 0) {
     pr.begin(s, PK_STAGE_BAND);
@@ -567,16 +597,9 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     }
     pr.end(s);
   }
-  if (n[3] > 0) {
-    a.work = w + 2 * o3; a.nwork = n[3];
-    pr.begin(s, PK_STAGE_SURFACE);
-    if (a.freenode) {
-      const long n1 = (long)c.d.nx * 6 * c.d.ny, n2 = (long)c.d.ni * 3 * c.d.nj;
-      hipLaunchKernelGGL(traction_top_kernel<MT>, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, s, a, c.d.nx);
-      hipLaunchKernelGGL((image_top_kernel<FX, FY, FZ, MT>), dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, a);
-    }
-    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
-    pr.end(s);
+  if (par) {
+    hipEventRecord(c.fence[2], ss);
+    hipStreamWaitEvent(s, c.fence[2], 0);
   }
   if (a.pml && !a.rhs_only) {
     PmlAuxArgs<MT> p{};
@@ -593,6 +616,10 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
       hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
       pr.end(s);
     }
+  }
+  if (par) {
+    hipEventRecord(c.fence[1], si);
+    hipStreamWaitEvent(s, c.fence[1], 0);
   }
 }
 
@@ -714,17 +741,25 @@ int upload_wave_cold(Ctx& c) {
   if (freenode) hi[2] = std::max(hi[2], 3);
   // the interior box holds whole tiles along j and k: a remainder joins the
   // high-side band (or the free-surface tiles), whose kernels treat any point
+  // (WaveTallInterior: interior tiles of 2 TK rows in blocks of 2 NTHR
+  // threads, which re-read the k halo rows for twice as many points)
+  const bool tall = p.WaveTallInterior != 0;
+  c.wave_tall = tall;
   for (int ax = 1; ax < 3; ax++) {
-    const int t = ax == 1 ? TJ : TK, e = n3[ax] - lo[ax] - hi[ax];
+    const int t = ax == 1 ? TJ : (tall ? 2 * TK : TK), e = n3[ax] - lo[ax] - hi[ax];
     if (e > t) hi[ax] += e % t;
   }
   const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
   const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
+  const std::vector<Seg> ski = tall ? axis_segments(n3[2], lo[2], hi[2], 2 * TK) : sk;  // interior tiles' rows
   const int nk = n3[2];
   auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
   int tiles[3] = {0, 0, 0};
   for (const Seg& b : sk)
     for (const Seg& a : sj) tiles[is_top(b) ? 2 : (a.in && b.in ? 0 : 1)]++;
+  int tiles_in = 0;  // interior blocks per chunk
+  for (const Seg& b : ski)
+    for (const Seg& a : sj) tiles_in += a.in && b.in;
   const int ni = n3[0], ia = std::min(lo[0], ni), ib = std::max(ia, ni - hi[0]);
   // planes per block of each kind: short chunks (several thousand blocks per
   // launch) balance the CUs better than the longer chunks would save on the
@@ -736,7 +771,7 @@ int upload_wave_cold(Ctx& c) {
     const int chunks = std::max(1, std::min(span, (target + std::max(ntiles, 1) - 1) / std::max(ntiles, 1)));
     return std::max((span + chunks - 1) / chunks, std::min(minlen, span));
   };
-  const int ich_in = chunk(ib - ia, tiles[0], 4096, 4);
+  const int ich_in = chunk(ib - ia, tiles_in, tall ? 2048 : 4096, 4);
   const int ich_band = chunk(ni, tiles[1] + tiles[0] * (ni - (ib - ia)) / std::max(ni, 1), 4096, 4);
   const int ich_top = chunk(ni, tiles[2], 2048, 1);
   // lists: 0 interior, 1 absorbing bands (64-wide tiles), 2 absorbing bands
@@ -760,7 +795,7 @@ int upload_wave_cold(Ctx& c) {
   for (const Seg& b : sk)
     if (is_top(b)) ktop = std::min(ktop, b.s);
   for (int i0 = ia; i0 < ib; i0 += ich_in)
-    for (const Seg& b : sk) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
+    for (const Seg& b : ski) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
   for (int i0 = 0; i0 < ni; i0 += ich_band)
     for (const Seg& b : sk)
       for (const Seg& a : sj) {

@@ -68,15 +68,16 @@ def test_wave_rhs_curved_free_surface(flags):
         assert e < 2e-6, (flags, c, e)
 
 
-@pytest.mark.parametrize("band_round", [0, 1])
+@pytest.mark.parametrize("layout", [{}, {"WaveBandRound": 1}, {"WaveTallInterior": 1}], ids=["default", "band_round", "tall"])
 @pytest.mark.parametrize("pml,cerjan", [(1, 0), (0, 1), (1, 1)])
-def test_wave_steps_absorbing(pml, cerjan, band_round):
+def test_wave_steps_absorbing(pml, cerjan, layout):
     """Three full RK4 steps without the fault: stencil + ADE-PML + RK +
-    Cerjan against the oracle; tolerance 1e-5 per component.  Both tilings
-    of the low j band: exactly its width (32-wide tiles) and rounded up to
-    whole 64-wide tiles (WaveBandRound, the default), on a grid wide enough
-    (NY 150) for interior tiles between the bands."""
-    case = cases.volume_case(curved=True, usePML=pml, useCerjan=cerjan, NY=150, WaveBandRound=band_round)
+    Cerjan against the oracle; tolerance 1e-5 per component.  Three tilings:
+    the low j band exactly its width (32-wide tiles), rounded up to whole
+    64-wide tiles (WaveBandRound), and 64 x 16 interior tiles in 1024-thread
+    blocks (WaveTallInterior), on a grid wide enough (NY 150) for interior
+    tiles between the bands."""
+    case = cases.volume_case(curved=True, usePML=pml, useCerjan=cerjan, NY=150, **layout)
     s, o = make_pair(case)
     d = o.dims()
     W = cases.random_wavefield(d["nx"], d["ny"], d["nz"], seed=5)
