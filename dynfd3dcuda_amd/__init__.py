@@ -32,7 +32,7 @@ EXPORTED = [
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
     "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
     "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
-    "cgfdm_rup_sensor", "cgfdm_fault_smooth",
+    "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
@@ -108,6 +108,7 @@ def load_library(path=None):
         getattr(lib, name).argtypes = [c_p, ctypes.c_int, ctypes.c_int]
     lib.cgfdm_rup_sensor.argtypes = [c_p]
     lib.cgfdm_fault_smooth.argtypes = [c_p]
+    lib.cgfdm_thermpress.argtypes = [c_p]
     for name in EXPORTED[1:]:
         getattr(lib, name).restype = ctypes.c_int
     if path is None:
@@ -237,6 +238,9 @@ class Solver:
 
     def fault_smooth(self):
         self._check(self.lib.cgfdm_fault_smooth(self.h))
+
+    def thermpress(self):
+        self._check(self.lib.cgfdm_thermpress(self.h))
 
     def stage_by_phases(self, it, irk):
         """The phases of one stage in cgfdm_stage's order."""

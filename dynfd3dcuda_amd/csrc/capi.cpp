@@ -123,6 +123,9 @@ FieldRef field(Ctx& c, const std::string& n) {
     else if (n == "hslip") f32(F.hslip, n2);
     else if (n == "slip") f64(F.slip, n2);
     else if (n == "State") f64(F.State, n2);
+    else if (n == "TP_T") f64(F.TP_T, F.TP_T ? (size_t)F.TP_n * n2 : 0);
+    else if (n == "TP_P") f64(F.TP_P, F.TP_P ? (size_t)F.TP_n * n2 : 0);
+    else if (n == "TP_hy") f64(F.TP_hy, F.TP_hy ? n2 : 0);
     else if (n == "init_t0") f32(F.init_t0, n2);
     else if (n == "friction") f32(F.friction, n2);
     else if (n == "rup_sensor") f32(F.rup_sensor, n2);
@@ -277,7 +280,8 @@ int cgfdm_set_media3d(cgfdm_ctx* h, const float* vp, const float* vs, const floa
 
 int cgfdm_set_init_stress(cgfdm_ctx* h, const char* name, const float* data, size_t n) {
   if (!h || !name || !data) return fail(CGFDM_EPARAM, "null argument");
-  static const char* ok[] = {"Tx", "Ty", "Tz", "dTx", "dTy", "dTz", "mu_s", "mu_d", "Dc", "C0", "a", "b", "L", "Vw", "State"};
+  static const char* ok[] = {"Tx", "Ty", "Tz", "dTx", "dTy", "dTz", "mu_s", "mu_d", "Dc", "C0", "a", "b", "L", "Vw", "State",
+                             "TP_hy"};
   for (const char* s : ok)
     if (std::strcmp(s, name) == 0) {
       h->c.stress_in[name] = std::vector<float>(data, data + n);
@@ -339,6 +343,7 @@ int cgfdm_step(cgfdm_ctx* h, int it) {
   Ctx& c = h->c;
   if (!c.setup_done) return fail(CGFDM_ESTATE, "step before setup");
   if (c.p.Rupture) launch_rup_sensor(c);
+  if (c.p.Rupture && c.p.Friction_type == 3) launch_thermpress(c);  // main.cu:445-447
   for (int irk = 0; irk < 4; irk++) {
     int rc = cgfdm_stage(h, it, irk);
     if (rc) return rc;
@@ -463,6 +468,14 @@ int cgfdm_rup_sensor(cgfdm_ctx* h) {
   if (int rc = fault_phase_ready(h, 0, 0, fx, fy, fz)) return rc;
   launch_rup_sensor(h->c);
   return check_hip("rupture sensor launch");
+}
+
+int cgfdm_thermpress(cgfdm_ctx* h) {
+  int fx, fy, fz;
+  if (int rc = fault_phase_ready(h, 0, 0, fx, fy, fz)) return rc;
+  if (h->c.p.Friction_type != 3) return fail(CGFDM_ESTATE, "thermal pressurization needs Friction_type 3");
+  launch_thermpress(h->c);
+  return check_hip("thermal pressurization launch");
 }
 
 int cgfdm_fault_smooth(cgfdm_ctx* h) {
@@ -757,9 +770,18 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
     o->fault.put_record(o->fv[8], rec, ib.data());
     hipMemcpy(buf.data(), c.f.rup_sensor, n2 * 4, hipMemcpyDeviceToHost);
     o->fault.put_record(o->fv[9], rec, buf.data());
-    std::fill(buf.begin(), buf.end(), 0.0f);  // thermal pressurization is not modelled
-    o->fault.put_record(o->fv[10], rec, buf.data());
-    o->fault.put_record(o->fv[11], rec, buf.data());
+    // thermal pressurization: the rises on the fault (level 0) as the
+    // reference writes them (io_fault.cu:262-266); zeros without it
+    for (int v = 0; v < 2; v++) {
+      const double* src = v ? c.f.TP_P : c.f.TP_T;
+      if (src) {
+        hipMemcpy(db.data(), src, n2 * 8, hipMemcpyDeviceToHost);
+        for (long q = 0; q < n2; q++) buf[q] = (float)db[q];
+      } else {
+        std::fill(buf.begin(), buf.end(), 0.0f);
+      }
+      o->fault.put_record(o->fv[10 + v], rec, buf.data());
+    }
     o->fault.sync();
   }
   // slices from the current wavefield

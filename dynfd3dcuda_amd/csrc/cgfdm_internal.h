@@ -95,6 +95,9 @@ struct FaultDev {
   int *init_t0_flag, *rup_index_y, *rup_index_z, *united, *faultgrid, *flag_rup;
   unsigned int* newton_stats;        // [3][nk][nj] per-node counts: solves, iterations, fallbacks
   float *Ws;                         // [9][2][nz][ny] smoothed T2*/T3* (FaultSmooth)
+  double *TP_T = nullptr, *TP_P = nullptr;  // Friction_type 3: [TP_n][nk][nj] temperature and pore-pressure rise
+  double *TP_hy = nullptr;                  // [nk][nj] hydraulic diffusivity
+  int TP_n = 1;
   double *rs_carry;                  // [10][nk][nj] rate-state Newton inputs handed from fault_pre to fault_rs
   float *rs_carryf;                  // [9][nk][nj] normal (3), jacvec, guess features (4), solved log10 V
   float *smooth_w;                   // [2][4][7][7] half-volume Gaussian weights (FaultSmooth)
@@ -198,6 +201,7 @@ void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy,
 void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz);
 void launch_fault_post(Ctx& c, const StageBufs& b, int irk);
 void launch_rup_sensor(Ctx& c);
+void launch_thermpress(Ctx& c);
 void launch_fault_smooth(Ctx& c);
 void launch_fault_rs(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz);
 // learned Newton guess (mlp_train.hip)

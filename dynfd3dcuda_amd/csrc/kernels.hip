@@ -363,6 +363,9 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     double Vt1 = cc * Tt[1], Vt2 = cc * Tt[2];
     double st = F.State[q], RS_a = F.a[q];
     double Tau_nd = Tt[0] >= -1e-1 ? -1e-1 : Tt[0];
+    // thermal pressurization: the fault's pore pressure makes the normal
+    // stress effective for the solve and the strength (trial_ratestate.cu:636-644)
+    if (F.TP_P) Tau_nd += F.TP_P[q];
     double coef = cc * RS_a * fabs(Tau_nd);
     double Vtn = sqrt(Vt1 * Vt1 + Vt2 * Vt2);
     double Ttn = sqrt(Tt[1] * Tt[1] + Tt[2] * Tt[2]);
@@ -429,7 +432,10 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
     }
   }
   if (!valid) return;
-  const double Tau_nd = C[5 * n2 + q], Tt1 = C[6 * n2 + q], Tt2 = C[7 * n2 + q];
+  // C[5] holds the effective normal stress; the traction vector takes the
+  // total one (trial_ratestate.cu:785-788)
+  const double Tau_eff = C[5 * n2 + q], Tt1 = C[6 * n2 + q], Tt2 = C[7 * n2 + q];
+  const double Tau_nd = F.TP_P ? Tau_eff - F.TP_P[q] : Tau_eff;
   double Ttn = C[8 * n2 + q];
   const float n[3] = {Cf[q], Cf[n2 + q], Cf[2 * n2 + q]};
   const float jacvec = Cf[3 * n2 + q];
@@ -445,7 +451,7 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
   F.newton_stats[2 * n2 + q] += fb ? 1u : 0u;
   const double RS_V0 = 1e-6;
   float V = (float)(2.0 * RS_V0 * exp(-st / RS_a) * sinh(xnew));
-  const double Tm = RS_a * xnew * fabs(Tau_nd);
+  const double Tm = RS_a * xnew * fabs(Tau_eff);
   F.friction[q] = (float)(RS_a * xnew);
   Ttn = fmax(Ttn, 1e-8);
   const double Ts1 = Tt1 / Ttn * Tm, Ts2 = Tt2 / Ttn * Tm;
@@ -791,6 +797,56 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
 }
 
 // cal_rup_sensor_D_cu / cal_rup_sensor_magn_cu (rup_sensor.cu:7-86).
+// Thermal pressurization (thermpress_cu, thermpress.cu:22-233), once per
+// step before the stages: per fault node, the temperature and pore-pressure
+// rises on TP_n points x = i dh normal to the fault (mirror symmetric at
+// both ends), driven by the frictional heating tau V spread as a Gaussian of
+// half-width h, in explicit Euler substeps over DT; tau takes the last
+// step's friction coefficient and total normal stress plus the fault's pore
+// pressure.  FP64, updated in place level by level: the old values of the
+// levels either side ride in registers.  With TP_n = 1 dh is infinite and
+// the diffusion terms vanish.
+__global__ void __launch_bounds__(256) thermpress_kernel(FaultDev F, int nj, int nk, double DT) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y * blockDim.y + threadIdx.y;
+  if (j >= nj || k >= nk) return;
+  const long q = j + (long)k * nj, n2 = (long)nj * nk;
+  if (F.united[q]) return;
+  const double Lamb = 0.1e6, rho_c = 2.7e6, h = 20e-3, alpha_th = 1.0e-6, SQRT2PI = 2.506628274631000;
+  const double h2 = h * h;
+  const int n = F.TP_n;
+  const double v1 = F.Vs1[q], v2 = F.Vs2[q];
+  const double rate = sqrt(v1 * v1 + v2 * v2);
+  const double alpha_hy = F.TP_hy[q];
+  const double alpha_max = fmax(alpha_th, alpha_hy);
+  const double w = sqrt(4.0 * alpha_max * 25.0);
+  const double dh = n > 1 ? w / (double)(n - 1) : (double)INFINITY;
+  const double dh2 = dh * dh;
+  const int nsub = (int)(DT / (0.1 * dh2 / alpha_max)) + 1;
+  const double dt = DT / (double)nsub;
+  const double tn = F.Tn[q], fr = F.friction[q];
+  double* T = F.TP_T;
+  double* P = F.TP_P;
+  for (int s = 0; s < nsub; s++) {
+    const double tauV = -fr * (tn + P[q]) * rate;
+    const double cw = tauV / (rho_c * h * SQRT2PI);
+    double To = T[q], Po = P[q];                                  // level i
+    double Tl = 0.0, Pl = 0.0;                                    // level i-1 (old)
+    double Tr = n > 1 ? T[n2 + q] : 0.0, Pr = n > 1 ? P[n2 + q] : 0.0;  // level i+1 (old)
+    for (int i = 0; i < n; i++) {
+      const double Tm = i == 0 ? Tr : Tl, Pm = i == 0 ? Pr : Pl;
+      const double Tp = i == n - 1 ? Tl : Tr, Pp = i == n - 1 ? Pl : Pr;
+      const double x = n > 1 ? i * dh : 0.0;
+      const double om = exp(-0.5 * x * x / h2) * cw;
+      const double dTv = alpha_th * ((Tp - 2.0 * To + Tm) / dh2) + om;
+      const double dPv = alpha_hy * ((Pp - 2.0 * Po + Pm) / dh2) + Lamb * dTv;
+      T[i * n2 + q] = To + dt * dTv;
+      P[i * n2 + q] = Po + dt * dPv;
+      Tl = To; Pl = Po; To = Tr; Po = Pr;
+      if (i + 2 < n) { Tr = T[(i + 2) * n2 + q]; Pr = P[(i + 2) * n2 + q]; }
+    }
+  }
+}
+
 __global__ void rup_sensor_d_kernel(FaultDev F, int nj, int nk, int g0, int g1, int g2, int g3) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -991,6 +1047,11 @@ void launch_fault_post(Ctx& c, const StageBufs& b, int irk) {
   g.z = 7;  // six slab planes and the split-node part
   hipLaunchKernelGGL(fault_post_kernel, g, dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
+}
+
+void launch_thermpress(Ctx& c) {
+  dim3 g((c.d.nj + 63) / 64, (c.d.nk + 3) / 4, 1);
+  hipLaunchKernelGGL(thermpress_kernel, g, dim3(64, 4, 1), 0, c.stream, c.f, c.d.nj, c.d.nk, c.p.DT);
 }
 
 void launch_rup_sensor(Ctx& c) {

@@ -144,6 +144,7 @@ int host_setup(Ctx& c, std::string& err) {
     std::vector<std::string> names = {"Tx", "Ty", "Tz", "dTx", "dTy", "dTz"};
     if (p.Friction_type == 0) for (auto s : {"mu_s", "mu_d", "Dc", "C0"}) names.push_back(s);
     else for (auto s : {"a", "b", "L", "Vw", "State"}) names.push_back(s);
+    if (p.Friction_type == 3) names.push_back("TP_hy");  // init.cu:999-1003
     for (auto& s : names) {
       std::vector<float> v;
       if (!nf.read_float(s, v, err)) return 4;
@@ -610,6 +611,16 @@ int host_setup(Ctx& c, std::string& err) {
       F.rs_carry = dmalloc<double>(c, 10 * n2);
       F.rs_carryf = dmalloc<float>(c, 9 * n2);
       if (!F.rs_carry || !F.rs_carryf) { err = "hipMalloc failed for the rate-state hand-over"; return 6; }
+    }
+    if (p.Friction_type == 3) {  // thermal pressurization, zero initial rises (alloc.cu:111-119)
+      F.TP_n = std::max(p.TP_n, 1);
+      F.TP_T = dmalloc<double>(c, (size_t)F.TP_n * n2);
+      F.TP_P = dmalloc<double>(c, (size_t)F.TP_n * n2);
+      std::vector<float> hy = get_in("TP_hy");
+      std::vector<double> hyd(hy.begin(), hy.end());
+      F.TP_hy = dmalloc<double>(c, n2);
+      if (!F.TP_T || !F.TP_P || !F.TP_hy) { err = "hipMalloc failed for thermal pressurization"; return 6; }
+      upload(F.TP_hy, hyd);
     }
     if (p.NewtonMode == 2) {
       if (p.Friction_type < 1) { err = "NewtonMode 2 needs rate-and-state friction"; return 1; }

@@ -70,7 +70,8 @@ int cgfdm_set_media1d(cgfdm_ctx* ctx, int nlayer, const float* depth, const floa
  * bimaterial and layered media. */
 int cgfdm_set_media3d(cgfdm_ctx* ctx, const float* vp, const float* vs, const float* rho, size_t n);
 /* Fault_init_stress variable by its netCDF name: Tx Ty Tz dTx dTy dTz mu_s
- * mu_d Dc C0 a b L Vw State, [nk][nj] (srcd/init.cu:904-1022). */
+ * mu_d Dc C0 a b L Vw State, and TP_hy for Friction_type 3, [nk][nj]
+ * (srcd/init.cu:904-1022). */
 int cgfdm_set_init_stress(cgfdm_ctx* ctx, const char* name, const float* data, size_t n);
 
 /* Build grid, metric, media, PML profiles, Cerjan profiles, fault
@@ -135,6 +136,10 @@ int cgfdm_fault_update(cgfdm_ctx* ctx, int it, int irk);
 /* cal_rup_sensor (rup_sensor.cu:7-97), run before the stages of a step
  * (main.cu:444). */
 int cgfdm_rup_sensor(cgfdm_ctx* ctx);
+/* thermpress (thermpress.cu:22-233): the temperature and pore-pressure
+ * rises normal to the fault, once per step before the stages when
+ * Friction_type is 3 (main.cu:445-447); CGFDM_ESTATE for other types. */
+int cgfdm_thermpress(cgfdm_ctx* ctx);
 /* smooth_gauss_volume (smooth.cu:13-367), after the stages of a step when
  * FaultSmooth is set (main.cu:516-522); CGFDM_ESTATE when it is not. */
 int cgfdm_fault_smooth(cgfdm_ctx* ctx);

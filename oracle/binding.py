@@ -41,6 +41,7 @@ def load():
     lib.orc_fault_trial.argtypes = [P] + [ctypes.c_int] * 5
     lib.orc_fault_deriv.argtypes = [P] + [ctypes.c_int] * 3
     lib.orc_rup_sensor.argtypes = [P]
+    lib.orc_thermpress.argtypes = [P]
     lib.orc_smooth.argtypes = [P]
     lib.orc_wave_deriv.argtypes = [P] + [ctypes.c_int] * 3
     lib.orc_pml_deriv.argtypes = [P] + [ctypes.c_int] * 3
@@ -140,6 +141,9 @@ class Oracle:
 
     def rup_sensor(self):
         self.lib.orc_rup_sensor(self.h)
+
+    def thermpress(self):
+        self.lib.orc_thermpress(self.h)
 
     def smooth(self):
         self.lib.orc_smooth(self.h)

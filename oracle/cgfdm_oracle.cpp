@@ -136,6 +136,7 @@ struct Oracle {
   int NX = 100, NY = 90, NZ = 80;
   real TMAX = 1.0, DT = 0.01, DH = 100.0;
   int INPORT_GRID_TYPE = 0, INPORT_STRESS_TYPE = 0, Friction_type = 0;
+  int TP_n = 1;  // thermal-pressurization grid points normal to the fault (set_params.cu:103, 281-282)
   int PML_N = 20;
   real PML_velocity = 3000.0, PML_bmax = 3.0, PML_fc = 1.0;
   int DAMP_N = 20;
@@ -164,7 +165,7 @@ struct Oracle {
   std::vector<real> m1d_depth, m1d_rho, m1d_vp, m1d_vs;  // Media1D layers
   std::vector<real> m3d_vp, m3d_vs, m3d_rho;            // Media3D [ni][nk][nj]
   std::vector<real> in_Tx, in_Ty, in_Tz, in_dTx, in_dTy, in_dTz;
-  std::vector<real> in_mu_s, in_mu_d, in_Dc, in_C0, in_a, in_b, in_L, in_Vw, in_State;
+  std::vector<real> in_mu_s, in_mu_d, in_Dc, in_C0, in_a, in_b, in_L, in_Vw, in_State, in_TP_hy;
 
   // ---- state
   std::vector<real> C, M;
@@ -189,6 +190,7 @@ struct Oracle {
   std::vector<real> T0x, T0y, T0z, dT0x, dT0y, dT0z, str_peak, Fmu_d, FDc, FC0;
   std::vector<real> Fa, Fb, FL, FVw, slip, hslip, mslip, tslip, Vs1, Vs2;
   std::vector<real> State, mState, hState, tState, init_t0, friction;
+  std::vector<real> TP_T, TP_P, TP_hy;  // [TP_n][nk][nj] temperature and pore pressure rise; [nk][nj] hydraulic diffusivity
   std::vector<real> rsD_y, rsD_z, rup_sensor;
   std::vector<int> init_t0_flag, rup_index_y, rup_index_z, united, faultgrid, flag_rup;
   // Newton statistics (extension: iteration counts of the rate-state solve)
@@ -205,7 +207,7 @@ struct Oracle {
 #define SETI(name) if (k == #name) { name = (int)v; return 0; }
 #define SETR(name) if (k == #name) { name = v; return 0; }
     SETI(NX) SETI(NY) SETI(NZ) SETR(TMAX) SETR(DT) SETR(DH)
-    SETI(INPORT_GRID_TYPE) SETI(INPORT_STRESS_TYPE) SETI(Friction_type)
+    SETI(INPORT_GRID_TYPE) SETI(INPORT_STRESS_TYPE) SETI(Friction_type) SETI(TP_n)
     SETI(PML_N) SETR(PML_velocity) SETR(PML_bmax) SETR(PML_fc) SETI(DAMP_N)
     SETR(mu_s) SETR(mu_d) SETR(mu_0) SETR(Dc) SETR(C0)
     SETR(smooth_load_T) SETR(bi_vp1) SETR(bi_vs1) SETR(bi_rho1) SETR(bi_vp2) SETR(bi_vs2)
@@ -799,6 +801,8 @@ struct Oracle {
     z(State); z(mState); z(hState); z(tState); z(init_t0); z(friction); z(rsD_y); z(rsD_z); z(rup_sensor);
     init_t0_flag.assign(n2, 0); rup_index_y.assign(n2, 0); rup_index_z.assign(n2, 0);
     united.assign(n2, 0); faultgrid.assign(n2, 0); flag_rup.assign(n2, 0);
+    // allocated and zeroed for every friction type (alloc.cu:111-119)
+    TP_T.assign((long)std::max(TP_n, 1) * n2, 0.0); TP_P.assign((long)std::max(TP_n, 1) * n2, 0.0); TP_hy.assign(n2, 0.0);
     FW.assign(18 * nyz, 0); FmW.assign(18 * nyz, 0); FhW.assign(18 * nyz, 0); FtW.assign(18 * nyz, 0);
     for (auto* v : {&T11, &T12, &T13, &mT11, &mT12, &mT13, &hT11, &hT12, &hT13}) v->assign(7 * nyz, 0.0);
     for (int k = 0; k < nk; k++)
@@ -831,6 +835,7 @@ struct Oracle {
       T0x = in_Tx; T0y = in_Ty; T0z = in_Tz; dT0x = in_dTx; dT0y = in_dTy; dT0z = in_dTz;
       if (Friction_type == 0) { str_peak = in_mu_s; Fmu_d = in_mu_d; FDc = in_Dc; FC0 = in_C0; }
       else { Fa = in_a; Fb = in_b; FL = in_L; FVw = in_Vw; State = in_State; }
+      if (Friction_type == 3) TP_hy = in_TP_hy;  // init.cu:999-1003
       for (int k = 0; k < nk; k++)
         for (int j = 0; j < nj; j++) {
           long q = F2(j, k), pv = FS(j + 3, k + 3) * 3;
@@ -1115,6 +1120,10 @@ struct Oracle {
         real Vt[3] = {c * Tt[0], c * Tt[1], c * Tt[2]};
         real st = State[q], RS_a = Fa[q];
         real Tau_n = Tt[0] >= -1e-1 ? -1e-1 : Tt[0];
+        // thermal pressurization: the pore pressure on the fault (level 0)
+        // makes the effective normal stress, trial_ratestate.cu:636-644
+        const real Pf = Friction_type == 3 ? TP_P[q] : 0.0;
+        Tau_n += Pf;
         real coef = c * RS_a * std::fabs(Tau_n);
         real Vtn = std::sqrt(Vt[1] * Vt[1] + Vt[2] * Vt[2]);
         real Ttn = std::sqrt(Tt[1] * Tt[1] + Tt[2] * Tt[2]);
@@ -1127,6 +1136,7 @@ struct Oracle {
         real V = 2.0 * RS_V0 * std::exp(-st / RS_a) * std::sinh(xnew);
         real T = RS_a * xnew * std::fabs(Tau_n);
         friction[q] = RS_a * xnew;
+        Tau_n -= Pf;  // back to the total normal stress, trial_ratestate.cu:785-788
         Ttn = std::max(Ttn, 1e-8);
         real Tau_s1 = Tt[1] / Ttn * T, Tau_s2 = Tt[2] / Ttn * T;
         V = std::max(V, 1e-30);
@@ -1552,10 +1562,57 @@ struct Oracle {
       }
   }
 
+  // thermpress_cu, thermpress.cu:22-233: per fault node, the temperature
+  // and pore-pressure rises on a 1-D grid of TP_n points normal to the fault
+  // (x = i dh, mirror symmetric at both ends), driven by the frictional
+  // heating tau V spread as a Gaussian of half-width h, advanced with
+  // explicit Euler substeps over DT.  tau uses the effective normal stress
+  // of the last step's output traction and friction coefficient.
+  void thermpress() {
+    const real Lamb = 0.1e6, rho_c = 2.7e6, h = 20e-3, alpha_th = 1.0e-6;
+    const real h2 = h * h, SQRT2PI = 2.506628274631000;
+    const long n2 = (long)nj * nk;
+    std::vector<real> dT(TP_n), dP(TP_n);
+    for (long q = 0; q < n2; q++) {
+      if (united[q]) continue;
+      const real rate = std::sqrt(Vs1[q] * Vs1[q] + Vs2[q] * Vs2[q]);
+      const real alpha_hy = TP_hy[q];
+      const real alpha_max = std::max(alpha_th, alpha_hy);
+      const real w = std::sqrt(4.0 * alpha_max * 25.0);
+      // TP_n = 1: dh is infinite and the diffusion terms vanish (the
+      // reference then reads one level past the array, times 1/inf)
+      const real dh = TP_n > 1 ? w / (real)(TP_n - 1) : INFINITY;
+      const real dh2 = dh * dh;
+      const int nsub = (int)(DT / (0.1 * dh2 / alpha_max)) + 1;
+      const real dt = DT / (real)nsub;
+      for (int s = 0; s < nsub; s++) {
+        const real tau = -friction[q] * (Tn[q] + TP_P[q]);
+        const real tauV = tau * rate;
+        for (int i = 0; i < TP_n; i++) {
+          const real x = TP_n > 1 ? i * dh : 0.0;
+          const real omega = std::exp(-0.5 * x * x / h2) * (tauV / (rho_c * h * SQRT2PI));
+          const int im = i == 0 ? 1 : i - 1, ip = i == TP_n - 1 ? i - 1 : i + 1;
+          real lapT = 0.0, lapP = 0.0;
+          if (TP_n > 1) {
+            lapT = (TP_T[ip * n2 + q] - 2.0 * TP_T[i * n2 + q] + TP_T[im * n2 + q]) / dh2;
+            lapP = (TP_P[ip * n2 + q] - 2.0 * TP_P[i * n2 + q] + TP_P[im * n2 + q]) / dh2;
+          }
+          dT[i] = alpha_th * lapT + omega;
+          dP[i] = alpha_hy * lapP + Lamb * dT[i];
+        }
+        for (int i = 0; i < TP_n; i++) {
+          TP_T[i * n2 + q] += dt * dT[i];
+          TP_P[i * n2 + q] += dt * dP[i];
+        }
+      }
+    }
+  }
+
   void step(int it) {
     static const int FD[8][3] = {{-1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, 1, 1}};
     int FX = FD[it % 8][0], FY = FD[it % 8][1], FZ = FD[it % 8][2];
     if (Rupture) rup_sensor_calc();
+    if (Rupture && Friction_type == 3) thermpress();  // main.cu:445-447
     for (int irk = 0; irk < 4; irk++) {
       stage(it, irk, FX, FY, FZ);
       FX = -FX; FY = -FY; FZ = -FZ;
@@ -1596,6 +1653,7 @@ struct Oracle {
     if (n == "Vs1") return &Vs1; if (n == "Vs2") return &Vs2; if (n == "slip") return &slip;
     if (n == "hslip") return &hslip; if (n == "State") return &State; if (n == "init_t0") return &init_t0;
     if (n == "friction") return &friction; if (n == "rup_sensor") return &rup_sensor;
+    if (n == "TP_T") return &TP_T; if (n == "TP_P") return &TP_P; if (n == "TP_hy") return &TP_hy;
     if (n == "str_init_x") return &str_init_x; if (n == "str_init_y") return &str_init_y; if (n == "str_init_z") return &str_init_z;
     if (n == "str_peak") return &str_peak; if (n == "vec_n") return &vec_n; if (n == "vec_s1") return &vec_s1; if (n == "vec_s2") return &vec_s2;
     if (n == "rho_f") return &rho_f;
@@ -1660,6 +1718,7 @@ int orc_set_stress(void* h, const char* name, const double* v, long n) {
   else if (s == "mu_s") t = &o->in_mu_s; else if (s == "mu_d") t = &o->in_mu_d; else if (s == "Dc") t = &o->in_Dc;
   else if (s == "C0") t = &o->in_C0; else if (s == "a") t = &o->in_a; else if (s == "b") t = &o->in_b;
   else if (s == "L") t = &o->in_L; else if (s == "Vw") t = &o->in_Vw; else if (s == "State") t = &o->in_State;
+  else if (s == "TP_hy") t = &o->in_TP_hy;
   if (!t) return 1;
   t->assign(v, v + n);
   return 0;
@@ -1678,6 +1737,7 @@ int orc_fault_deriv(void* h, int fx, int fy, int fz) {
   return 0;
 }
 int orc_rup_sensor(void* h) { ((Oracle*)h)->rup_sensor_calc(); return 0; }
+int orc_thermpress(void* h) { ((Oracle*)h)->thermpress(); return 0; }
 int orc_smooth(void* h) { ((Oracle*)h)->smooth_gauss(); return 0; }
 // kernel-level entry points for unit parity tests
 int orc_wave_deriv(void* h, int fx, int fy, int fz) { ((Oracle*)h)->wave_deriv(fx, fy, fz); return 0; }

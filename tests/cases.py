@@ -71,9 +71,10 @@ def sw_case(NX=30, NY=80, NZ=50, curved=False, seed=1, **kw):
 
 def rs_case(NX=30, NY=80, NZ=50, curved=False, seed=2, **kw):
     """Rate-and-state aging law with a Gaussian-in-time overstress patch."""
-    p = base_params(NX=NX, NY=NY, NZ=NZ, Rupture=1, Friction_type=1, INPORT_STRESS_TYPE=1,
+    p = base_params(NX=NX, NY=NY, NZ=NZ, Rupture=1, INPORT_STRESS_TYPE=1,
                     Fault_grid=[31, NY - 30, 31, NZ], Asp_grid=[0, 0, 0, 0],
-                    RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12, RS_fw=0.2, smooth_load_T=0.1, **dict({"RS_L": 0.02}, **kw))
+                    RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12, RS_fw=0.2, smooth_load_T=0.1,
+                    **dict({"RS_L": 0.02, "Friction_type": 1}, **kw))
     case = {"params": p}
     if curved:
         p["INPORT_GRID_TYPE"] = 1

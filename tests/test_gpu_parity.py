@@ -293,3 +293,59 @@ def test_fault_rs_slip_law_steps():
         e = relerr(s.get(name), o.get(name))
         print(name, e)
         assert e < tol, (name, e)
+
+
+def tp_case(TP_n=21, hy=2.0 ** -11):
+    """Rate and state with thermal pressurization (Friction_type 3: the slip
+    law with flash weakening for the state, rk4.cu:902-911, plus the pore
+    pressure of thermpress.cu in the trial), L = 0.1 m as the slip-law test.
+    The hydraulic diffusivity (about 4.9e-4 m^2/s) is exact in FP32, the
+    precision of the stress input."""
+    case = cases.rs_case(RS_L=0.1, Friction_type=3, TP_n=TP_n)
+    case["stress"]["L"] = np.full_like(case["stress"]["L"], 0.1)
+    case["stress"]["TP_hy"] = np.full_like(case["stress"]["L"], hy)
+    return case
+
+
+def test_thermpress_phase():
+    """cgfdm_thermpress (thermpress.cu:22-233) alone, from the same slip
+    rate, friction and normal stress (values exact in FP32) on both sides,
+    five calls: the FP64 temperature and pressure rises agree with the
+    oracle's to 1e-12 relative (both FP64; the GPU's exp differs in the
+    last bits)."""
+    case = tp_case()
+    s, o = make_pair(case)
+    n2 = case["params"]["NY"] * case["params"]["NZ"]
+    for side in (s, o):
+        side.put("Vs1", np.full(n2, 2.0 ** -6, np.float32))
+        side.put("Vs2", np.full(n2, 2.0 ** -8, np.float32))
+        side.put("friction", np.full(n2, 0.5, np.float32))
+        side.put("Tn", np.full(n2, -1e8, np.float32))
+    for _ in range(5):
+        s.thermpress()
+        o.thermpress()
+    for name in ("TP_T", "TP_P"):
+        a, b = s.get(name), o.get(name)
+        assert np.abs(b).max() > 0
+        e = relerr(a, b)
+        assert e < 1e-12, (name, e)
+
+
+def test_fault_tp_steps():
+    """Rate and state with thermal pressurization, 40 steps into the
+    rupture (slip rate ~0.4 m/s, pore pressure ~1 MPa): slip rate,
+    tractions, slip, state and the temperature and pressure rises against
+    the oracle, with the bounds of the rate-state tests."""
+    case = tp_case()
+    s, o = make_pair(case)
+    for it in range(40):
+        s.step(it)
+        o.step(it)
+    assert np.max(np.abs(o.get("Vs1"))) > 0.1  # nucleated
+    assert np.max(o.get("TP_P")) > 0
+    errs = {}
+    for name, tol in (("Vs1", 1e-3), ("Ts1", 1e-4), ("Tn", 1e-4), ("slip", 1e-3), ("State", 1e-4),
+                      ("TP_T", 1e-3), ("TP_P", 1e-3)):
+        errs[name] = relerr(s.get(name), o.get(name))
+        assert errs[name] < tol, (name, errs[name])
+    print("thermal pressurization, 40 steps, rel L2 vs oracle:", errs)

@@ -480,3 +480,111 @@ def test_smooth_sides_read_their_own_planes():
     new = o.get("FWs").reshape(9, 2, nz, ny)
     np.testing.assert_array_equal(new[:, 0], base[:, 0])
     assert np.abs(new[3:, 1, 3:-3, 3:-3] - base[3:, 1, 3:-3, 3:-3]).max() > 1.0
+
+
+# ---------------------------------------------------------------------------
+# Thermal pressurization (thermpress.cu:22-233): temperature T and pore
+# pressure P on a 1-D grid x = i dh normal to the fault, mirror symmetric at
+# both ends, T' = a_th T'' + omega(x), P' = a_hy P'' + Lambda T', with the
+# frictional heating tau V spread as a Gaussian of half-width h.
+TP_LAMBDA, TP_RHOC, TP_H, TP_ATH = 0.1e6, 2.7e6, 20e-3, 1.0e-6
+
+
+def _tp_oracle(TP_n, hy, DT=0.008):
+    case = cases.rs_case(NX=12, NY=64, NZ=36, Friction_type=3, TP_n=TP_n, DT=DT)
+    case["stress"]["TP_hy"] = np.full((36, 64), hy)
+    o = Oracle(case["params"])
+    cases.apply_to_oracle(case, o)
+    o.setup()
+    n2 = 64 * 36
+    united = o.get("united").astype(bool) if o.size("united") else np.zeros(n2, bool)
+    return o, n2, united
+
+
+def _tp_drive(o, n2, V=1e-3, f=0.6, tn=-100e6):
+    o.put("Vs1", np.full(n2, V))
+    o.put("Vs2", np.zeros(n2))
+    o.put("friction", np.full(n2, f))
+    o.put("Tn", np.full(n2, tn))
+
+
+def test_thermpress_undrained_limit_p_equals_lambda_t():
+    """Without hydraulic diffusion (a_hy = 0) P' = Lambda T', so from zero
+    P = Lambda T at every level, whatever the heating and the thermal
+    diffusion (the coupling takes all of T', diffusive part included)."""
+    o, n2, united = _tp_oracle(21, 0.0)
+    _tp_drive(o, n2)
+    for _ in range(10):
+        o.thermpress()
+    T = o.get("TP_T").reshape(21, n2)[:, ~united]
+    P = o.get("TP_P").reshape(21, n2)[:, ~united]
+    assert T[0].min() > 0
+    np.testing.assert_allclose(P, TP_LAMBDA * T, rtol=1e-12, atol=1e-9 * np.abs(P).max())
+
+
+def test_thermpress_mirror_boundaries_conserve_heat():
+    """The mirror-symmetric discrete Laplacian conserves the trapezoid
+    integral of T over [0, w]: after each call it has grown by exactly the
+    integral of the heating over the substeps (constant here: P stays small
+    against the normal stress only approximately, so tau is recomputed from
+    the oracle's own P(0) per substep)."""
+    TP_n, hy, DT = 31, 4e-4, 0.008
+    o, n2, united = _tp_oracle(TP_n, hy, DT)
+    V, f, tn = 1e-2, 0.6, -100e6
+    _tp_drive(o, n2, V, f, tn)
+    w = math.sqrt(4.0 * max(TP_ATH, hy) * 25.0)
+    dh = w / (TP_n - 1)
+    nsub = int(DT / (0.1 * dh * dh / max(TP_ATH, hy))) + 1
+    dt = DT / nsub
+    x = np.arange(TP_n) * dh
+    g = np.exp(-0.5 * x * x / TP_H ** 2) / (TP_RHOC * TP_H * math.sqrt(2 * math.pi))
+    wts = np.full(TP_n, dh)
+    wts[0] = wts[-1] = dh / 2
+    q = int(np.flatnonzero(~united)[0])
+    heat = 0.0
+    for _ in range(3):
+        T0 = o.get("TP_T").reshape(TP_n, n2)[:, q].copy()
+        P0 = o.get("TP_P").reshape(TP_n, n2)[:, q].copy()
+        # replay the substeps' heating (tau from P(0) of each substep) with
+        # an independent implementation of the same explicit update
+        T, P = T0.copy(), P0.copy()
+        for _s in range(nsub):
+            tauV = -f * (tn + P[0]) * V
+            om = g * tauV
+            lap = lambda u: (np.r_[u[1:], u[-2]] - 2 * u + np.r_[u[1], u[:-1]]) / dh ** 2
+            dT = TP_ATH * lap(T) + om
+            dP = hy * lap(P) + TP_LAMBDA * dT
+            heat += dt * float(wts @ om)
+            T, P = T + dt * dT, P + dt * dP
+        o.thermpress()
+        T1 = o.get("TP_T").reshape(TP_n, n2)[:, q]
+        assert abs(float(wts @ T1) - heat) <= 1e-11 * heat
+        np.testing.assert_allclose(T1, T, rtol=1e-12)
+
+
+def test_thermpress_matches_continuous_solution():
+    """Heating at a rate that stays constant (P << |tau_n|), on a domain
+    wide against the diffusion length (a_hy = 1e-2 m^2/s: w = 1 m), with
+    T nearly adiabatic (a_th = 1e-6): the fault pressure follows the
+    continuous solution of P' = a_hy P'' + Lambda omega(x) for a Gaussian
+    source of half-width h,
+        P(0, t) = Lambda (tau V / rho c) (sqrt(h^2 + 2 a_hy t) - h) / (a_hy sqrt(2 pi)),
+    and T(0, t) the adiabatic (tau V / rho c) t / (h sqrt(2 pi)).  TP_n = 201
+    (dh = 5 mm = h / 4); bound 1.5% (grid, explicit Euler, the small
+    a_th)."""
+    hy, V, f, tn = 1e-2, 1e-3, 0.6, -100e6
+    o, n2, united = _tp_oracle(201, hy, DT=0.01)
+    _tp_drive(o, n2, V, f, tn)
+    steps = 50
+    for _ in range(steps):
+        o.thermpress()
+    t = steps * 0.01
+    q = int(np.flatnonzero(~united)[0])
+    P0 = o.get("TP_P").reshape(201, n2)[0, q]
+    T0 = o.get("TP_T").reshape(201, n2)[0, q]
+    tauV = -f * tn * V
+    P_ref = TP_LAMBDA * tauV / TP_RHOC * (math.sqrt(TP_H ** 2 + 2 * hy * t) - TP_H) / (hy * math.sqrt(2 * math.pi))
+    T_ref = tauV / TP_RHOC * t / (TP_H * math.sqrt(2 * math.pi))
+    assert P0 < 1e-3 * abs(tn)
+    assert abs(P0 / P_ref - 1) < 0.015, (P0, P_ref)
+    assert abs(T0 / T_ref - 1) < 0.015, (T0, T_ref)
