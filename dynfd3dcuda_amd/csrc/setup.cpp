@@ -1,5 +1,6 @@
-// Host-side setup in double precision: grid, metric, media, absorbing-layer
-// profiles, fault coefficients and fault initial state, then upload to HBM.
+// Setup in double precision: grid on the host, metric and media on the GPU,
+// absorbing-layer profiles, fault coefficients and fault initial state on
+// the host, then upload to HBM.
 // Replaces construct_coord (coord.cu:153-328), extend_crew_array /
 // cal_metric / extend_Symm_array (metric.cu), init_media1d/3d (media.cu),
 // abs_init (init_pml.cu), init_cerjan_host (cerjan.cu:31-164),
@@ -63,6 +64,102 @@ uint16_t to_bf16(float f) {  // round to nearest even
   std::memcpy(&u, &f, 4);
   uint32_t r = u + 0x7fff + ((u >> 16) & 1);
   return (uint16_t)(r >> 16);
+}
+
+// Metric and media of one interior point (cal_metric, metric.cu:74-109;
+// bimaterial split metric.cu:123-150; init_media1d/3d, media.cu:63-204) in
+// FP64 from the FP64 coordinates: the same function runs on the GPU for
+// the whole grid and on the host for the planes its fault and free-surface
+// coefficients read.  out: xi(3) eta(3) zeta(3) jac lam mu rho.
+struct MetricIn {
+  const double* X;
+  const double* Y;
+  const double* Z;
+  long nyz;
+  int ny, nz, nx, nj, nk, NX;
+  double rDH;
+  double lam1, miu1, rho1, lam2, miu2, rho2;
+  int nl;
+  const float* d1;
+  const float* r1;
+  const float* p1;
+  const float* s1;
+  const float* vp3;
+  const float* vs3;
+  const float* rho3;
+};
+__host__ __device__ inline void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__host__ __device__ inline double dot3d(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+// index of the interior point whose values a ghost point mirrors
+// (extend_Symm_array, metric.cu:173-246, applied along x, then y, then z)
+__host__ __device__ inline int mirror_idx(int a, int n) { return a < 3 ? 6 - a : (a > n - 4 ? 2 * (n - 4) - a : a); }
+__host__ __device__ inline void metric_point(const MetricIn& m, int i, int j, int k, double* o) {
+  const long q = j + (long)k * m.ny + (long)i * m.nyz;
+  const long s[3] = {m.nyz, 1, (long)m.ny};
+  double t[3][3];  // t[axis][component] = d(x,y,z)/d(xi,eta,zeta)
+  for (int a = 0; a < 3; a++) {
+    t[a][0] = 0.5 * m.rDH * (m.X[q + s[a]] - m.X[q - s[a]]);
+    t[a][1] = 0.5 * m.rDH * (m.Y[q + s[a]] - m.Y[q - s[a]]);
+    t[a][2] = 0.5 * m.rDH * (m.Z[q + s[a]] - m.Z[q - s[a]]);
+  }
+  double g[3];
+  cross3(t[0], t[1], g);
+  const double jac = dot3d(g, t[2]);
+  o[9] = jac;
+  cross3(t[1], t[2], g);
+  for (int a = 0; a < 3; a++) o[a] = g[a] / jac;
+  cross3(t[2], t[0], g);
+  for (int a = 0; a < 3; a++) o[3 + a] = g[a] / jac;
+  cross3(t[0], t[1], g);
+  for (int a = 0; a < 3; a++) o[6 + a] = g[a] / jac;
+  double lam = i < m.NX / 2 ? m.lam1 : m.lam2, miu = i < m.NX / 2 ? m.miu1 : m.miu2, rho = i < m.NX / 2 ? m.rho1 : m.rho2;
+  if (m.nl > 0) {
+    const double depth1 = -m.Z[q];
+    double r = m.r1[m.nl - 1], va = m.p1[m.nl - 1], vb = m.s1[m.nl - 1];
+    for (int l = m.nl - 1; l >= 0; l--)
+      if (depth1 > m.d1[l]) { r = m.r1[l]; va = m.p1[l]; vb = m.s1[l]; break; }
+    lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
+  }
+  if (m.vp3) {
+    const long q3 = (j - 3) + (long)(k - 3) * m.nj + (long)(i - 3) * m.nj * m.nk;
+    const double r = m.rho3[q3], va = m.vp3[q3], vb = m.vs3[q3];
+    lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
+  }
+  o[10] = lam; o[11] = miu; o[12] = rho;
+}
+
+__device__ inline float metric_store(double v, float*) { return (float)v; }
+__device__ inline uint16_t metric_store(double v, uint16_t*) {  // bf16, round to nearest even
+  const uint32_t u = __float_as_uint((float)v);
+  return (uint16_t)((u + 0x7fff + ((u >> 16) & 1)) >> 16);
+}
+
+// The 13 metric and media words of every stored point (ghosts mirrored), in
+// the device layout, and the largest P-wave speed of the interior as the
+// bits of a positive double.  One thread per point, j fastest.
+template <class OT>
+__global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, long pyz, int py, int jpad,
+                                                           unsigned long long* vpmax_bits) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long total = (long)m.nx * m.nz * m.ny;
+  double vp = 0.0;
+  if (idx < total) {
+    const int j = (int)(idx % m.ny);
+    const long r = idx / m.ny;
+    const int k = (int)(r % m.nz), i = (int)(r / m.nz);
+    double v[13];
+    metric_point(m, mirror_idx(i, m.nx), mirror_idx(j, m.ny), mirror_idx(k, m.nz), v);
+    const long base = (long)k * py + jpad + j;
+    for (int c = 0; c < 13; c++) out[((long)i * 13 + c) * pyz + base] = metric_store(v[c], out);
+    if (i >= 3 && i < m.nx - 3 && j >= 3 && j < m.ny - 3 && k >= 3 && k < m.nz - 3)
+      vp = sqrt((v[10] + 2 * v[11]) / v[12]);
+  }
+  for (int o = 32; o > 0; o >>= 1) vp = fmax(vp, __shfl_xor(vp, o));
+  if ((threadIdx.x & 63) == 0 && vp > 0.0) atomicMax(vpmax_bits, (unsigned long long)__double_as_longlong(vp));
 }
 
 template <class T>
@@ -229,75 +326,96 @@ int host_setup(Ctx& c, std::string& err) {
       }
   c.hmin = hmin;
 
-  // ---- metric + media (metric.cu:15-171, media.cu:63-204), kept in double here
-  std::vector<double> Md(13 * nxyz, 0.0);
+  // ---- metric + media (metric.cu:15-171, extend_Symm_array metric.cu:173-246,
+  // media.cu:63-204): on the GPU, FP64 from the FP64 coordinates, rounded
+  // once into the device layout (FP32 or BF16); the host keeps no metric
+  // volume.  The host evaluates the same point function on the planes its
+  // own coefficients read: i0-3..i0+3 around the fault and the row k = nz-4
+  // under the free surface.
+  const long pyz = d.pyz;
+  MetricIn mh{};
+  mh.X = X; mh.Y = Y; mh.Z = Z;
+  mh.nyz = nyz; mh.ny = ny; mh.nz = nz; mh.nx = nx; mh.nj = nj; mh.nk = nk; mh.NX = p.NX;
+  mh.rDH = 1.0 / p.DH;
+  mh.lam1 = p.bi_rho1 * (p.bi_vp1 * p.bi_vp1 - 2 * p.bi_vs1 * p.bi_vs1); mh.miu1 = p.bi_rho1 * p.bi_vs1 * p.bi_vs1;
+  mh.lam2 = p.bi_rho2 * (p.bi_vp2 * p.bi_vp2 - 2 * p.bi_vs2 * p.bi_vs2); mh.miu2 = p.bi_rho2 * p.bi_vs2 * p.bi_vs2;
+  mh.rho1 = p.bi_rho1; mh.rho2 = p.bi_rho2;
+  mh.nl = (int)c.m1d_depth.size();
+  mh.d1 = c.m1d_depth.data(); mh.r1 = c.m1d_rho.data(); mh.p1 = c.m1d_vp.data(); mh.s1 = c.m1d_vs.data();
+  if (!c.m3d_vp.empty()) { mh.vp3 = c.m3d_vp.data(); mh.vs3 = c.m3d_vs.data(); mh.rho3 = c.m3d_rho.data(); }
   {
-    double rDH = 1.0 / p.DH;
-    double lam1 = p.bi_rho1 * (p.bi_vp1 * p.bi_vp1 - 2 * p.bi_vs1 * p.bi_vs1), miu1 = p.bi_rho1 * p.bi_vs1 * p.bi_vs1;
-    double lam2 = p.bi_rho2 * (p.bi_vp2 * p.bi_vp2 - 2 * p.bi_vs2 * p.bi_vs2), miu2 = p.bi_rho2 * p.bi_vs2 * p.bi_vs2;
-    int nl = (int)c.m1d_depth.size();
+    // device copies of the inputs, freed when the metric is written
+    std::vector<void*> tmp;
+    auto dput = [&](const void* src, size_t bytes) -> void* {
+      void* q = nullptr;
+      if (bytes == 0) return nullptr;
+      if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+      tmp.push_back(q);
+      if (hipMemcpy(q, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+      return q;
+    };
+    MetricIn md = mh;
+    double* dC = (double*)dput(c.hostC.data(), c.hostC.size() * sizeof(double));
+    md.X = dC; md.Y = dC ? dC + nxyz : nullptr; md.Z = dC ? dC + 2 * nxyz : nullptr;
+    const size_t n1 = c.m1d_depth.size() * sizeof(float);
+    md.d1 = (const float*)dput(c.m1d_depth.data(), n1); md.r1 = (const float*)dput(c.m1d_rho.data(), n1);
+    md.p1 = (const float*)dput(c.m1d_vp.data(), n1); md.s1 = (const float*)dput(c.m1d_vs.data(), n1);
+    if (mh.vp3) {
+      const size_t n3 = c.m3d_vp.size() * sizeof(float);
+      md.vp3 = (const float*)dput(c.m3d_vp.data(), n3); md.vs3 = (const float*)dput(c.m3d_vs.data(), n3);
+      md.rho3 = (const float*)dput(c.m3d_rho.data(), n3);
+    }
+    unsigned long long* dvp = nullptr;
+    void* vq = nullptr;
+    if (hipMalloc(&vq, sizeof(unsigned long long)) == hipSuccess) { tmp.push_back(vq); dvp = (unsigned long long*)vq; }
+    bool ok = dC && dvp && (mh.nl == 0 || (md.d1 && md.r1 && md.p1 && md.s1)) && (!mh.vp3 || (md.vp3 && md.vs3 && md.rho3));
+    const long nmet = (long)nx * 13 * pyz;
+    void* met = nullptr;
+    if (ok) met = p.MetricBF16 ? (void*)dmalloc<uint16_t>(c, nmet) : (void*)dmalloc<float>(c, nmet);
+    ok = ok && met && hipMemset(dvp, 0, sizeof(unsigned long long)) == hipSuccess;
+    if (ok) {
+      const long total = (long)nx * nz * ny;
+      const unsigned blocks = (unsigned)((total + 255) / 256);
+      if (p.MetricBF16)
+        hipLaunchKernelGGL(metric_media_kernel<uint16_t>, dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp);
+      else
+        hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks), dim3(256), 0, 0, md, (float*)met, pyz, d.py, d.jpad, dvp);
+      unsigned long long bits = 0;
+      ok = hipGetLastError() == hipSuccess && hipMemcpy(&bits, dvp, sizeof bits, hipMemcpyDeviceToHost) == hipSuccess;
+      double vpm;
+      std::memcpy(&vpm, &bits, sizeof vpm);
+      c.vpmax = vpm;
+    }
+    for (void* q : tmp) hipFree(q);
+    if (!ok) { err = "metric computation on the device failed"; return 6; }
+    c.met = met;
+  }
+  const int ip0 = std::max(d.i0 - 3, 0), ip1 = std::min(d.i0 + 3, nx - 1);
+  std::vector<double> mdx((long)(ip1 - ip0 + 1) * nyz * 13), mdz((long)nx * ny * 13);
+  if (p.Rupture) {
 #pragma omp parallel for
-    for (int i = 3; i < nx - 3; i++)
-      for (int k = 3; k < nz - 3; k++)
-        for (int j = 3; j < ny - 3; j++) {
-          long q = P(i, j, k);
-          const long s[3] = {nyz, 1, (long)ny};
-          double t[3][3];  // t[axis][component] = d(x,y,z)/d(xi,eta,zeta)
-          for (int a = 0; a < 3; a++) {
-            t[a][0] = 0.5 * rDH * (X[q + s[a]] - X[q - s[a]]);
-            t[a][1] = 0.5 * rDH * (Y[q + s[a]] - Y[q - s[a]]);
-            t[a][2] = 0.5 * rDH * (Z[q + s[a]] - Z[q - s[a]]);
-          }
-          double g[3];
-          cross(t[0], t[1], g);
-          double jac = dot(g, t[2]);
-          Md[9 * nxyz + q] = jac;
-          cross(t[1], t[2], g);
-          for (int a = 0; a < 3; a++) Md[a * nxyz + q] = g[a] / jac;
-          cross(t[2], t[0], g);
-          for (int a = 0; a < 3; a++) Md[(3 + a) * nxyz + q] = g[a] / jac;
-          cross(t[0], t[1], g);
-          for (int a = 0; a < 3; a++) Md[(6 + a) * nxyz + q] = g[a] / jac;
-          double lam = i < p.NX / 2 ? lam1 : lam2, miu = i < p.NX / 2 ? miu1 : miu2, rho = i < p.NX / 2 ? p.bi_rho1 : p.bi_rho2;
-          if (nl > 0) {
-            double depth1 = -Z[q];
-            double r = c.m1d_rho[nl - 1], va = c.m1d_vp[nl - 1], vb = c.m1d_vs[nl - 1];
-            for (int l = nl - 1; l >= 0; l--)
-              if (depth1 > c.m1d_depth[l]) { r = c.m1d_rho[l]; va = c.m1d_vp[l]; vb = c.m1d_vs[l]; break; }
-            lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
-          }
-          if (!c.m3d_vp.empty()) {  // init_media3d (media.cu:140-204)
-            const long q3 = (j - 3) + (long)(k - 3) * nj + (long)(i - 3) * nj * nk;
-            const double r = c.m3d_rho[q3], va = c.m3d_vp[q3], vb = c.m3d_vs[q3];
-            lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
-          }
-          Md[10 * nxyz + q] = lam; Md[11 * nxyz + q] = miu; Md[12 * nxyz + q] = rho;
-        }
-    // mirror into ghosts (extend_Symm_array, metric.cu:173-246)
-#pragma omp parallel for
-    for (int l = 0; l < 13; l++) {
-      double* A = &Md[l * nxyz];
+    for (int i = ip0; i <= ip1; i++)
       for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
-          for (int n = 1; n <= 3; n++) { A[P(3 - n, j, k)] = A[P(3 + n, j, k)]; A[P(nx - 4 + n, j, k)] = A[P(nx - 4 - n, j, k)]; }
-      for (int k = 0; k < nz; k++)
-        for (int i = 0; i < nx; i++)
-          for (int n = 1; n <= 3; n++) { A[P(i, 3 - n, k)] = A[P(i, 3 + n, k)]; A[P(i, ny - 4 + n, k)] = A[P(i, ny - 4 - n, k)]; }
-      for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++)
-          for (int n = 1; n <= 3; n++) { A[P(i, j, 3 - n)] = A[P(i, j, 3 + n)]; A[P(i, j, nz - 4 + n)] = A[P(i, j, nz - 4 - n)]; }
-    }
+          metric_point(mh, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(k, nz), &mdx[(((long)(i - ip0) * nz + k) * ny + j) * 13]);
   }
+  if (freenode) {
+#pragma omp parallel for
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        metric_point(mh, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(nz - 4, nz), &mdz[((long)i * ny + j) * 13]);
+  }
+  // metric word m of point q = P(i, j, k), from the plane caches
+  auto MD = [&](int m, long q) -> double {
+    const int i = (int)(q / nyz), k = (int)((q % nyz) / ny), j = (int)(q % ny);
+    if (p.Rupture && i >= ip0 && i <= ip1) return mdx[(((long)(i - ip0) * nz + k) * ny + j) * 13 + m];
+    if (freenode && k == nz - 4) return mdz[((long)i * ny + j) * 13 + m];
+    double v[13];
+    metric_point(mh, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(k, nz), v);
+    return v[m];
+  };
   // CFL (main.cu:302-345)
-  double vpmax = 0;
-#pragma omp parallel for reduction(max : vpmax)
-  for (int i = 3; i < nx - 3; i++)
-    for (int k = 3; k < nz - 3; k++)
-      for (int j = 3; j < ny - 3; j++) {
-        long q = P(i, j, k);
-        vpmax = std::max(vpmax, std::sqrt((Md[10 * nxyz + q] + 2 * Md[11 * nxyz + q]) / Md[12 * nxyz + q]));
-      }
-  c.vpmax = vpmax;
+  const double vpmax = c.vpmax;
   float dtmax = (float)(1.3 * hmin / vpmax);
   if (!((float)p.DT < dtmax)) {
     char b[256];
@@ -342,8 +460,8 @@ int host_setup(Ctx& c, std::string& err) {
         double rb[3] = {1.0 / Bp[0][i - 3], 1.0 / Bp[1][j - 3], 1.0 / Bp[2][k - 3]};
         long q = P(i, j, k);
         double e[3][3];
-        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = Md[a * nxyz + q];
-        double lam = Md[10 * nxyz + q], mu = Md[11 * nxyz + q];
+        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = MD(a, q);
+        double lam = MD(10, q), mu = MD(11, q);
         // traction on the zeta surface produced by derivatives along axis b:
         // K_b[r][c] = lam e3[r] eb[c] + mu e3[c] eb[r] + mu (e3.eb) delta_rc
         double K[3][3][3];
@@ -367,37 +485,13 @@ int host_setup(Ctx& c, std::string& err) {
   }
 
   // ---- device allocations: 4 wavefield buffers (step start S, stage A/B, accumulator T)
-  const long pyz = d.pyz, nvol = (long)nx * 9 * pyz;  // device wavefield words (padded rows)
+  const long nvol = (long)nx * 9 * pyz;  // device wavefield words (padded rows)
   c.S = dmalloc<float>(c, nvol);
   c.A = dmalloc<float>(c, nvol);
   c.B = dmalloc<float>(c, nvol);
   c.Tacc = dmalloc<float>(c, nvol);
   if (!c.S || !c.A || !c.B || !c.Tacc) { err = "hipMalloc failed for the wavefield"; return 6; }
   c.cur = c.S;
-  // device metric: plane-interleaved with padded rows (Dims)
-  const long nmet = (long)nx * 13 * pyz;
-  auto met_dev = [&](int i, int m, int k, int j) { return ((long)i * 13 + m) * pyz + (long)k * d.py + d.jpad + j; };
-  if (p.MetricBF16) {
-    std::vector<uint16_t> h(nmet, 0);
-#pragma omp parallel for
-    for (int i = 0; i < nx; i++)
-      for (int m = 0; m < 13; m++)
-        for (int k = 0; k < nz; k++)
-          for (int j = 0; j < ny; j++) h[met_dev(i, m, k, j)] = to_bf16((float)Md[m * nxyz + P(i, j, k)]);
-    uint16_t* dm = dmalloc<uint16_t>(c, nmet);
-    if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
-    c.met = dm;
-  } else {
-    std::vector<float> h(nmet, 0.0f);
-#pragma omp parallel for
-    for (int i = 0; i < nx; i++)
-      for (int m = 0; m < 13; m++)
-        for (int k = 0; k < nz; k++)
-          for (int j = 0; j < ny; j++) h[met_dev(i, m, k, j)] = (float)Md[m * nxyz + P(i, j, k)];
-    float* dm = dmalloc<float>(c, nmet);
-    if (!dm || !upload(dm, h)) { err = "metric upload failed"; return 6; }
-    c.met = dm;
-  }
   c.matVx2Vz = dmalloc<float>(c, mVx.size());
   c.matVy2Vz = dmalloc<float>(c, mVy.size());
   upload(c.matVx2Vz, mVx);
@@ -454,8 +548,8 @@ int host_setup(Ctx& c, std::string& err) {
       for (int j = 0; j < ny; j++) {
         long q = P(i0, j, k), pf = j + (long)k * ny;
         double e[3][3];
-        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = Md[a * nxyz + q];
-        double jac = Md[9 * nxyz + q];
+        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = MD(a, q);
+        double jac = MD(9, q);
         double mt[3][3];
         std::memcpy(mt, e, sizeof mt);
         inv3(mt);
@@ -470,10 +564,10 @@ int host_setup(Ctx& c, std::string& err) {
         }
         jacvec[pf] = (float)(jac * nn);
         jac0[pf] = (float)jac;
-        double lamf[2] = {Md[10 * nxyz + q - nyz], Md[10 * nxyz + q + nyz]};
-        double muf[2] = {Md[11 * nxyz + q - nyz], Md[11 * nxyz + q + nyz]};
-        rho_f[pf] = (float)Md[12 * nxyz + q - nyz];
-        rho_f[nyz + pf] = (float)Md[12 * nxyz + q + nyz];
+        double lamf[2] = {MD(10, q - nyz), MD(10, q + nyz)};
+        double muf[2] = {MD(11, q - nyz), MD(11, q + nyz)};
+        rho_f[pf] = (float)MD(12, q - nyz);
+        rho_f[nyz + pf] = (float)MD(12, q + nyz);
         double Dl[2][3][3][3][3];
         for (int m = 0; m < 2; m++) {
           for (int a = 0; a < 3; a++)
@@ -512,7 +606,7 @@ int host_setup(Ctx& c, std::string& err) {
           double v1n = nrm(v1);
           cross(v1, x_et, v3);
           double h[3] = {dot(v3, e[0]) / v1n, dot(v3, e[1]) / v1n, dot(v3, e[2]) / v1n};
-          double miu = Md[11 * nxyz + q];
+          double miu = MD(11, q);
           double Vxf[2][3][3], Vyf[2][3][3];
           for (int m = 0; m < 2; m++) {
             double Df[3][3][3];
