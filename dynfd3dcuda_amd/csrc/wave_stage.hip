@@ -36,7 +36,7 @@ namespace {
 constexpr int NTHR = 512;  // threads per block
 constexpr int TJ = 64;     // default tile: one wavefront per k-row, j contiguous
 constexpr int TK = NTHR / TJ;
-constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j x 16 k)
+constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j x 8 k)
 constexpr int NTHRN = 256;  // threads per block of the narrow tiles
 constexpr int TKN = NTHRN / TJN;
 
@@ -709,8 +709,8 @@ int interior_lo_j(const Params& p) {
 // lists.  The interior box excludes every face band where a PML profile or
 // the Cerjan sponge is non-trivial and the three free-surface rows; its
 // tiles start at the box edge so that no interior tile straddles a band.
-// Interior items come first (chunks of about 2048 blocks in total, so 256
-// CUs stay balanced), then the edge items; each list is chunk-major so that
+// Interior items come first (chunks sized for about 4096 blocks in total, so
+// 256 CUs stay balanced), then the edge items; each list is chunk-major so that
 // one XCD's contiguous share covers neighbouring tiles.
 int upload_wave_cold(Ctx& c) {
   WaveCold w{};
