@@ -15,7 +15,8 @@
 Kernel classes: stage_interior / stage_band / stage_surface (the three
 instantiations of the wave stage kernel; stage_surface includes the two
 free-surface precompute passes), pml_aux, fault_pre, fault_rs, fault_deriv,
-fault_post, rup_sensor; everything else is listed under its own name.
+fault_post, rup_sensor, thermpress, setup_metric (the setup's metric
+pass); everything else is listed under its own name.
 """
 import argparse
 import collections
@@ -44,6 +45,10 @@ def kernel_class(name):
             return k[: -len("_kernel")]
     if "rup_sensor" in name:
         return "rup_sensor"
+    if "thermpress_kernel" in name:
+        return "thermpress"
+    if "metric_media_kernel" in name:
+        return "setup_metric"
     return name.split("(")[0].strip()
 
 
@@ -97,8 +102,8 @@ def traffic(a):
     nst = len(fetch.get("fault_pre", fetch.get("stage_interior", [1])))
     out.append(f"rk_stages_profiled: {nst}")
     for k in sorted(fetch):
-        if k not in write or k.startswith("__"):
-            continue
+        if k not in write or k.startswith("__") or k.startswith("setup_"):
+            continue  # copies, fills and setup passes are not per-stage work
         fb = sum(fetch[k]) / nst * 1024.0 * f_fetch
         wb = sum(write[k]) / nst * 1024.0 * f_write
         out.append(f"{k}_fetch_bytes_per_stage: {fb:.6g}")
