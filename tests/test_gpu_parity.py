@@ -307,13 +307,14 @@ def tp_case(TP_n=21, hy=2.0 ** -11):
     return case
 
 
-def test_thermpress_phase():
+@pytest.mark.parametrize("TP_n", [1, 21])
+def test_thermpress_phase(TP_n):
     """cgfdm_thermpress (thermpress.cu:22-233) alone, from the same slip
     rate, friction and normal stress (values exact in FP32) on both sides,
     five calls: the FP64 temperature and pressure rises agree with the
     oracle's to 1e-12 relative (both FP64; the GPU's exp differs in the
-    last bits)."""
-    case = tp_case()
+    last bits).  TP_n = 1 is the diffusion-free case."""
+    case = tp_case(TP_n=TP_n)
     s, o = make_pair(case)
     n2 = case["params"]["NY"] * case["params"]["NZ"]
     for side in (s, o):
@@ -349,3 +350,17 @@ def test_fault_tp_steps():
         errs[name] = relerr(s.get(name), o.get(name))
         assert errs[name] < tol, (name, errs[name])
     print("thermal pressurization, 40 steps, rel L2 vs oracle:", errs)
+
+
+def test_thermpress_needs_friction_type_3():
+    """cgfdm_thermpress on a slip-weakening fault is a state error, and the
+    TP output fields are empty there."""
+    case = cases.sw_case()
+    s = dyn.Solver(case["params"])
+    cases.apply_to_solver(case, s)
+    s.setup()
+    with pytest.raises(dyn.CgfdmError) as e:
+        s.thermpress()
+    assert e.value.code == dyn.ESTATE
+    with pytest.raises(dyn.CgfdmError):  # no thermal-pressurization state to read
+        s.field_info("TP_T")
