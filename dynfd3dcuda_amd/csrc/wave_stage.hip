@@ -242,6 +242,16 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
   constexpr int QLO = FX > 0 ? -1 : -3;  // queue holds planes i+QLO .. i+QLO+4
   constexpr int CTR = -QLO;
   __shared__ float lds[2][9][LK][LJ];
+  // the absorbing layers' uniform descriptors, staged once per block: read
+  // through `cold` inside the plane loop they would be fetched again from
+  // global memory after every store (a chain of dependent round trips per
+  // plane and face), and registers for them would spill
+  __shared__ const float* s_auxY[6];
+  __shared__ const float* s_pBx;
+  __shared__ const float* s_cjx;
+  __shared__ const float* s_mV[2];
+  __shared__ int s_on[6];
+  __shared__ int s_N;
 
   const int item = xcd_remap((int)blockIdx.x, a.nwork);
   const int4 wk = a.work[2 * item], we = a.work[2 * item + 1];
@@ -282,6 +292,12 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
   }
   const uint32_t hcol4 = hcol << 2;
 
+  if (KIND >= 1 && threadIdx.y == 0 && threadIdx.x < 6) {
+    s_auxY[threadIdx.x] = cold->auxY[threadIdx.x];
+    s_on[threadIdx.x] = cold->on[threadIdx.x];
+    if (threadIdx.x == 0) { s_N = cold->N; s_pBx = cold->pB[0]; s_cjx = cold->cj[0]; }
+    if (threadIdx.x < 2) s_mV[threadIdx.x] = threadIdx.x ? cold->mVy : cold->mVx;
+  }
   float rb2 = 1.0f, rb3 = 1.0f, cjk = 1.0f;
   if (act && pml) { rb2 = 1.0f / cold->pB[1][j]; rb3 = 1.0f / cold->pB[2][k]; }
   if (act && cerjan) cjk = fminf(cold->cj[1][j + 3], cold->cj[2][k + 3]);
@@ -334,7 +350,7 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
       const float rrho = 1.0f / MET(12);
       const float l2m = lam + 2.0f * mu;
       const float rDH = a.rDH;
-      const float rb1 = pml ? 1.0f / cold->pB[0][i] : 1.0f;
+      const float rb1 = pml ? 1.0f / s_pBx[i] : 1.0f;
       const float(*L)[LK][LJ] = lds[b];
       const int lj = tj + HJL, lk = tk + HKL;
       // one-sided DRP difference of field c along each axis (macdrp.h:91-95),
@@ -365,8 +381,8 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
         float dzv[3];
         if (k == nk - 1) {  // DzV from the traction-free condition (wave.cu:207-232)
           const uint32_t ij = ((uint32_t)(j + 3) + (uint32_t)(i + 3) * (uint32_t)ny) * 9;
-          const float* mVx = cold->mVx;
-          const float* mVy = cold->mVy;
+          const float* mVx = s_mV[0];
+          const float* mVy = s_mV[1];
           const float dxv[3] = {rDH * Dx(0), rDH * Dx(1), rDH * Dx(2)}, dyv[3] = {rDH * Dy(0), rDH * Dy(1), rDH * Dy(2)};
 #pragma unroll
           for (int r = 0; r < 3; r++)
@@ -400,7 +416,7 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
       // themselves are advanced afterwards by pml_aux_kernel from the same
       // stage input.
       if (pml) {
-        const int N = cold->N;
+        const int N = s_N;
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
           const int li = ax == 0 ? i : ax == 1 ? j : k;
@@ -409,13 +425,13 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
           for (int side = 0; side < 2; side++) {
             const int f = 2 * ax + side;
             const int off = side ? nn - N : 0;
-            if (!cold->on[f] || li < off || li >= off + N) continue;
+            if (!s_on[f] || li < off || li >= off + N) continue;
             const uint32_t pi = ax == 0 ? i - off : i, pj = ax == 1 ? j - off : j, pk = ax == 2 ? k - off : k;
             const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
             const size_t ps = (size_t)pni * pnj * pnk;
             const uint32_t pa4 = (pj + pk * pnj + pi * pnj * pnk) << 2;
             const float rbb = ax == 0 ? rb1 : ax == 1 ? rb2 : rb3;  // 1 / beta at li
-            const float* Y = cold->auxY[f];
+            const float* Y = s_auxY[f];
 #pragma unroll
             for (int c = 0; c < 9; c++) h[c] += -ldo(Y + c * ps, pa4) * rbb;
           }
@@ -430,7 +446,7 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
 #pragma unroll
         for (int c = 0; c < 9; c++) bst(r, col4, c * pb, h[c]);
       } else {
-        const float damp = cerjan ? fminf(cold->cj[0][i + 3], cjk) : 1.0f;
+        const float damp = cerjan ? fminf(s_cjx[i + 3], cjk) : 1.0f;
         const Rsrc rs = plane_rsrc(a.S, i + 3, 9, nyz), rt = plane_rsrc(a.T, i + 3, 9, nyz);
         const Rsrc ro = plane_rsrc(a.Wout, i + 3, 9, nyz);
 #pragma unroll
