@@ -488,7 +488,17 @@ template <class MT>
 struct PmlAuxArgs {
   const float* __restrict__ Win;
   const MT* __restrict__ met;
-  const WaveCold* __restrict__ cold;
+  // the face's arrays, passed directly (kernel arguments are preloaded):
+  // auxiliary value, step start and RK accumulator, the axis's PML
+  // profiles, and the free-surface velocity map of x and y faces
+  float* __restrict__ Y;
+  float* __restrict__ S;
+  float* __restrict__ T;
+  const float* __restrict__ pA;
+  const float* __restrict__ pB;
+  const float* __restrict__ pD;
+  const float* __restrict__ mat;
+  int N;
   int f, ni, nj, nk, ny, py, jpad;
   unsigned pyz;
   int flag;  // one-sided operator sign along the face's axis
@@ -498,8 +508,7 @@ struct PmlAuxArgs {
 
 template <class MT>
 __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
-  const WaveCold* __restrict__ cold = a.cold;
-  const int f = a.f, ax = f >> 1, N = cold->N;
+  const int f = a.f, ax = f >> 1, N = a.N;
   const int nn = ax == 0 ? a.ni : ax == 1 ? a.nj : a.nk;
   const int off = (f & 1) ? nn - N : 0;
   const int pnj = ax == 1 ? N : a.nj, pnk = ax == 2 ? N : a.nk, pni = ax == 0 ? N : a.ni;
@@ -533,20 +542,20 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
   auto MET = [&](int m) { return mat_(M, m, x, (uint32_t)col, pyz); };
   const float e0 = MET(3 * ax), e1 = MET(3 * ax + 1), e2 = MET(3 * ax + 2);
   const float lam = MET(10), mu = MET(11), rrho = 1.0f / MET(12), l2m = lam + 2.0f * mu;
-  const float b1 = cold->pB[ax][li], d1 = cold->pD[ax][li], ad1 = -(cold->pA[ax][li] + d1);
+  const float b1 = a.pB[li], d1 = a.pD[li], ad1 = -(a.pA[li] + d1);
   const float c0 = d1 * b1;
   const bool fsx = ax < 2 && a.freenode && k == a.nk - 1;
   float z1[3] = {0.0f, 0.0f, 0.0f}, ztx = 0.0f, zty = 0.0f, ztz = 0.0f;
   if (fsx) {
-    const float* mat = ax == 0 ? cold->mVx : cold->mVy;
+    const float* mat = a.mat;
     const long ij = ((long)(j + 3) + (long)(i + 3) * a.ny) * 9;
 #pragma unroll
     for (int r = 0; r < 3; r++) z1[r] = mat[ij + 3 * r] * dd[0] + mat[ij + 3 * r + 1] * dd[1] + mat[ij + 3 * r + 2] * dd[2];
     ztx = MET(6); zty = MET(7); ztz = MET(8);
   }
-  float* Y = cold->auxY[f];
-  float* S = cold->auxS[f];
-  float* T = cold->auxT[f];
+  float* Y = a.Y;
+  float* S = a.S;
+  float* T = a.T;
 #pragma unroll
   for (int c = 0; c < 9; c++) {
     const long q = c * ps + pa;
@@ -619,7 +628,7 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   }
   if (a.pml && !a.rhs_only) {
     PmlAuxArgs<MT> p{};
-    p.Win = a.Win; p.met = a.met; p.cold = a.cold;
+    p.Win = a.Win; p.met = a.met; p.N = c.pml.N;
     p.ni = a.ni; p.nj = a.nj; p.nk = a.nk; p.ny = a.ny; p.py = a.py; p.jpad = a.jpad; p.pyz = a.nyz;
     p.rDH = a.rDH; p.rka = a.rka; p.rkb = a.rkb; p.irk = a.irk; p.freenode = a.freenode;
     const int flags[3] = {FX, FY, FZ};
@@ -628,6 +637,9 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
       const int ax = f / 2;
       const long n = (long)c.pml.N * (ax == 0 ? (long)a.nj * a.nk : ax == 1 ? (long)a.nk * a.ni : (long)a.ni * a.nj);
       p.f = f; p.flag = flags[ax];
+      p.Y = c.pml.Y[f]; p.S = c.pml.S[f]; p.T = c.pml.T[f];
+      p.pA = c.pml.A[ax]; p.pB = c.pml.B[ax]; p.pD = c.pml.D[ax];
+      p.mat = ax == 0 ? c.matVx2Vz : c.matVy2Vz;
       pr.begin(s, PK_PML_AUX);
       hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
       pr.end(s);
