@@ -766,7 +766,11 @@ int upload_wave_cold(Ctx& c) {
     lo[ax] = ax == 1 ? interior_lo_j(p) : std::max(c.pml.on[2 * ax] ? c.pml.N : 0, p.useCerjan ? p.DAMP_N : 0);
     hi[ax] = std::max(c.pml.on[2 * ax + 1] ? c.pml.N : 0, p.useCerjan && (ax < 2 || !freenode) ? p.DAMP_N : 0);
   }
-  if (freenode) hi[2] = std::max(hi[2], 3);
+  // WaveSurfaceTiles: the free-surface tiles are exactly the top TK rows
+  // (full tiles) and the interior's k remainder joins the low band instead;
+  // their j tiles are laid out on their own, line-aligned and full width
+  const bool surf = freenode && p.WaveSurfaceTiles != 0;
+  if (freenode) hi[2] = std::max(hi[2], surf ? TK : 3);
   // the interior box holds whole tiles along j and k: a remainder joins the
   // high-side band (or the free-surface tiles), whose kernels treat any point
   // (WaveTallInterior: interior tiles of 2 TK rows in blocks of 2 NTHR
@@ -775,16 +779,25 @@ int upload_wave_cold(Ctx& c) {
   c.wave_tall = tall;
   for (int ax = 1; ax < 3; ax++) {
     const int t = ax == 1 ? TJ : (tall ? 2 * TK : TK), e = n3[ax] - lo[ax] - hi[ax];
-    if (e > t) hi[ax] += e % t;
+    if (e > t) (ax == 2 && surf ? lo[ax] : hi[ax]) += e % t;
   }
   const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
   const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
   const std::vector<Seg> ski = tall ? axis_segments(n3[2], lo[2], hi[2], 2 * TK) : sk;  // interior tiles' rows
   const int nk = n3[2];
   auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
+  std::vector<Seg> sjt = sj;  // j tiles of the free-surface rows
+  if (surf) {
+    sjt.clear();
+    const int s0 = lo[1] - 32 * ((lo[1] + 31) / 32);  // j = s0 starts a 128-byte line
+    for (int x = s0; x < n3[1]; x += TJ) sjt.push_back({x, std::min(TJ, n3[1] - x), false, std::max(0, -x)});
+  }
   int tiles[3] = {0, 0, 0};
   for (const Seg& b : sk)
-    for (const Seg& a : sj) tiles[is_top(b) ? 2 : (a.in && b.in ? 0 : 1)]++;
+    for (const Seg& a : sj)
+      if (!is_top(b)) tiles[a.in && b.in ? 0 : 1]++;
+  for (const Seg& b : sk)
+    if (is_top(b)) tiles[2] += (int)sjt.size();
   int tiles_in = 0;  // interior blocks per chunk
   for (const Seg& b : ski)
     for (const Seg& a : sj) tiles_in += a.in && b.in;
@@ -840,8 +853,8 @@ int upload_wave_cold(Ctx& c) {
         if (narrow(a)) push(2, narrowed(a), Seg{k0, std::min(TKN, ktop - k0), false, 0}, i0, std::min(ni, i0 + ich_band));
   for (int i0 = 0; i0 < ni; i0 += ich_top)
     for (const Seg& b : sk)
-      for (const Seg& a : sj)
-        if (is_top(b)) push(3, a, b, i0, std::min(ni, i0 + ich_top));
+      if (is_top(b))
+        for (const Seg& a : sjt) push(3, a, b, i0, std::min(ni, i0 + ich_top));
   std::vector<int4> all;
   for (int l = 0; l < 4; l++) {
     all.insert(all.end(), lists[l].begin(), lists[l].end());
