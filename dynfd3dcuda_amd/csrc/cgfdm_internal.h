@@ -43,6 +43,7 @@ struct Params {
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
   int StageStreams = 0;  // 1: interior and free-surface launches of a stage on side streams
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
+  int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
   int WaveBandRound = 0;  // 1: low j band rounded up to whole 64-wide tiles (0: band width exactly)
