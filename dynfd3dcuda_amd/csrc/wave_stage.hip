@@ -488,34 +488,50 @@ template <class MT>
 struct PmlAuxArgs {
   const float* __restrict__ Win;
   const MT* __restrict__ met;
-  // the face's arrays, passed directly (kernel arguments are preloaded):
-  // auxiliary value, step start and RK accumulator, the axis's PML
-  // profiles, and the free-surface velocity map of x and y faces
-  float* __restrict__ Y;
-  float* __restrict__ S;
-  float* __restrict__ T;
-  const float* __restrict__ pA;
-  const float* __restrict__ pB;
-  const float* __restrict__ pD;
-  const float* __restrict__ mat;
+  // every face's arrays, passed directly (kernel arguments are preloaded):
+  // auxiliary value, step start and RK accumulator; per axis the PML
+  // profiles; the free-surface velocity maps of the x and y faces.  Picked
+  // with constant indices (a runtime index into the argument block would
+  // copy it to scratch memory).
+  float* Y[6];
+  float* S[6];
+  float* T[6];
+  const float* pA[3];
+  const float* pB[3];
+  const float* pD[3];
+  const float* mat[2];
+  int blk0[7];  // first block of each face (prefix sums; off faces have none)
+  int flag[3];  // one-sided operator sign along each axis
   int N;
-  int f, ni, nj, nk, ny, py, jpad;
+  int ni, nj, nk, ny, py, jpad;
   unsigned pyz;
-  int flag;  // one-sided operator sign along the face's axis
   float rDH, rka, rkb;
   int irk, freenode;
 };
 
+template <class T>
+__device__ __forceinline__ T pick6(int f, const T (&v)[6]) {
+  return f == 0 ? v[0] : f == 1 ? v[1] : f == 2 ? v[2] : f == 3 ? v[3] : f == 4 ? v[4] : v[5];
+}
+template <class T>
+__device__ __forceinline__ T pick3(int x, const T (&v)[3]) { return x == 0 ? v[0] : x == 1 ? v[1] : v[2]; }
+
+// One launch for all absorbing faces: blocks [blk0[f], blk0[f+1]) serve
+// face f, numbered XCD-contiguously so that the blocks sharing stencil
+// planes (neighbouring band planes on x faces, rows on z faces) share one L2.
 template <class MT>
 __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
-  const int f = a.f, ax = f >> 1, N = a.N;
+  const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  int f = 0, fb0 = 0;
+#pragma unroll
+  for (int q = 1; q < 6; q++)
+    if (b >= a.blk0[q]) { f = q; fb0 = a.blk0[q]; }
+  const int ax = f >> 1, N = a.N, flag = pick3(ax, a.flag);
   const int nn = ax == 0 ? a.ni : ax == 1 ? a.nj : a.nk;
   const int off = (f & 1) ? nn - N : 0;
   const int pnj = ax == 1 ? N : a.nj, pnk = ax == 2 ? N : a.nk, pni = ax == 0 ? N : a.ni;
   const long ps = (long)pni * pnj * pnk;
-  // blocks numbered XCD-contiguously: the blocks sharing stencil planes
-  // (neighbouring band planes on x faces, rows on z faces) share one L2
-  const long idx = (long)xcd_remap((int)blockIdx.x, (int)gridDim.x) * 256L + threadIdx.x;
+  const long idx = (long)(b - fb0) * 256L + threadIdx.x;
   if (idx >= ps) return;
   const int pj = (int)(idx % pnj);
   const long t = idx / pnj;
@@ -535,27 +551,30 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
 #pragma unroll
   for (int c = 0; c < 9; c++) {
     const float* w = W + base + (long)c * pyz;
-    dd[c] = a.rDH * (a.flag > 0 ? drp5<1>(w[-st], w[0], w[st], w[2 * st], w[3 * st])
+    dd[c] = a.rDH * (flag > 0 ? drp5<1>(w[-st], w[0], w[st], w[2 * st], w[3 * st])
                                 : drp5<-1>(w[-3 * st], w[-2 * st], w[-st], w[0], w[st]));
   }
   const MT* M = a.met;
   auto MET = [&](int m) { return mat_(M, m, x, (uint32_t)col, pyz); };
   const float e0 = MET(3 * ax), e1 = MET(3 * ax + 1), e2 = MET(3 * ax + 2);
   const float lam = MET(10), mu = MET(11), rrho = 1.0f / MET(12), l2m = lam + 2.0f * mu;
-  const float b1 = a.pB[li], d1 = a.pD[li], ad1 = -(a.pA[li] + d1);
+  const float* pAx = pick3(ax, a.pA);
+  const float* pBx = pick3(ax, a.pB);
+  const float* pDx = pick3(ax, a.pD);
+  const float b1 = pBx[li], d1 = pDx[li], ad1 = -(pAx[li] + d1);
   const float c0 = d1 * b1;
   const bool fsx = ax < 2 && a.freenode && k == a.nk - 1;
   float z1[3] = {0.0f, 0.0f, 0.0f}, ztx = 0.0f, zty = 0.0f, ztz = 0.0f;
   if (fsx) {
-    const float* mat = a.mat;
+    const float* mat = ax == 0 ? a.mat[0] : a.mat[1];
     const long ij = ((long)(j + 3) + (long)(i + 3) * a.ny) * 9;
 #pragma unroll
     for (int r = 0; r < 3; r++) z1[r] = mat[ij + 3 * r] * dd[0] + mat[ij + 3 * r + 1] * dd[1] + mat[ij + 3 * r + 2] * dd[2];
     ztx = MET(6); zty = MET(7); ztz = MET(8);
   }
-  float* Y = a.Y;
-  float* S = a.S;
-  float* T = a.T;
+  float* Y = pick6(f, a.Y);
+  float* S = pick6(f, a.S);
+  float* T = pick6(f, a.T);
 #pragma unroll
   for (int c = 0; c < 9; c++) {
     const long q = c * ps + pa;
@@ -631,17 +650,22 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     p.Win = a.Win; p.met = a.met; p.N = c.pml.N;
     p.ni = a.ni; p.nj = a.nj; p.nk = a.nk; p.ny = a.ny; p.py = a.py; p.jpad = a.jpad; p.pyz = a.nyz;
     p.rDH = a.rDH; p.rka = a.rka; p.rkb = a.rkb; p.irk = a.irk; p.freenode = a.freenode;
-    const int flags[3] = {FX, FY, FZ};
+    p.flag[0] = FX; p.flag[1] = FY; p.flag[2] = FZ;
+    for (int ax = 0; ax < 3; ax++) { p.pA[ax] = c.pml.A[ax]; p.pB[ax] = c.pml.B[ax]; p.pD[ax] = c.pml.D[ax]; }
+    p.mat[0] = c.matVx2Vz; p.mat[1] = c.matVy2Vz;
+    int blocks = 0;
     for (int f = 0; f < 6; f++) {
+      p.blk0[f] = blocks;
+      p.Y[f] = c.pml.Y[f]; p.S[f] = c.pml.S[f]; p.T[f] = c.pml.T[f];
       if (!c.pml.on[f]) continue;
       const int ax = f / 2;
       const long n = (long)c.pml.N * (ax == 0 ? (long)a.nj * a.nk : ax == 1 ? (long)a.nk * a.ni : (long)a.ni * a.nj);
-      p.f = f; p.flag = flags[ax];
-      p.Y = c.pml.Y[f]; p.S = c.pml.S[f]; p.T = c.pml.T[f];
-      p.pA = c.pml.A[ax]; p.pB = c.pml.B[ax]; p.pD = c.pml.D[ax];
-      p.mat = ax == 0 ? c.matVx2Vz : c.matVy2Vz;
+      blocks += (int)((n + 255) / 256);
+    }
+    p.blk0[6] = blocks;
+    if (blocks > 0) {
       pr.begin(s, PK_PML_AUX);
-      hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p);
+      hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)blocks), dim3(256), 0, s, p);
       pr.end(s);
     }
   }
