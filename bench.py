@@ -200,6 +200,15 @@ def mixed_leg(case, steps, warmup):
             "B_alg_bytes_per_pt_step": ba, "step_hbm_roofline_frac": v * ba / (PEAK_HBM_GBS * 1e9)}
 
 
+def overrides(args):
+    """params.json keys given with --set KEY=VALUE."""
+    out = {}
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        out[k] = float(v) if any(ch in v for ch in ".eE") else int(v)
+    return out
+
+
 def rate_state_leg(args):
     """The rate-state solve three ways on the rate-state configuration: the
     reference's stopping rule (NewtonMode 0: |dx|/|x| < 1e-12, at most 100
@@ -216,6 +225,7 @@ def rate_state_leg(args):
     for mode in (0, 1, 2):
         case = configs.curved_rs_400x300x200(nt=lead + window + 2)
         case["params"].update({"NewtonMode": mode, "NewtonTol": 1e-10, "NewtonFallbackIter": 20})
+        case["params"].update(overrides(args))
         s = dyn.Solver(case["params"])
         configs.apply(case, s)
         if mode == 2:
@@ -312,9 +322,7 @@ def main():
         kw["seed"] = args.seed
     case = gen(**kw)
     p = case["params"]
-    for kv in args.set:
-        k, v = kv.split("=", 1)
-        p[k] = float(v) if any(ch in v for ch in ".eE") else int(v)
+    p.update(overrides(args))
     note = None
     if p.get("NewtonMode") == 2 and not p.get("NewtonMLP"):
         # no network given: the step runs with plain Newton (same residual rule)

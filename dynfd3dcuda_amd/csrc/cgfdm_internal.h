@@ -38,6 +38,7 @@ struct Params {
   int MetricBF16 = 0;   // store metric + media in bfloat16 (FP32 compute)
   int NewtonMode = 0;   // 0: reference NRsearch; 1: residual tolerance + plain fallback
   double NewtonTol = 1e-10;
+  double NewtonMLPGate = 2.0;  // learned guess consulted where the plain guess's relative residual exceeds this
   int NewtonFallbackIter = 20;
   std::string NewtonMLP; // path of trained MLP weights for the Newton guess
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)

@@ -45,7 +45,7 @@ struct FArgs {
   float mu_d, viscosity, smooth_load_T, Vini;
   double f0, V0, fw;
   int newton_mode, newton_fallback;
-  double newton_tol;
+  double newton_tol, mlp_gate;
 };
 
 __device__ __forceinline__ float LM(const FArgs& a, long i) {
@@ -419,11 +419,11 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
     // 2 V0 e^(-psi/a) sinh(x_plain) is the previous slip rate Rn, so the
     // plain guess's residual costs no transcendental.  The network is
     // consulted by the waves in which some node's plain guess is poor
-    // (relative residual above kMlpGate); the vote is wave-uniform, so all
+    // (relative residual above the gate, NewtonMLPGate); the vote is wave-uniform, so all
     // 64 lanes enter the matrix instructions together.
     const double Rn = C[9 * n2 + q];
     const double f_pln = Rn + coef * x_plain - Vtn;
-    const bool poor = valid && fabs(f_pln) > kMlpGate * fmax(Vtn, 1e-300);
+    const bool poor = valid && fabs(f_pln) > a.mlp_gate * fmax(Vtn, 1e-300);
     if (__any(poor)) {
       float feat[4];
       for (int f = 0; f < 4; f++) feat[f] = valid ? Cf[(4 + f) * n2 + q] : 0.0f;
@@ -997,6 +997,7 @@ FArgs make_fargs(Ctx& c) {
   a.Vini = (float)c.p.Vini;
   a.f0 = c.p.f0; a.V0 = c.p.V0; a.fw = c.p.fw;
   a.newton_mode = c.p.NewtonMode; a.newton_tol = c.p.NewtonTol; a.newton_fallback = c.p.NewtonFallbackIter;
+  a.mlp_gate = c.p.NewtonMLPGate;
   return a;
 }
 

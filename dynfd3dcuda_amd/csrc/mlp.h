@@ -146,9 +146,6 @@ __device__ inline float mlp_guess_wave(const MlpDev* __restrict__ m, const float
   return (y + m->b3) * m->yscale + m->ymean;
 }
 
-// Relative residual of the plain guess above which a node asks the network.
-constexpr double kMlpGate = 0.05;
-
 // Start of the Newton solve: the learned guess (log10 slip rate lv) or the
 // plain one (residual f_pln), whichever has the smaller residual |f| of
 // f(x) = 2 V0 e^(-psi/a) sinh x + coef x - Vt.  x of the learned guess is
