@@ -228,17 +228,18 @@ int cgfdm_destroy(cgfdm_ctx* h) {
   if (!h) return 0;
   Ctx& c = h->c;
   if (c.out) { cgfdm_close_outputs(h); }
-  if (c.setup_done) hipSetDevice(c.device);
-  if (c.stream) hipStreamSynchronize(c.stream);
-  if (c.own_stream) { hipStreamSynchronize(c.own_stream); hipStreamDestroy(c.own_stream); }
+  // teardown is best effort: each call runs whatever an earlier one returned
+  if (c.setup_done) (void)hipSetDevice(c.device);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  if (c.own_stream) { (void)hipStreamSynchronize(c.own_stream); (void)hipStreamDestroy(c.own_stream); }
   for (hipStream_t st : c.side)
-    if (st) { hipStreamSynchronize(st); hipStreamDestroy(st); }
+    if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
   for (hipEvent_t e : c.fence)
-    if (e) hipEventDestroy(e);
-  for (void* p : c.allocations) hipFree(p);
-  if (c.hW) hipFree(c.hW);
-  if (c.wave_work) hipFree(c.wave_work);
-  if (c.mlp_dev) hipFree(c.mlp_dev);
+    if (e) (void)hipEventDestroy(e);
+  for (void* p : c.allocations) (void)hipFree(p);
+  if (c.hW) (void)hipFree(c.hW);
+  if (c.wave_work) (void)hipFree(c.wave_work);
+  if (c.mlp_dev) (void)hipFree(c.mlp_dev);
   delete h;
   return 0;
 }
@@ -297,7 +298,7 @@ int cgfdm_setup(cgfdm_ctx* h) {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CGFDM_EHIP, "no HIP device");
   c.device = ndev > 1 ? c.p.igpu % ndev : 0;
-  hipSetDevice(c.device);
+  if (hipError_t e = hipSetDevice(c.device)) return fail(CGFDM_EHIP, hipGetErrorString(e));
   std::string err;
   int rc = host_setup(c, err);
   if (rc) {
@@ -520,7 +521,7 @@ int cgfdm_get_field(cgfdm_ctx* h, const char* name, void* dst, size_t bytes) {
   FieldRef r = field(c, name);
   if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
   if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
-  hipStreamSynchronize(c.stream);
+  if (hipError_t e = hipStreamSynchronize(c.stream)) return fail(CGFDM_EHIP, hipGetErrorString(e));
   if (!r.nf) {
     if (hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
     return 0;
@@ -542,7 +543,7 @@ int cgfdm_put_field(cgfdm_ctx* h, const char* name, const void* src, size_t byte
   FieldRef r = field(c, name);
   if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
   if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
-  hipStreamSynchronize(c.stream);
+  if (hipError_t e = hipStreamSynchronize(c.stream)) return fail(CGFDM_EHIP, hipGetErrorString(e));
   if (!r.nf) {
     if (hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
     return 0;
@@ -567,7 +568,7 @@ int cgfdm_get_dims(cgfdm_ctx* h, int dims[8]) {
 int cgfdm_profile(cgfdm_ctx* h, int enable) {
   if (!h) return fail(CGFDM_EPARAM, "null context");
   Ctx& c = h->c;
-  if (c.stream) hipStreamSynchronize(c.stream);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);  // a failure shows in the event times read next
   c.prof.reset();
   c.prof.on = enable != 0;
   if (c.prof.on) c.prof.grow(8192);
@@ -581,7 +582,7 @@ int cgfdm_profile_get(cgfdm_ctx* h, const char* name, double* total_ms, long* la
   for (int q = 0; q < PK_COUNT; q++)
     if (std::strcmp(name, kProfNames[q]) == 0) k = q;
   if (k < 0) return fail(CGFDM_EPARAM, std::string("unknown kernel class ") + name);
-  if (c.stream) hipStreamSynchronize(c.stream);
+  if (c.stream) (void)hipStreamSynchronize(c.stream);  // a failure shows in the event times read next
   double ms = 0.0;
   long n = 0;
   for (const auto& m : c.prof.marks) {
@@ -637,7 +638,7 @@ int cgfdm_newton_stats(cgfdm_ctx* h, double* solves, double* iterations, double*
   if (c.setup_done && c.p.Rupture && c.f.newton_stats) {
     const long n2 = (long)c.d.nj * c.d.nk;
     std::vector<unsigned int> v(3 * n2);
-    hipStreamSynchronize(c.stream);
+    if (hipError_t e = hipStreamSynchronize(c.stream)) return fail(CGFDM_EHIP, hipGetErrorString(e));
     if (hipMemcpy(v.data(), c.f.newton_stats, v.size() * sizeof(unsigned int), hipMemcpyDeviceToHost) != hipSuccess)
       return fail(CGFDM_EHIP, "copy failed");
     for (int t = 0; t < 3; t++)
@@ -683,13 +684,16 @@ int cgfdm_open_outputs(cgfdm_ctx* h) {
         for (int j = 0; j < nj; j++) buf[j + (long)k * nj] = (float)C[comp * nxyz + cpos(c.d.i0, j + 3, k + 3)];
       f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, buf.data());
     }
-    auto put2 = [&](const char* fld, int vid) {
+    auto put2 = [&](const char* fld, int vid) -> int {
       FieldRef r = field(c, fld);
-      hipMemcpy(buf.data(), r.ptr, buf.size() * 4, hipMemcpyDeviceToHost);
+      if (hipMemcpy(buf.data(), r.ptr, buf.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
       f.put_var(vid, buf.data());
+      return 0;
     };
-    put2("C0", vc0); put2("str_peak", vpk); put2("a", va); put2("b", vb); put2("Vw", vw);
-    hipMemcpy(ib.data(), c.f.united, ib.size() * 4, hipMemcpyDeviceToHost);
+    const std::pair<const char*, int> statics[] = {{"C0", vc0}, {"str_peak", vpk}, {"a", va}, {"b", vb}, {"Vw", vw}};
+    for (const auto& fv : statics)
+      if (int rc = put2(fv.first, fv.second)) return rc;
+    if (hipMemcpy(ib.data(), c.f.united, ib.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
     f.put_var(vun, ib.data());
   }
   // wave slices: xy at global k, xz at global j, yz at global i
@@ -730,7 +734,14 @@ int cgfdm_open_outputs(cgfdm_ctx* h) {
   return 0;
 }
 
+// Device-to-host copy that keeps the first failure in *e (the output
+// writer reports it once the records of the call are written).
+static void d2h(void* dst, const void* src, size_t bytes, hipError_t* e) {
+  if (*e == hipSuccess) *e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
 int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
+  hipError_t herr = hipSuccess;  // first failed copy, reported after the records
   if (!h) return fail(CGFDM_EPARAM, "null context");
   Ctx& c = h->c;
   if (!c.out) {
@@ -741,7 +752,7 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
   if (it % skip) return 0;
   long rec = it / skip;
   NcOut* o = c.out;
-  hipStreamSynchronize(c.stream);
+  if (hipError_t e = hipStreamSynchronize(c.stream)) return fail(CGFDM_EHIP, hipGetErrorString(e));
   const int nj = c.d.nj, nk = c.d.nk, ni = c.d.ni, ny = c.d.ny;
   const long nxyz = c.d.nxyz, nyz = c.d.nyz, pyz = c.d.pyz;
   const int py = c.d.py, jpad = c.d.jpad;
@@ -755,27 +766,27 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
     for (int v = 0; v < 7; v++) {
       FieldRef r = field(c, names[v]);
       if (r.dtype == 1) {
-        hipMemcpy(db.data(), r.ptr, n2 * 8, hipMemcpyDeviceToHost);
+        d2h(db.data(), r.ptr, n2 * 8, &herr);
         for (long q = 0; q < n2; q++) buf[q] = (float)db[q];
       } else {
-        hipMemcpy(buf.data(), r.ptr, n2 * 4, hipMemcpyDeviceToHost);
+        d2h(buf.data(), r.ptr, n2 * 4, &herr);
       }
       o->fault.put_record(o->fv[v], rec, buf.data());
     }
-    hipMemcpy(buf.data(), c.f.init_t0, n2 * 4, hipMemcpyDeviceToHost);
+    d2h(buf.data(), c.f.init_t0, n2 * 4, &herr);
     o->fault.put_var(o->f_init_t0, buf.data());
-    hipMemcpy(ib.data(), c.f.rup_index_y, n2 * 4, hipMemcpyDeviceToHost);
+    d2h(ib.data(), c.f.rup_index_y, n2 * 4, &herr);
     o->fault.put_record(o->fv[7], rec, ib.data());
-    hipMemcpy(ib.data(), c.f.rup_index_z, n2 * 4, hipMemcpyDeviceToHost);
+    d2h(ib.data(), c.f.rup_index_z, n2 * 4, &herr);
     o->fault.put_record(o->fv[8], rec, ib.data());
-    hipMemcpy(buf.data(), c.f.rup_sensor, n2 * 4, hipMemcpyDeviceToHost);
+    d2h(buf.data(), c.f.rup_sensor, n2 * 4, &herr);
     o->fault.put_record(o->fv[9], rec, buf.data());
     // thermal pressurization: the rises on the fault (level 0) as the
     // reference writes them (io_fault.cu:262-266); zeros without it
     for (int v = 0; v < 2; v++) {
       const double* src = v ? c.f.TP_P : c.f.TP_T;
       if (src) {
-        hipMemcpy(db.data(), src, n2 * 8, hipMemcpyDeviceToHost);
+        d2h(db.data(), src, n2 * 8, &herr);
         for (long q = 0; q < n2; q++) buf[q] = (float)db[q];
       } else {
         std::fill(buf.begin(), buf.end(), 0.0f);
@@ -792,18 +803,18 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
       if (axis == 2) {  // xy: rows of ny for each i
         line.resize(ny);
         for (int a = 0; a < d1n; a++) {
-          hipMemcpy(line.data(), c.cur + ((long)(a + 3) * 9 + comp) * pyz + jpad + (long)(idx + 3) * py, ny * 4, hipMemcpyDeviceToHost);
+          d2h(line.data(), c.cur + ((long)(a + 3) * 9 + comp) * pyz + jpad + (long)(idx + 3) * py, ny * 4, &herr);
           std::memcpy(&buf[(long)a * d2n], &line[3], d2n * 4);
         }
       } else if (axis == 1) {  // xz
         std::vector<float> plane(pyz);
         for (int a = 0; a < d1n; a++) {
-          hipMemcpy(plane.data(), c.cur + ((long)(a + 3) * 9 + comp) * pyz, pyz * 4, hipMemcpyDeviceToHost);
+          d2h(plane.data(), c.cur + ((long)(a + 3) * 9 + comp) * pyz, pyz * 4, &herr);
           for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[jpad + (idx + 3) + (long)(b + 3) * py];
         }
       } else {  // yz plane at i
         std::vector<float> plane(pyz);
-        hipMemcpy(plane.data(), c.cur + ((long)(idx + 3) * 9 + comp) * pyz, pyz * 4, hipMemcpyDeviceToHost);
+        d2h(plane.data(), c.cur + ((long)(idx + 3) * 9 + comp) * pyz, pyz * 4, &herr);
         for (int a = 0; a < d1n; a++)
           for (int b = 0; b < d2n; b++) buf[(long)a * d2n + b] = plane[jpad + (b + 3) + (long)(a + 3) * py];
       }
@@ -817,6 +828,7 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
   put_slice(o->xy, o->wxy, 2, kz, ni, nj);
   put_slice(o->xz, o->wxz, 1, jy, ni, nk);
   put_slice(o->yz, o->wyz, 0, ix, nk, nj);
+  if (herr != hipSuccess) return fail(CGFDM_EHIP, std::string("output copy failed: ") + hipGetErrorString(herr));
   return 0;
 }
 

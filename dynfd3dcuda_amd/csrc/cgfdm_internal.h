@@ -130,12 +130,12 @@ struct Profiler {
   void begin(hipStream_t s, int k) {
     if (!on) return;
     if (used + 2 > pool.size()) grow(used + 2048);
-    hipEventRecord(pool[used], s);
+    (void)hipEventRecord(pool[used], s);  // timing is best effort
     marks.push_back({k, used});
     used += 2;
   }
   void end(hipStream_t s) {
-    if (on && !marks.empty()) hipEventRecord(pool[marks.back().second + 1], s);
+    if (on && !marks.empty()) (void)hipEventRecord(pool[marks.back().second + 1], s);
   }
   void grow(size_t n) {
     while (pool.size() < n) {
@@ -145,7 +145,7 @@ struct Profiler {
     }
   }
   void reset() { marks.clear(); used = 0; }
-  ~Profiler() { for (hipEvent_t e : pool) hipEventDestroy(e); }
+  ~Profiler() { for (hipEvent_t e : pool) (void)hipEventDestroy(e); }
 };
 
 struct Ctx {

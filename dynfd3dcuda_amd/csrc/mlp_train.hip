@@ -191,7 +191,7 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   ~DevBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
   template <class T>
   T* alloc(size_t n) {
@@ -202,9 +202,11 @@ struct DevBuf {
 
 struct Stream {
   hipStream_t s = nullptr;
-  Stream() { hipStreamCreateWithFlags(&s, hipStreamNonBlocking); }
+  Stream() {  // s stays null on failure, which the callers check
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+  }
   ~Stream() {
-    if (s) hipStreamDestroy(s);
+    if (s) (void)hipStreamDestroy(s);
   }
 };
 

@@ -166,7 +166,10 @@ template <class T>
 T* dmalloc(Ctx& c, size_t n) {
   void* p = nullptr;
   if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
-  hipMemset(p, 0, n * sizeof(T));
+  if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) {
+    (void)hipFree(p);
+    return nullptr;
+  }
   c.allocations.push_back(p);
   return (T*)p;
 }
@@ -386,7 +389,7 @@ int host_setup(Ctx& c, std::string& err) {
       std::memcpy(&vpm, &bits, sizeof vpm);
       c.vpmax = vpm;
     }
-    for (void* q : tmp) hipFree(q);
+    for (void* q : tmp) (void)hipFree(q);
     if (!ok) { err = "metric computation on the device failed"; return 6; }
     c.met = met;
   }

@@ -599,15 +599,16 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   // on the solver's stream; the auxiliary update waits for the surface tiles
   // (which read the aux fields) and the caller's next launch for the interior.
   // While kernel classes are timed the launches stay serial, so that each
-  // class's events bracket its own kernels only.
+  // class's events bracket its own kernels only.  A failing event call sets
+  // HIP's last error, which the caller's launch check reports.
   const bool par = c.side[0] != nullptr && !c.prof.on;
   hipStream_t si = s, ss = s;
   if (par) {
-    hipEventRecord(c.fence[0], s);
+    (void)hipEventRecord(c.fence[0], s);
     si = c.side[0];
     ss = c.side[1];
-    hipStreamWaitEvent(si, c.fence[0], 0);
-    hipStreamWaitEvent(ss, c.fence[0], 0);
+    (void)hipStreamWaitEvent(si, c.fence[0], 0);
+    (void)hipStreamWaitEvent(ss, c.fence[0], 0);
   }
   if (n[0] > 0) {
     a.work = w; a.nwork = n[0];
@@ -642,8 +643,8 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     pr.end(s);
   }
   if (par) {
-    hipEventRecord(c.fence[2], ss);
-    hipStreamWaitEvent(s, c.fence[2], 0);
+    (void)hipEventRecord(c.fence[2], ss);
+    (void)hipStreamWaitEvent(s, c.fence[2], 0);
   }
   if (a.pml && !a.rhs_only) {
     PmlAuxArgs<MT> p{};
@@ -670,8 +671,8 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     }
   }
   if (par) {
-    hipEventRecord(c.fence[1], si);
-    hipStreamWaitEvent(s, c.fence[1], 0);
+    (void)hipEventRecord(c.fence[1], si);
+    (void)hipStreamWaitEvent(s, c.fence[1], 0);
   }
 }
 
@@ -888,7 +889,7 @@ int upload_wave_cold(Ctx& c) {
     all.insert(all.end(), lists[l].begin(), lists[l].end());
     c.wave_nkind[l] = (int)lists[l].size() / 2;
   }
-  if (c.wave_work) { hipFree(c.wave_work); c.wave_work = nullptr; }
+  if (c.wave_work) { (void)hipFree(c.wave_work); c.wave_work = nullptr; }
   if (hipMalloc(&c.wave_work, all.size() * sizeof(int4)) != hipSuccess) return 1;
   if (hipMemcpy(c.wave_work, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess) return 1;
   c.wave_nwork = (int)all.size() / 2;
