@@ -24,17 +24,23 @@ dyn = pytest.importorskip("dynfd3dcuda_amd")
 FIELDS = ("W", "FW", "Vs1", "Vs2", "Ts1", "Ts2", "Tn", "slip", "State", "init_t0", "rup_sensor")
 
 
-def step_by_phases(s, it, smooth):
+def step_by_phases(s, it, smooth, tp=False):
     s.rup_sensor()
+    if tp:
+        s.thermpress()
     for irk in range(4):
         s.stage_by_phases(it, irk)
     if smooth:
         s.fault_smooth()
 
 
-@pytest.mark.parametrize("kind", ["sw_smooth", "rs"])
+@pytest.mark.parametrize("kind", ["sw_smooth", "rs", "tp"])
 def test_phases_compose_to_step(kind):
-    case = cases.sw_case(curved=True, FaultSmooth=1) if kind == "sw_smooth" else cases.rs_case()
+    if kind == "tp":
+        from test_gpu_parity import tp_case
+        case = tp_case()
+    else:
+        case = cases.sw_case(curved=True, FaultSmooth=1) if kind == "sw_smooth" else cases.rs_case()
     a = dyn.Solver(case["params"])
     b = dyn.Solver(case["params"])
     for s in (a, b):
@@ -43,12 +49,15 @@ def test_phases_compose_to_step(kind):
     smooth = kind == "sw_smooth"
     for it in range(12):
         a.step(it)
-        step_by_phases(b, it, smooth)
+        step_by_phases(b, it, smooth, tp=kind == "tp")
     a.synchronize()
     b.synchronize()
     assert np.max(np.abs(a.get("Vs1"))) > 0
     for name in FIELDS:
         np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
+    if kind == "tp":
+        for name in ("TP_T", "TP_P"):
+            np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
 
 
 def test_phase_boundaries_match_oracle():
