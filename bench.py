@@ -12,8 +12,11 @@ Default workload: the curved-grid slip-weakening configuration, 400 x 300 x
 200 points at 50 m (topography, bent fault, 3-layer medium, 20-layer ADE-PML,
 FP32).  Prints one JSON line:
   value          grid-point updates per second (pt-steps/s) of the whole step
-  roofline       the dominant stage kernel, timed with HIP events on the
-                 solver's stream; achieved = algorithmic bytes per launch
+  roofline       the dominant stage kernel (the stage-kernel class with the
+                 largest GPU time in an untimed window of PROFILE_STEPS steps
+                 with every class timed), timed with HIP events on the
+                 solver's stream around its own launches over the timed
+                 region; achieved = algorithmic bytes per launch
                  (points x B_alg/4, B_alg = 768 B/pt-step FP32, 672 B with
                  BF16 metric) / average launch time; traffic = HBM bytes per
                  launch from the committed rocprofv3 FETCH/WRITE summary when
@@ -23,8 +26,11 @@ FP32).  Prints one JSON line:
   mixed          the same workload with the metric and media stored in BF16
                  (FP32 arithmetic): throughput and the step's fraction of the
                  672 B/pt-step roofline
-  rate_state     rate-and-state fault step with plain Newton against the
-                 learned initial guess, same grid with the aging law: the
+  kernel_ms_per_step  every kernel class's GPU time per step, from the
+                 untimed window
+  rate_state     rate-and-state fault step: the reference's Newton rule,
+                 plain Newton and the learned initial guess, same grid with
+                 the aging law: the
                  plain run's Newton samples train the network on the GPU,
                  then both runs are timed over the same window of steps
                  inside the rupture (GPU time per RK stage of the Newton
@@ -43,6 +49,7 @@ if ROOT not in sys.path:
 
 METRIC = "grid-pt updates/s + HBM roofline %; RS fault step time MLP-Newton vs plain"
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
+PROFILE_STEPS = 5      # untimed steps with every kernel class timed (the per-class breakdown)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_{workload}.txt")
 
 
@@ -317,7 +324,7 @@ def main():
     if args.workload not in configs.WORKLOADS:
         ap.error(f"unknown workload {args.workload}; choose from {', '.join(configs.WORKLOADS)}")
     gen = configs.WORKLOADS[args.workload]
-    kw = {"nt": args.steps + args.warmup + 2}
+    kw = {"nt": args.steps + args.warmup + PROFILE_STEPS + 2}
     if args.seed is not None:
         kw["seed"] = args.seed
     case = gen(**kw)
@@ -339,20 +346,35 @@ def main():
     for it in range(args.warmup):
         solver.step(it)
     solver.synchronize()
+    # an untimed window with every kernel class timed: the per-class
+    # breakdown, and the stage-kernel class with the largest GPU time
+    nprof = PROFILE_STEPS
+    it0 = args.warmup
+    solver.profile(True)
+    for it in range(it0, it0 + nprof):
+        solver.step(it)
+    solver.synchronize()
+    prof_all = solver.profile_report()
+    solver.profile(False)
+    stage = {k: v for k, v in prof_all.items() if k.startswith("stage_") and v[1] > 0}
+    dom = max(stage, key=lambda k: stage[k][0])
+    # the timed region: HIP events around that class's launches only (events
+    # around every launch would add their own packets to each step)
+    it0 += nprof
+    solver.profile_classes([dom])
     solver.profile(True)
     t0 = time.perf_counter()
-    for it in range(args.warmup, args.warmup + args.steps):
+    for it in range(it0, it0 + args.steps):
         solver.step(it)
     solver.synchronize()
     t = time.perf_counter() - t0
     prof = solver.profile_report()
     solver.profile(False)
+    solver.profile_classes(None)
 
     value = npts * args.steps / t
     ba = b_alg(p)
-    stage = {k: v for k, v in prof.items() if k.startswith("stage_") and v[1] > 0}
-    dom = max(stage, key=lambda k: stage[k][0])
-    ms, n, pts = stage[dom]
+    ms, n, pts = prof[dom]
     avg_s = ms / n / 1e3
     bytes_per_launch = pts * ba / 4.0  # one launch = one RK stage of its points
     achieved = bytes_per_launch / avg_s / 1e9
@@ -369,7 +391,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "points_per_launch": pts, "avg_launch_ms": avg_s * 1e3},
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1] > 0},
+        "kernel_ms_per_step": {k: v[0] / nprof for k, v in prof_all.items() if v[1] > 0},
         "cpu_baseline": None,
         "mixed": None,
         "rate_state": None,

@@ -32,7 +32,7 @@ EXPORTED = [
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
     "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
     "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
-    "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress",
+    "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress", "cgfdm_profile_classes",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
@@ -109,6 +109,7 @@ def load_library(path=None):
     lib.cgfdm_rup_sensor.argtypes = [c_p]
     lib.cgfdm_fault_smooth.argtypes = [c_p]
     lib.cgfdm_thermpress.argtypes = [c_p]
+    lib.cgfdm_profile_classes.argtypes = [c_p, ctypes.c_char_p]
     for name in EXPORTED[1:]:
         getattr(lib, name).restype = ctypes.c_int
     if path is None:
@@ -306,6 +307,10 @@ class Solver:
     def profile(self, enable=True):
         """Start (clearing earlier records) or stop per-kernel GPU timing."""
         self._check(self.lib.cgfdm_profile(self.h, 1 if enable else 0))
+
+    def profile_classes(self, names=None):
+        """Time only these kernel classes (None: all)."""
+        self._check(self.lib.cgfdm_profile_classes(self.h, ",".join(names).encode() if names else None))
 
     def profile_report(self):
         """{kernel class: (total ms, launches, grid points per launch)} since profile(True)."""

@@ -575,6 +575,29 @@ int cgfdm_profile(cgfdm_ctx* h, int enable) {
   return check_hip("profile");
 }
 
+int cgfdm_profile_classes(cgfdm_ctx* h, const char* names) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  unsigned mask = 0;
+  if (!names || !*names) {
+    mask = ~0u;
+  } else {
+    std::string list(names);
+    size_t pos = 0;
+    while (pos <= list.size()) {
+      const size_t e = std::min(list.find(',', pos), list.size());
+      const std::string nm = list.substr(pos, e - pos);
+      int k = -1;
+      for (int q = 0; q < PK_COUNT; q++)
+        if (nm == kProfNames[q]) k = q;
+      if (k < 0) return fail(CGFDM_EPARAM, "unknown kernel class " + nm);
+      mask |= 1u << k;
+      pos = e + 1;
+    }
+  }
+  h->c.prof.mask = mask;
+  return 0;
+}
+
 int cgfdm_profile_get(cgfdm_ctx* h, const char* name, double* total_ms, long* launches, long* points) {
   if (!h || !name) return fail(CGFDM_EPARAM, "null argument");
   Ctx& c = h->c;

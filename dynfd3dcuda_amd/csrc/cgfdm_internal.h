@@ -124,18 +124,22 @@ enum ProfKernel {
 extern const char* const kProfNames[PK_COUNT];
 struct Profiler {
   bool on = false;
+  unsigned mask = ~0u;  // kernel classes timed while on (bit k: class k)
+  bool open = false;    // the last begin() recorded
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<int, size_t>> marks;  // (kernel class, index of its start event)
   size_t used = 0;
   void begin(hipStream_t s, int k) {
-    if (!on) return;
+    open = on && ((mask >> k) & 1u);
+    if (!open) return;
     if (used + 2 > pool.size()) grow(used + 2048);
     (void)hipEventRecord(pool[used], s);  // timing is best effort
     marks.push_back({k, used});
     used += 2;
   }
   void end(hipStream_t s) {
-    if (on && !marks.empty()) (void)hipEventRecord(pool[marks.back().second + 1], s);
+    if (open && !marks.empty()) (void)hipEventRecord(pool[marks.back().second + 1], s);
+    open = false;
   }
   void grow(size_t n) {
     while (pool.size() < n) {

@@ -175,6 +175,9 @@ int cgfdm_newton_stats(cgfdm_ctx* ctx, double* solves, double* iterations, doubl
  * since recording started (it synchronizes the stream), and the grid points
  * one launch of a stage-kernel class updates (0 for the others). */
 int cgfdm_profile(cgfdm_ctx* ctx, int enable);
+/* Restrict the timing to a comma-separated list of kernel classes (NULL or
+ * "" for all, the default); the other launches then carry no events. */
+int cgfdm_profile_classes(cgfdm_ctx* ctx, const char* names);
 int cgfdm_profile_get(cgfdm_ctx* ctx, const char* name, double* total_ms, long* launches, long* points_per_launch);
 
 /* Learned initial guess of the rate-and-state Newton solve (extension, no
