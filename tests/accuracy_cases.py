@@ -34,7 +34,24 @@ def rs_curved():
     return c
 
 
-CASES = {"sw_curved": sw_curved, "rs_curved": rs_curved}
+def tp_curved():
+    """The rate-and-state case with thermal pressurization (Friction_type 3:
+    slip law with flash weakening, pore pressure from the 1-D diffusion
+    normal to the fault, TP_n = 21, hydraulic diffusivity 2^-11 m^2/s, exact
+    in FP32), 400 steps.  L = 0.4 m (as in the reference's slip-law set-up,
+    rk4.cu:883-886): with L = 0.1 m the slip-law state update of this
+    strongly weakening rupture goes unstable near step 200 in the oracle
+    itself.  The pore pressure approaches the normal stress, and slip rates
+    reach about 90 m/s."""
+    c = rs_curved()
+    p = c["params"]
+    p.update({"Friction_type": 3, "TP_n": 21, "RS_L": 0.4})
+    c["stress"]["L"] = np.full_like(c["stress"]["L"], 0.4)
+    c["stress"]["TP_hy"] = np.full_like(c["stress"]["L"], 2.0 ** -11)
+    return c
+
+
+CASES = {"sw_curved": sw_curved, "rs_curved": rs_curved, "tp_curved": tp_curved}
 
 
 def apply_to_oracle(case, o):

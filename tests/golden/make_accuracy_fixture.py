@@ -1,5 +1,5 @@
 """Writes the FP64-oracle reference outputs of the accuracy cases
-(tests/golden/accuracy_<case>.npz) that tests/test_accuracy.py compares the
+(tests/golden/accuracy_<case>.npz) that tests/test_rupture_accuracy.py compares the
 HIP solver with: final slip, rupture time (init_t0), peak slip rate and
 surface velocity seismograms at a few receivers.
 
