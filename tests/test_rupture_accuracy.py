@@ -30,15 +30,15 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, float) - np.asarray(b, float)) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("name", sorted(ac.CASES))
-def test_rupture_outputs_match_oracle(name):
+def run_against_fixture(name, **overrides):
+    """Runs case `name` on the GPU (with parameter overrides) and compares
+    its outputs with the FP64-oracle fixture; returns the report."""
     path = os.path.join(HERE, "golden", f"accuracy_{name}.npz")
     if not os.path.exists(path):
         pytest.skip(f"fixture {path} not generated")
     ref = np.load(path)
     case = ac.CASES[name]()
-    s = dyn.Solver(case["params"])
+    s = dyn.Solver(dict(case["params"], **overrides))
     ac.apply_to_solver(case, s)
     s.setup()
     d = s.dims()
@@ -64,9 +64,33 @@ def test_rupture_outputs_match_oracle(name):
         "seismogram_rel_l2": rel(seis, ref["seis"]),
     }
     print(json.dumps(report))
+    return report
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(ac.CASES))
+def test_rupture_outputs_match_oracle(name):
+    report = run_against_fixture(name)
     assert report["ruptured_nodes"] > 100
     assert report["slip_rel_l2"] <= 1e-3, report
     assert report["rupture_time_rms_over_duration"] <= 1e-3, report
     assert report["ruptured_set_agreement"] >= 0.999, report
     assert report["peak_rate_rel_l2"] <= 1e-2, report
     assert report["seismogram_rel_l2"] <= 1e-2, report
+
+
+@pytest.mark.gpu
+def test_rupture_outputs_bf16_metric():
+    """The slip-weakening run with the metric and media stored in BF16 (FP32
+    arithmetic), against the same FP64 fixture.  Each stored word carries up
+    to 2^-9 relative rounding (lambda, mu, rho included, so wave speeds move
+    by up to 0.2%); measured: final slip 7e-3, rupture time 4.7e-3 of the
+    duration, bounds 2e-2 and 1e-2.  (The rate-and-state case, nucleating
+    from creep, shifts its nucleation time under that change of the medium
+    by a tenth of the run, so its final slip is not comparable; the 1e-3
+    targets hold for FP32 storage.)"""
+    report = run_against_fixture("sw_curved", MetricBF16=1)
+    assert report["ruptured_nodes"] > 100
+    assert report["slip_rel_l2"] <= 2e-2, report
+    assert report["rupture_time_rms_over_duration"] <= 1e-2, report
+    assert report["ruptured_set_agreement"] >= 0.99, report
