@@ -24,6 +24,7 @@ FP32).  Prints one JSON line:
   cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, same
                  workload, a few steps
   mixed          the same workload with the metric and media stored in BF16
+  mixed_deviation  the same with the metric as BF16 deviations, media exact
                  (FP32 arithmetic): throughput and the step's fraction of the
                  672 B/pt-step roofline
   kernel_ms_per_step  every kernel class's GPU time per step, from the
@@ -55,8 +56,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_{workload}.txt")
 
 def b_alg(params):
     """Algorithmic bytes per grid-point update (one full RK4 step): 144
-    wavefield words + 48 coefficient words (12 per stage)."""
-    return 144 * 4 + 48 * (2 if params.get("MetricBF16") else 4)
+    wavefield words + 48 coefficient words (12 per stage: FP32; bfloat16;
+    or 9 bfloat16 metric deviations and 3 exact FP32 media words)."""
+    m = params.get("MetricBF16", 0)
+    return 144 * 4 + (48 * 4 if not m else 48 * 2 if m == 1 else 4 * (9 * 2 + 3 * 4))
 
 
 def workload_name(name, p):
@@ -183,11 +186,13 @@ def cpu_baseline(case, budget_s):
                       f"warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), {cpu_model()}"}
 
 
-def mixed_leg(case, steps, warmup):
-    """Same workload with MetricBF16=1: throughput of the whole step."""
+def mixed_leg(case, steps, warmup, mode=1):
+    """Same workload with MetricBF16=mode (1: metric and media words in
+    bfloat16; 2: metric words as bfloat16 deviations, media words exact):
+    throughput of the whole step."""
     import dynfd3dcuda_amd as dyn
     from dynfd3dcuda_amd import configs
-    p = dict(case["params"], MetricBF16=1)
+    p = dict(case["params"], MetricBF16=mode)
     s = dyn.Solver(p)
     configs.apply(case, s)
     s.setup()
@@ -203,7 +208,7 @@ def mixed_leg(case, steps, warmup):
     s.close()
     v = d["ni"] * d["nj"] * d["nk"] * steps / t
     ba = b_alg(p)
-    return {"metric_storage": "bf16", "value": v, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * t / steps,
+    return {"metric_storage": "bf16" if mode == 1 else "bf16_deviation_exact_media", "value": v, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * t / steps,
             "B_alg_bytes_per_pt_step": ba, "step_hbm_roofline_frac": v * ba / (PEAK_HBM_GBS * 1e9)}
 
 
@@ -394,6 +399,7 @@ def main():
         "kernel_ms_per_step": {k: v[0] / nprof for k, v in prof_all.items() if v[1] > 0},
         "cpu_baseline": None,
         "mixed": None,
+        "mixed_deviation": None,
         "rate_state": None,
     }
     solver.close()
@@ -402,6 +408,7 @@ def main():
     out["roofline"]["frac_of_copy_ceiling"] = achieved / ceil if ceil else None
     if not args.no_mixed:
         out["mixed"] = mixed_leg(case, args.steps, args.warmup)
+        out["mixed_deviation"] = mixed_leg(case, args.steps, args.warmup, mode=2)
     if not args.no_rs:
         out["rate_state"] = rate_state_leg(args)
     if not args.no_cpu:

@@ -74,7 +74,10 @@ FieldRef field(Ctx& c, const std::string& n) {
   if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; r.planes = nx; }
-  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; r.nf = 13; r.planes = nx; }
+  else if (n == "M_bf16") {
+    const int nf = c.p.MetricBF16 == 2 ? 16 : 13;
+    r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = (long)nf * nxyz; r.dtype = 3; r.nf = nf; r.planes = nx;
+  }
   else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "matVy2Vz") f32(c.matVy2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "Bx") f32(c.pml.B[0], c.d.ni);

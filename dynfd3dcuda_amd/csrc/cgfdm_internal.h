@@ -35,7 +35,7 @@ struct Params {
   int usePML = 1, useCerjan = 0, FreeSurface = 1, NormalRefine = 1, Rupture = 1;
   int FaultSmooth = 0;  // Makefile:104: Gaussian smoothing of the fault tractions once per step
   // extensions
-  int MetricBF16 = 0;   // store metric + media in bfloat16 (FP32 compute)
+  int MetricBF16 = 0;   // 1: metric + media stored in bfloat16; 2: metric as bfloat16 deviations from the Cartesian grid, media exact (FP32 compute)
   int NewtonMode = 0;   // 0: reference NRsearch; 1: residual tolerance + plain fallback
   double NewtonTol = 1e-10;
   double NewtonMLPGate = 2.0;  // learned guess consulted where the plain guess's relative residual exceeds this
@@ -182,6 +182,7 @@ struct Ctx {
   float* img_top = nullptr;     // free surface: image velocity RHS of the three top rows [3][3][ni][nj]
   int wave_nwork = 0;                  // all work items
   bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
+  float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
   void* mlp_dev = nullptr;          // MlpDev of the learned Newton guess (NewtonMode 2)

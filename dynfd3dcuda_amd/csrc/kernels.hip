@@ -23,7 +23,7 @@ struct FArgs {
   float* __restrict__ Wi0;       // wavefield receiving fault2wave
   const float* __restrict__ M;   // metric (fp32 path; bf16 handled via Mb)
   const uint16_t* __restrict__ Mb;
-  int bf16;
+  int bf16;   // 0 FP32 words; 1 bfloat16 words; 2 bfloat16 metric deviations, exact media words
   const float* mVx; const float* mVy;
   float* slabH;
   const float* S; float* T;      // for the slab RK
@@ -32,10 +32,11 @@ struct FArgs {
   int ni, nj, nk, nx, ny, nz, i0;
   long nyz;        // words of a 2-D fault array (ny*nz)
   long pyz;        // words of one field of one plane of a volume array (py*nz, padded rows)
-  long wps, mps;   // plane strides of the wave (9 pyz) and metric (13 pyz) volume arrays
+  long wps, mps;   // plane strides of the wave (9 pyz) and metric (13 or 16 pyz) volume arrays
   int py, jpad;    // row pitch and front pad of volume rows
   int NY, NZ;
   float DH, rDH, DT;
+  float mref[10];   // metric reference values (bfloat16-deviation form)
   double DTd;
   int it, irk, FX, FY, FZ;
   int freenode, friction, stress_type, cerjan;
@@ -51,7 +52,19 @@ struct FArgs {
 __device__ __forceinline__ float LM(const FArgs& a, long i) {
   return a.bf16 ? __uint_as_float(((unsigned)a.Mb[i]) << 16) : a.M[i];
 }
-
+// Metric or media word c at plane offset p.  In the bfloat16-deviation form
+// (bf16 == 2) words 0-9 are stored as their deviation from their field's
+// reference value mref[c] and the media words 10-12 as two exact halves, the
+// low half three fields further on.
+__device__ __forceinline__ float LMF(const FArgs& a, int c, long p) {
+  const long i = c * a.pyz + p;
+  if (a.bf16 == 2) {
+    const unsigned h = a.Mb[i];
+    if (c >= 10) return __uint_as_float((h << 16) | a.Mb[i + 3 * a.pyz]);
+    return __uint_as_float(h << 16) + a.mref[c];
+  }
+  return LM(a, i);
+}
 __device__ float Gt_func(float t, float T) {  // device.cu:134-143
   if (t < T) {
     float t1 = t - T, t2 = t - 2.0f * T;
@@ -68,8 +81,8 @@ __device__ float Fr_func(float r, float R) { return r < R ? expf(r * r / (r * r 
 // Conservative traction J*(grad xi . sigma) at plane i, volume column jk along metric row 'row'.
 __device__ __forceinline__ void trac(const FArgs& a, const float* W, int i, long jk, int row, float* o) {
   const long nyz = a.pyz, pm = jk + i * a.mps, pw = jk + i * a.wps;
-  float e0 = LM(a, (3 * row) * nyz + pm), e1 = LM(a, (3 * row + 1) * nyz + pm), e2 = LM(a, (3 * row + 2) * nyz + pm);
-  float J = LM(a, 9 * nyz + pm);
+  float e0 = LMF(a, 3 * row, pm), e1 = LMF(a, 3 * row + 1, pm), e2 = LMF(a, 3 * row + 2, pm);
+  float J = LMF(a, 9, pm);
   float txx = W[3 * nyz + pw], tyy = W[4 * nyz + pw], tzz = W[5 * nyz + pw];
   float txy = W[6 * nyz + pw], txz = W[7 * nyz + pw], tyz = W[8 * nyz + pw];
   o[0] = J * (e0 * txx + e1 * txy + e2 * txz);
@@ -159,10 +172,10 @@ __device__ __forceinline__ void fault_tail(const FArgs& a, const FaultDev& F, lo
   // ---- fault2wave into the stage input at i0
   {
     const long p = pc + i0 * mps, pw = pc + i0 * wps;
-    float mt[3][3] = {{LM(a, p), LM(a, 3 * pyz + p), LM(a, 6 * pyz + p)},
-                      {LM(a, pyz + p), LM(a, 4 * pyz + p), LM(a, 7 * pyz + p)},
-                      {LM(a, 2 * pyz + p), LM(a, 5 * pyz + p), LM(a, 8 * pyz + p)}};
-    float rj = 1.0f / LM(a, 9 * pyz + p);
+    float mt[3][3] = {{LMF(a, 0, p), LMF(a, 3, p), LMF(a, 6, p)},
+                      {LMF(a, 1, p), LMF(a, 4, p), LMF(a, 7, p)},
+                      {LMF(a, 2, p), LMF(a, 5, p), LMF(a, 8, p)}};
+    float rj = 1.0f / LMF(a, 9, p);
     float inv[3][3];
     inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
     inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
@@ -256,7 +269,7 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     float jac = 1.f;
     for (int l = 1; l <= 3; l++) {
       const int ii = i0 + (2 * m - 1) * l;
-      jac = LM(a, 9 * pyz + pc + ii * mps);
+      jac = LMF(a, 9, pc + ii * mps);
       float o[3];
       trac(a, a.W, ii, pc, 0, o);
       long pl = (3 + (2 * m - 1) * l) * nyz + pf;
@@ -286,9 +299,9 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
   }
   float t = irk == 0 ? a.it * DT : (irk == 3 ? (a.it + 1) * DT : (a.it + 0.5f) * DT);
   long pm = pc + i0 * mps;
-  float n[3] = {LM(a, pm), LM(a, pyz + pm), LM(a, 2 * pyz + pm)};
+  float n[3] = {LMF(a, 0, pm), LMF(a, 1, pm), LMF(a, 2, pm)};
   float n0 = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  float jacvec = LM(a, 9 * pyz + pm) * n0;
+  float jacvec = LMF(a, 9, pm) * n0;
   for (int c = 0; c < 3; c++) n[c] /= n0;
   float dV[3];
   for (int c = 0; c < 3; c++) dV[c] = F.mW[c * 2 * nyz + nyz + pf] - F.mW[c * 2 * nyz + pf];
@@ -566,7 +579,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
           for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
         }
       const long p = pc + i * mps;
-      float rrj = 1.0f / (LM(a, 12 * pyz + p) * LM(a, 9 * pyz + p));
+      float rrj = 1.0f / (LMF(a, 12, p) * LMF(a, 9, p));
       for (int c = 0; c < 3; c++) {
         float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
         slabx(i, c) = (dxt + dyt + dzt) * rrj;
@@ -593,10 +606,10 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
         else if (a.freenode && km == 3) dz[c] = d24(&W[c * pyz + pw], py, FZ) * rDH;
         else dz[c] = drp(&W[c * pyz + pw], py, FZ) * rDH;
       }
-      float lam = LM(a, 10 * pyz + p), mu = LM(a, 11 * pyz + p), l2m = lam + 2.0f * mu;
-      float xi[3] = {LM(a, p), LM(a, pyz + p), LM(a, 2 * pyz + p)};
-      float et[3] = {LM(a, 3 * pyz + p), LM(a, 4 * pyz + p), LM(a, 5 * pyz + p)};
-      float zt[3] = {LM(a, 6 * pyz + p), LM(a, 7 * pyz + p), LM(a, 8 * pyz + p)};
+      float lam = LMF(a, 10, p), mu = LMF(a, 11, p), l2m = lam + 2.0f * mu;
+      float xi[3] = {LMF(a, 0, p), LMF(a, 1, p), LMF(a, 2, p)};
+      float et[3] = {LMF(a, 3, p), LMF(a, 4, p), LMF(a, 5, p)};
+      float zt[3] = {LMF(a, 6, p), LMF(a, 7, p), LMF(a, 8, p)};
       slabx(i, 3) = l2m * xi[0] * dx[0] + lam * xi[1] * dx[1] + lam * xi[2] * dx[2] + l2m * et[0] * dy[0] + lam * et[1] * dy[1] +
                     lam * et[2] * dy[2] + l2m * zt[0] * dz[0] + lam * zt[1] * dz[1] + lam * zt[2] * dz[2];
       slabx(i, 4) = lam * xi[0] * dx[0] + l2m * xi[1] * dx[1] + lam * xi[2] * dx[2] + lam * et[0] * dy[0] + l2m * et[1] * dy[1] +
@@ -612,7 +625,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   if (part < 14) {  // split-node velocity rates of side part-12 (fault_dvelo.cu:260-385)
     const int m = part - 12;
     const float* TT[3] = {F.T11, F.T12, F.T13};
-    float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LM(a, 9 * pyz + pc + i0 * mps));
+    float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LMF(a, 9, pc + i0 * mps));
     for (int c = 0; c < 3; c++) {
       const float* T = TT[c];
       float dxt = m == 0 ? rDH * (A0M * T[3 * nyz + pf] - A1 * T[2 * nyz + pf] - A2 * T[1 * nyz + pf] - A3 * T[0 * nyz + pf])
@@ -761,10 +774,10 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   if (uni) return;
   // fault2wave into the stage output at i0
   const long p = pc + i0 * mps, pw = pc + i0 * wps;
-  float mt[3][3] = {{LM(a, p), LM(a, 3 * pyz + p), LM(a, 6 * pyz + p)},
-                    {LM(a, pyz + p), LM(a, 4 * pyz + p), LM(a, 7 * pyz + p)},
-                    {LM(a, 2 * pyz + p), LM(a, 5 * pyz + p), LM(a, 8 * pyz + p)}};
-  float rj = 1.0f / LM(a, 9 * pyz + p);
+  float mt[3][3] = {{LMF(a, 0, p), LMF(a, 3, p), LMF(a, 6, p)},
+                    {LMF(a, 1, p), LMF(a, 4, p), LMF(a, 7, p)},
+                    {LMF(a, 2, p), LMF(a, 5, p), LMF(a, 8, p)}};
+  float rj = 1.0f / LMF(a, 9, p);
   float inv[3][3];
   inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
   inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
@@ -930,11 +943,11 @@ __global__ void __launch_bounds__(256) smooth_avg_kernel(FArgs a, const float* _
           const long pm = pc + i * a.mps, pw = pc + i * a.wps;
           float E[3][3];  // columns grad xi, grad eta, grad zeta
           for (int r = 0; r < 3; r++)
-            for (int cc = 0; cc < 3; cc++) E[r][cc] = LM(a, (3 * cc + r) * pyz + pm);
+            for (int cc = 0; cc < 3; cc++) E[r][cc] = LMF(a, 3 * cc + r, pm);
           const float sxx = Wv[3 * pyz + pw], syy = Wv[4 * pyz + pw], szz = Wv[5 * pyz + pw];
           const float sxy = Wv[6 * pyz + pw], sxz = Wv[7 * pyz + pw], syz = Wv[8 * pyz + pw];
           const float S[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
-          const float w = LM(a, 9 * pyz + pm) * wp[(l * 7 + aa) * 7 + b];
+          const float w = LMF(a, 9, pm) * wp[(l * 7 + aa) * 7 + b];
           for (int r = 0; r < 3; r++) {
             const float t1 = S[r][0] * E[0][1] + S[r][1] * E[1][1] + S[r][2] * E[2][1];
             const float t2 = S[r][0] * E[0][2] + S[r][1] * E[1][2] + S[r][2] * E[2][2];
@@ -975,14 +988,15 @@ __global__ void fill_kernel(float* p, long n, float v) {
 FArgs make_fargs(Ctx& c) {
   FArgs a{};
   a.F = c.f;
-  a.bf16 = c.p.MetricBF16 ? 1 : 0;
+  a.bf16 = c.p.MetricBF16;
+  for (int m = 0; m < 10; m++) a.mref[m] = c.met_ref[m];
   a.M = a.bf16 ? nullptr : (const float*)c.met;
   a.Mb = a.bf16 ? (const uint16_t*)c.met : nullptr;
   a.mVx = c.matVx2Vz; a.mVy = c.matVy2Vz;
   a.slabH = c.slabH;
   for (int x = 0; x < 3; x++) a.cj[x] = c.cj[x];
   a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.nx = c.d.nx; a.ny = c.d.ny; a.nz = c.d.nz; a.i0 = c.d.i0;
-  a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = 13 * c.d.pyz;
+  a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = (c.p.MetricBF16 == 2 ? 16 : 13) * c.d.pyz;
   a.py = c.d.py; a.jpad = c.d.jpad;
   for (int q = 0; q < 4; q++) a.G[q] = c.p.Fault_grid[q];
   a.mlp = c.p.NewtonMode == 2 ? (const MlpDev*)c.mlp_dev : nullptr;

@@ -140,10 +140,17 @@ __device__ inline uint16_t metric_store(double v, uint16_t*) {  // bf16, round t
 
 // The 13 metric and media words of every stored point (ghosts mirrored), in
 // the device layout, and the largest P-wave speed of the interior as the
-// bits of a positive double.  One thread per point, j fastest.
-template <class OT>
+// bits of a positive double.  One thread per point, j fastest.  NF = 16 is
+// the bfloat16-deviation form: fields 0-9 hold the bfloat16 of each metric
+// word less its field's reference value ref.r[c], and the media words are
+// exact, the high 16 bits of the FP32 lambda, mu, rho in fields 10-12, the
+// low 16 bits in fields 13-15.
+struct MetRef {
+  double r[10];
+};
+template <class OT, int NF = 13>
 __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, long pyz, int py, int jpad,
-                                                           unsigned long long* vpmax_bits) {
+                                                           unsigned long long* vpmax_bits, MetRef ref) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   const long total = (long)m.nx * m.nz * m.ny;
   double vp = 0.0;
@@ -154,7 +161,18 @@ __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, 
     double v[13];
     metric_point(m, mirror_idx(i, m.nx), mirror_idx(j, m.ny), mirror_idx(k, m.nz), v);
     const long base = (long)k * py + jpad + j;
-    for (int c = 0; c < 13; c++) out[((long)i * 13 + c) * pyz + base] = metric_store(v[c], out);
+    if constexpr (NF == 16) {
+      for (int c = 0; c < 10; c++) {
+        out[((long)i * 16 + c) * pyz + base] = metric_store(v[c] - ref.r[c], out);
+      }
+      for (int c = 10; c < 13; c++) {
+        const uint32_t u = __float_as_uint((float)v[c]);
+        out[((long)i * 16 + c) * pyz + base] = (OT)(u >> 16);
+        out[((long)i * 16 + c + 3) * pyz + base] = (OT)(u & 0xffffu);
+      }
+    } else {
+      for (int c = 0; c < 13; c++) out[((long)i * 13 + c) * pyz + base] = metric_store(v[c], out);
+    }
     if (i >= 3 && i < m.nx - 3 && j >= 3 && j < m.ny - 3 && k >= 3 && k < m.nz - 3)
       vp = sqrt((v[10] + 2 * v[11]) / v[12]);
   }
@@ -210,6 +228,7 @@ int host_setup(Ctx& c, std::string& err) {
   d.py = (d.jpad + d.ny + 31) / 32 * 32;
   d.pyz = (long)d.py * d.nz;
   if (p.NX < 7 || p.NY < 7 || p.NZ < 7) { err = "NX, NY and NZ must be at least 7"; return 1; }
+  if (p.MetricBF16 < 0 || p.MetricBF16 > 2) { err = "MetricBF16 must be 0, 1 or 2"; return 1; }
   const int nx = d.nx, ny = d.ny, nz = d.nz, ni = d.ni, nj = d.nj, nk = d.nk;
   const long nxyz = d.nxyz, nyz = d.nyz;
   auto P = [&](int i, int j, int k) { return j + (long)k * ny + (long)i * nyz; };
@@ -372,17 +391,53 @@ int host_setup(Ctx& c, std::string& err) {
     void* vq = nullptr;
     if (hipMalloc(&vq, sizeof(unsigned long long)) == hipSuccess) { tmp.push_back(vq); dvp = (unsigned long long*)vq; }
     bool ok = dC && dvp && (mh.nl == 0 || (md.d1 && md.r1 && md.p1 && md.s1)) && (!mh.vp3 || (md.vp3 && md.vs3 && md.rho3));
-    const long nmet = (long)nx * 13 * pyz;
+    const long nmet = (long)nx * (p.MetricBF16 == 2 ? 16 : 13) * pyz;
     void* met = nullptr;
     if (ok) met = p.MetricBF16 ? (void*)dmalloc<uint16_t>(c, nmet) : (void*)dmalloc<float>(c, nmet);
     ok = ok && met && hipMemset(dvp, 0, sizeof(unsigned long long)) == hipSuccess;
+    MetRef ref{};
+    if (ok && p.MetricBF16 == 2) {
+      // reference value of each metric field: the midrange over every 4th
+      // stored point along each axis (and the last), ghosts mirrored as the
+      // device stores them, rounded to FP32 (the kernels add it in FP32)
+      auto axis = [](int n) {
+        std::vector<int> v;
+        for (int a = 0; a < n; a += 4) v.push_back(mirror_idx(a, n));
+        v.push_back(mirror_idx(n - 1, n));
+        return v;
+      };
+      const std::vector<int> ai = axis(nx), aj = axis(ny), ak = axis(nz);
+      double lo[10], hi[10];
+      for (int q = 0; q < 10; q++) { lo[q] = 1e300; hi[q] = -1e300; }
+#pragma omp parallel
+      {
+        double l[10], h[10], v[13];
+        for (int q = 0; q < 10; q++) { l[q] = 1e300; h[q] = -1e300; }
+#pragma omp for nowait
+        for (long t = 0; t < (long)ai.size() * (long)ak.size(); t++) {
+          const int i = ai[t / (long)ak.size()], k = ak[t % (long)ak.size()];
+          for (int j : aj) {
+            metric_point(mh, i, j, k, v);
+            for (int q = 0; q < 10; q++) { l[q] = std::min(l[q], v[q]); h[q] = std::max(h[q], v[q]); }
+          }
+        }
+#pragma omp critical
+        for (int q = 0; q < 10; q++) { lo[q] = std::min(lo[q], l[q]); hi[q] = std::max(hi[q], h[q]); }
+      }
+      for (int q = 0; q < 10; q++) {
+        c.met_ref[q] = (float)(0.5 * (lo[q] + hi[q]));
+        ref.r[q] = c.met_ref[q];
+      }
+    }
     if (ok) {
       const long total = (long)nx * nz * ny;
       const unsigned blocks = (unsigned)((total + 255) / 256);
-      if (p.MetricBF16)
-        hipLaunchKernelGGL(metric_media_kernel<uint16_t>, dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp);
+      if (p.MetricBF16 == 2)
+        hipLaunchKernelGGL((metric_media_kernel<uint16_t, 16>), dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref);
+      else if (p.MetricBF16)
+        hipLaunchKernelGGL(metric_media_kernel<uint16_t>, dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref);
       else
-        hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks), dim3(256), 0, 0, md, (float*)met, pyz, d.py, d.jpad, dvp);
+        hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks), dim3(256), 0, 0, md, (float*)met, pyz, d.py, d.jpad, dvp, ref);
       unsigned long long bits = 0;
       ok = hipGetLastError() == hipSuccess && hipMemcpy(&bits, dvp, sizeof bits, hipMemcpyDeviceToHost) == hipSuccess;
       double vpm;

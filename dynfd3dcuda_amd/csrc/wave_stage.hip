@@ -26,6 +26,7 @@
 // fault_dstrs) before fault_post advances the slab.
 #include "fd_ops.h"
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace cgfdm {
@@ -43,7 +44,8 @@ constexpr int TKN = NTHRN / TJN;
 
 // Volume arrays are plane-interleaved with padded rows: word (field c,
 // padded plane x, column jk = jpad + j + k*py) sits at (x*nf + c)*pyz + jk,
-// nf = 9 for wavefields, 13 for the metric; py is a multiple of 32 words.  A plane of an array is addressed as one buffer resource
+// nf = 9 for wavefields, 13 for the metric (16 in its bfloat16-deviation form); py
+// is a multiple of 32 words.  A plane of an array is addressed as one buffer resource
 // (base = the plane's first word, range = the plane's bytes) with the lane's
 // column byte offset as voffset and the field's offset c*nyz*size as
 // soffset: no per-lane 64-bit address arithmetic, a handful of scalar
@@ -82,11 +84,44 @@ __device__ __forceinline__ float bldm(Rsrc r, uint32_t voff, uint32_t soff) {
 __device__ __forceinline__ float wat(const float* W, int c, int x, uint32_t jk, uint32_t nyz) {
   return W[((size_t)x * 9 + c) * nyz + jk];
 }
-__device__ __forceinline__ float mat_(const float* M, int c, int x, uint32_t jk, uint32_t nyz) {
+__device__ __forceinline__ float mat_(const float* M, int c, int x, uint32_t jk, uint32_t nyz, const float*) {
   return M[((size_t)x * 13 + c) * nyz + jk];
 }
-__device__ __forceinline__ float mat_(const uint16_t* M, int c, int x, uint32_t jk, uint32_t nyz) {
+__device__ __forceinline__ float mat_(const uint16_t* M, int c, int x, uint32_t jk, uint32_t nyz, const float*) {
   return __uint_as_float(((unsigned)M[((size_t)x * 13 + c) * nyz + jk]) << 16);
+}
+// Metric in bfloat16 with the media words exact (16 fields per plane):
+// fields 0-9 hold the bfloat16 of each metric word's deviation from a
+// reference value of its field (ref[c], the midrange of the field over the
+// grid), so the rounding scales with the field's variation rather than its
+// size; fields 10-12 hold the high and 13-15 the low 16 bits of the FP32
+// lambda, mu and rho.
+struct MetBf16Delta {
+  uint16_t h;
+};
+template <class MT>
+constexpr uint32_t met_nf() {
+  return std::is_same<MT, MetBf16Delta>::value ? 16u : 13u;
+}
+__device__ __forceinline__ float mat_(const MetBf16Delta* Ms, int c, int x, uint32_t jk, uint32_t nyz, const float* ref) {
+  const uint16_t* M = &Ms->h;
+  const unsigned hi = M[((size_t)x * 16 + c) * nyz + jk];
+  if (c < 10) return __uint_as_float(hi << 16) + ref[c];
+  return __uint_as_float((hi << 16) | M[((size_t)x * 16 + c + 3) * nyz + jk]);
+}
+// Field c of a metric plane resource r at byte column voff (mpb bytes a
+// field); ref is the field's reference value in the deviation form.
+template <class MT>
+__device__ __forceinline__ float bldmc(Rsrc r, uint32_t voff, int c, uint32_t mpb, float ref) {
+  if constexpr (std::is_same<MT, MetBf16Delta>::value) {
+    if (c >= 10) {
+      const unsigned hi = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)(c * mpb), kStream);
+      const unsigned lo = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)((c + 3) * mpb), kStream);
+      return __uint_as_float((hi << 16) | lo);
+    }
+    return bldm<MT>(r, voff, c * mpb) + ref;
+  }
+  return bldm<MT>(r, voff, c * mpb);
 }
 
 // RK_Syn combine (rk4.cu:20-33) of field c of the planes rs (S) and rt (T).
@@ -118,6 +153,7 @@ struct WaveArgs {
   int py, jpad;                   // row pitch and front pad of volume rows
   int nwork, ni, nj, nk, ny, ii0;
   float rDH, rka, rkb;
+  float mref[10];   // metric reference values (bfloat16-deviation form)
   int irk, freenode, cerjan, fault, rhs_only, pml;
 };
 
@@ -148,13 +184,13 @@ __global__ void __launch_bounds__(256) traction_top_kernel(WaveArgs<MT> a, int n
   const uint32_t q = (uint32_t)a.jpad + (uint32_t)jj + (uint32_t)(a.nk - 6 + r + 3) * (uint32_t)a.py;
   const float* W = a.Win;
   const MT* M = a.met;
-  const float J = mat_(M, 9, x, q, nyz);
+  const float J = mat_(M, 9, x, q, nyz, a.mref);
   const float txx = wat(W, 3, x, q, nyz), tyy = wat(W, 4, x, q, nyz), tzz = wat(W, 5, x, q, nyz);
   const float txy = wat(W, 6, x, q, nyz), txz = wat(W, 7, x, q, nyz), tyz = wat(W, 8, x, q, nyz);
   float* o = a.ttop + ((long)x * 6 + r) * 9 * ny + jj;
 #pragma unroll
   for (int ax = 0; ax < 3; ax++) {
-    const float e0 = mat_(M, 3 * ax, x, q, nyz), e1 = mat_(M, 3 * ax + 1, x, q, nyz), e2 = mat_(M, 3 * ax + 2, x, q, nyz);
+    const float e0 = mat_(M, 3 * ax, x, q, nyz, a.mref), e1 = mat_(M, 3 * ax + 1, x, q, nyz, a.mref), e2 = mat_(M, 3 * ax + 2, x, q, nyz, a.mref);
     o[(3 * ax + 0) * ny] = (e0 * txx + e1 * txy + e2 * txz) * J;
     o[(3 * ax + 1) * ny] = (e0 * txy + e1 * tyy + e2 * tyz) * J;
     o[(3 * ax + 2) * ny] = (e0 * txz + e1 * tyz + e2 * tzz) * J;
@@ -176,7 +212,7 @@ __global__ void __launch_bounds__(256) image_top_kernel(WaveArgs<MT> a) {
   const int k = nk - 3 + row, ik = 2 - row, x = i + 3;
   const uint32_t col = (uint32_t)a.jpad + (uint32_t)(j + 3) + (uint32_t)(k + 3) * (uint32_t)a.py;
   const MT* M = a.met;
-  const float rrhojac = 1.0f / (mat_(M, 12, x, col, a.nyz) * mat_(M, 9, x, col, a.nyz));
+  const float rrhojac = 1.0f / (mat_(M, 12, x, col, a.nyz, a.mref) * mat_(M, 9, x, col, a.nyz, a.mref));
   float rb[3] = {1.0f, 1.0f, 1.0f};
   if (a.pml) {
     const WaveCold* cold = a.cold;
@@ -341,8 +377,8 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
     }
 
     if (act) {
-      const Rsrc rm = plane_rsrc(a.met, i + 3, 13, nyz);
-      auto MET = [&](int c) { return bldm<MT>(rm, mcol, c * mpb); };
+      const Rsrc rm = plane_rsrc(a.met, i + 3, met_nf<MT>(), nyz);
+      auto MET = [&](int c) { return bldmc<MT>(rm, mcol, c, mpb, c < 10 ? a.mref[c] : 0.0f); };
       const float xix = MET(0), xiy = MET(1), xiz = MET(2);
       const float etx = MET(3), ety = MET(4), etz = MET(5);
       const float ztx = MET(6), zty = MET(7), ztz = MET(8);
@@ -506,6 +542,7 @@ struct PmlAuxArgs {
   int ni, nj, nk, ny, py, jpad;
   unsigned pyz;
   float rDH, rka, rkb;
+  float mref[10];   // metric reference values (bfloat16-deviation form)
   int irk, freenode;
 };
 
@@ -555,7 +592,7 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
                                 : drp5<-1>(w[-3 * st], w[-2 * st], w[-st], w[0], w[st]));
   }
   const MT* M = a.met;
-  auto MET = [&](int m) { return mat_(M, m, x, (uint32_t)col, pyz); };
+  auto MET = [&](int m) { return mat_(M, m, x, (uint32_t)col, pyz, a.mref); };
   const float e0 = MET(3 * ax), e1 = MET(3 * ax + 1), e2 = MET(3 * ax + 2);
   const float lam = MET(10), mu = MET(11), rrho = 1.0f / MET(12), l2m = lam + 2.0f * mu;
   const float* pAx = pick3(ax, a.pA);
@@ -650,6 +687,7 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     PmlAuxArgs<MT> p{};
     p.Win = a.Win; p.met = a.met; p.N = c.pml.N;
     p.ni = a.ni; p.nj = a.nj; p.nk = a.nk; p.ny = a.ny; p.py = a.py; p.jpad = a.jpad; p.pyz = a.nyz;
+    for (int m = 0; m < 10; m++) p.mref[m] = a.mref[m];
     p.rDH = a.rDH; p.rka = a.rka; p.rkb = a.rkb; p.irk = a.irk; p.freenode = a.freenode;
     p.flag[0] = FX; p.flag[1] = FY; p.flag[2] = FZ;
     for (int ax = 0; ax < 3; ax++) { p.pA[ax] = c.pml.A[ax]; p.pB[ax] = c.pml.B[ax]; p.pD[ax] = c.pml.D[ax]; }
@@ -708,6 +746,7 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
   a.nyz = (unsigned)c.d.pyz;
   a.py = c.d.py; a.jpad = c.d.jpad;
   a.rDH = (float)(1.0 / c.p.DH);
+  for (int m = 0; m < 10; m++) a.mref[m] = c.met_ref[m];
   const double RA[4] = {0.5, 0.5, 1.0, 1e10}, RB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
   a.rka = (float)(RA[irk] * c.p.DT);
   a.rkb = (float)(RB[irk] * c.p.DT);
@@ -897,7 +936,10 @@ int upload_wave_cold(Ctx& c) {
 }
 
 void launch_wave_stage(Ctx& c, const StageBufs& b, int irk, int fx, int fy, int fz, bool rhs_only, float* rhs_out) {
-  if (c.p.MetricBF16) {
+  if (c.p.MetricBF16 == 2) {
+    WaveArgs<MetBf16Delta> a = make_args<MetBf16Delta>(c, b, irk, rhs_only, rhs_out);
+    launch_wave_dispatch(a, c, fx, fy, fz, c.stream);
+  } else if (c.p.MetricBF16) {
     WaveArgs<uint16_t> a = make_args<uint16_t>(c, b, irk, rhs_only, rhs_out);
     launch_wave_dispatch(a, c, fx, fy, fz, c.stream);
   } else {

@@ -83,8 +83,8 @@ def test_rupture_outputs_match_oracle(name):
 def test_rupture_outputs_bf16_metric():
     """The slip-weakening run with the metric and media stored in BF16 (FP32
     arithmetic), against the same FP64 fixture.  Each stored word carries up
-    to 2^-9 relative rounding (lambda, mu, rho included, so wave speeds move
-    by up to 0.2%); measured: final slip 7e-3, rupture time 4.7e-3 of the
+    to 2^-8 relative rounding (lambda, mu, rho included, so wave speeds move
+    by up to 0.4%); measured: final slip 7e-3, rupture time 4.7e-3 of the
     duration, bounds 2e-2 and 1e-2.  (The rate-and-state case, nucleating
     from creep, shifts its nucleation time under that change of the medium
     by a tenth of the run, so its final slip is not comparable; the 1e-3
@@ -93,4 +93,22 @@ def test_rupture_outputs_bf16_metric():
     assert report["ruptured_nodes"] > 100
     assert report["slip_rel_l2"] <= 2e-2, report
     assert report["rupture_time_rms_over_duration"] <= 1e-2, report
+    assert report["ruptured_set_agreement"] >= 0.99, report
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,slip_tol,time_tol", [("sw_curved", 1e-2, 5e-3), ("tp_curved", 5e-3, 1e-2)])
+def test_rupture_outputs_bf16_metric_deviations(name, slip_tol, time_tol):
+    """MetricBF16=2: the metric words as bfloat16 deviations from a reference
+    value per field, lambda, mu, rho exact (FP32 arithmetic), against the
+    FP64 fixtures.  On these grids the stored metric words are then within
+    1.2e-4 (absolute) of their FP32 values against 3.9e-3 with every word in
+    bfloat16; measured: slip-weakening final slip 4.7e-3 and rupture time
+    2.5e-3 of the duration (against 7e-3 and 4.7e-3 with every word in
+    bfloat16), thermal pressurization 2.3e-3 and 4.7e-3 (against 1.2e-2 and
+    1.3e-2).  Bounds about twice the measured values."""
+    report = run_against_fixture(name, MetricBF16=2)
+    assert report["ruptured_nodes"] > 100
+    assert report["slip_rel_l2"] <= slip_tol, report
+    assert report["rupture_time_rms_over_duration"] <= time_tol, report
     assert report["ruptured_set_agreement"] >= 0.99, report
