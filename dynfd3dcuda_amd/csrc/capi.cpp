@@ -61,6 +61,8 @@ struct FieldRef {
   int dtype = 0;   // 0 f32, 1 f64, 2 i32, 3 bf16
   int nf = 0;      // >0: device volume array, nf fields per plane, padded rows (count is the caller's size)
   int planes = 0;  // planes of such an array
+  int f0 = 0, fn = 0;  // fn > 0: only the fields f0 .. f0+fn-1 of each plane are exchanged
+  bool host = false;   // a host-side value (read only)
 };
 
 FieldRef field(Ctx& c, const std::string& n) {
@@ -74,10 +76,15 @@ FieldRef field(Ctx& c, const std::string& n) {
   if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; r.planes = nx; }
-  else if (n == "M_bf16") {
-    const int nf = c.p.MetricBF16 == 2 ? 16 : 13;
-    r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = (long)nf * nxyz; r.dtype = 3; r.nf = nf; r.planes = nx;
+  else if (n == "M_bf16") {  // MetricBF16 = 2: the 10 metric deviations from M_ref
+    const int nf = c.p.MetricBF16 == 2 ? 16 : 13, nout = c.p.MetricBF16 == 2 ? 10 : 13;
+    r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = (long)nout * nxyz; r.dtype = 3; r.nf = nf; r.planes = nx;
+    r.fn = nout;
   }
+  else if (n == "M_media") {  // MetricBF16 = 2: lambda, mu, rho, the FP32 planes in slots 10-15
+    f32(c.p.MetricBF16 == 2 ? c.met : nullptr, 3 * nxyz); r.nf = 8; r.planes = nx; r.f0 = 5; r.fn = 3;
+  }
+  else if (n == "M_ref") { r.ptr = c.p.MetricBF16 == 2 ? c.met_ref : nullptr; r.count = 10; r.dtype = 0; r.host = true; }
   else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "matVy2Vz") f32(c.matVy2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "Bx") f32(c.pml.B[0], c.d.ni);
@@ -500,10 +507,11 @@ static void interleave_copy(const Ctx& c, const FieldRef& r, const char* from, c
   const size_t np = (size_t)r.planes, nf = (size_t)r.nf, pyz = (size_t)c.d.pyz, nyz = ny * nz;
   const bool slab = np != (size_t)c.d.nx;
   for (size_t i = 0; i < np; i++)
-    for (size_t f = 0; f < nf; f++)
+    for (size_t f = (size_t)r.f0; f < (r.fn ? (size_t)(r.f0 + r.fn) : nf); f++)
       for (size_t k = 0; k < nz; k++) {
+        const size_t fo = f - (size_t)r.f0, nfo = r.fn ? (size_t)r.fn : nf;
         const size_t dev = ((i * nf + f) * pyz + k * py + c.d.jpad) * es;
-        const size_t ref = ((slab ? i * nf + f : f * np + i) * nyz + k * ny) * es;
+        const size_t ref = ((slab ? i * nfo + fo : fo * np + i) * nyz + k * ny) * es;
         std::memcpy(to + (to_dev ? dev : ref), from + (to_dev ? ref : dev), ny * es);
       }
 }
@@ -524,6 +532,10 @@ int cgfdm_get_field(cgfdm_ctx* h, const char* name, void* dst, size_t bytes) {
   FieldRef r = field(c, name);
   if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
   if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
+  if (r.host) {
+    std::memcpy(dst, r.ptr, bytes);
+    return 0;
+  }
   if (hipError_t e = hipStreamSynchronize(c.stream)) return fail(CGFDM_EHIP, hipGetErrorString(e));
   if (!r.nf) {
     if (hipMemcpy(dst, r.ptr, bytes, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
@@ -546,6 +558,7 @@ int cgfdm_put_field(cgfdm_ctx* h, const char* name, const void* src, size_t byte
   FieldRef r = field(c, name);
   if (!r.ptr) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
   if (bytes != r.count * dsize(r.dtype)) return fail(CGFDM_ESIZE, std::string("size mismatch for ") + name);
+  if (r.host) return fail(CGFDM_EPARAM, std::string("read-only field ") + name);
   if (hipError_t e = hipStreamSynchronize(c.stream)) return fail(CGFDM_EHIP, hipGetErrorString(e));
   if (!r.nf) {
     if (hipMemcpy(r.ptr, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");

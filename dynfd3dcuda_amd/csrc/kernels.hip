@@ -23,7 +23,7 @@ struct FArgs {
   float* __restrict__ Wi0;       // wavefield receiving fault2wave
   const float* __restrict__ M;   // metric (fp32 path; bf16 handled via Mb)
   const uint16_t* __restrict__ Mb;
-  int bf16;   // 0 FP32 words; 1 bfloat16 words; 2 bfloat16 metric deviations, exact media words
+  int bf16;   // 0 FP32 words; 1 bfloat16 words; 2 bfloat16 metric deviations, FP32 media words
   const float* mVx; const float* mVy;
   float* slabH;
   const float* S; float* T;      // for the slab RK
@@ -52,18 +52,20 @@ struct FArgs {
 __device__ __forceinline__ float LM(const FArgs& a, long i) {
   return a.bf16 ? __uint_as_float(((unsigned)a.Mb[i]) << 16) : a.M[i];
 }
-// Metric or media word c at plane offset p.  In the bfloat16-deviation form
-// (bf16 == 2) words 0-9 are stored as their deviation from their field's
-// reference value mref[c] and the media words 10-12 as two exact halves, the
-// low half three fields further on.
+// Metric word c (0-9) at plane offset p.  In the bfloat16-deviation form
+// (bf16 == 2) it is stored as its deviation from its field's reference value
+// mref[c].
 __device__ __forceinline__ float LMF(const FArgs& a, int c, long p) {
   const long i = c * a.pyz + p;
-  if (a.bf16 == 2) {
-    const unsigned h = a.Mb[i];
-    if (c >= 10) return __uint_as_float((h << 16) | a.Mb[i + 3 * a.pyz]);
-    return __uint_as_float(h << 16) + a.mref[c];
-  }
+  if (a.bf16 == 2) return __uint_as_float(((unsigned)a.Mb[i]) << 16) + a.mref[c];
   return LM(a, i);
+}
+// Media word c (10 lambda, 11 mu, 12 rho) of padded plane x at column pc.  In
+// the bfloat16-deviation form the slots 10-15 of a plane hold the three
+// media words as FP32 planes.
+__device__ __forceinline__ float LMEDIA(const FArgs& a, int c, long x, long pc) {
+  if (a.bf16 == 2) return ((const float*)(a.Mb + x * a.mps + 10 * a.pyz))[(c - 10) * a.pyz + pc];
+  return LM(a, c * a.pyz + pc + x * a.mps);
 }
 __device__ float Gt_func(float t, float T) {  // device.cu:134-143
   if (t < T) {
@@ -579,7 +581,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
           for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
         }
       const long p = pc + i * mps;
-      float rrj = 1.0f / (LMF(a, 12, p) * LMF(a, 9, p));
+      float rrj = 1.0f / (LMEDIA(a, 12, i, pc) * LMF(a, 9, p));
       for (int c = 0; c < 3; c++) {
         float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
         slabx(i, c) = (dxt + dyt + dzt) * rrj;
@@ -606,7 +608,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
         else if (a.freenode && km == 3) dz[c] = d24(&W[c * pyz + pw], py, FZ) * rDH;
         else dz[c] = drp(&W[c * pyz + pw], py, FZ) * rDH;
       }
-      float lam = LMF(a, 10, p), mu = LMF(a, 11, p), l2m = lam + 2.0f * mu;
+      float lam = LMEDIA(a, 10, i, pc), mu = LMEDIA(a, 11, i, pc), l2m = lam + 2.0f * mu;
       float xi[3] = {LMF(a, 0, p), LMF(a, 1, p), LMF(a, 2, p)};
       float et[3] = {LMF(a, 3, p), LMF(a, 4, p), LMF(a, 5, p)};
       float zt[3] = {LMF(a, 6, p), LMF(a, 7, p), LMF(a, 8, p)};

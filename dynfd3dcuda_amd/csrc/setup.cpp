@@ -142,9 +142,8 @@ __device__ inline uint16_t metric_store(double v, uint16_t*) {  // bf16, round t
 // the device layout, and the largest P-wave speed of the interior as the
 // bits of a positive double.  One thread per point, j fastest.  NF = 16 is
 // the bfloat16-deviation form: fields 0-9 hold the bfloat16 of each metric
-// word less its field's reference value ref.r[c], and the media words are
-// exact, the high 16 bits of the FP32 lambda, mu, rho in fields 10-12, the
-// low 16 bits in fields 13-15.
+// word less its field's reference value ref.r[c], and the slots 10-15 of a
+// plane hold lambda, mu, rho as three FP32 planes.
 struct MetRef {
   double r[10];
 };
@@ -165,11 +164,8 @@ __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, 
       for (int c = 0; c < 10; c++) {
         out[((long)i * 16 + c) * pyz + base] = metric_store(v[c] - ref.r[c], out);
       }
-      for (int c = 10; c < 13; c++) {
-        const uint32_t u = __float_as_uint((float)v[c]);
-        out[((long)i * 16 + c) * pyz + base] = (OT)(u >> 16);
-        out[((long)i * 16 + c + 3) * pyz + base] = (OT)(u & 0xffffu);
-      }
+      float* media = reinterpret_cast<float*>(out + ((long)i * 16 + 10) * pyz);
+      for (int c = 10; c < 13; c++) media[(long)(c - 10) * pyz + base] = (float)v[c];
     } else {
       for (int c = 0; c < 13; c++) out[((long)i * 13 + c) * pyz + base] = metric_store(v[c], out);
     }

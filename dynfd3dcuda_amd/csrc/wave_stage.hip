@@ -94,8 +94,7 @@ __device__ __forceinline__ float mat_(const uint16_t* M, int c, int x, uint32_t 
 // fields 0-9 hold the bfloat16 of each metric word's deviation from a
 // reference value of its field (ref[c], the midrange of the field over the
 // grid), so the rounding scales with the field's variation rather than its
-// size; fields 10-12 hold the high and 13-15 the low 16 bits of the FP32
-// lambda, mu and rho.
+// size; the slots 10-15 hold lambda, mu and rho as three FP32 planes.
 struct MetBf16Delta {
   uint16_t h;
 };
@@ -105,20 +104,15 @@ constexpr uint32_t met_nf() {
 }
 __device__ __forceinline__ float mat_(const MetBf16Delta* Ms, int c, int x, uint32_t jk, uint32_t nyz, const float* ref) {
   const uint16_t* M = &Ms->h;
-  const unsigned hi = M[((size_t)x * 16 + c) * nyz + jk];
-  if (c < 10) return __uint_as_float(hi << 16) + ref[c];
-  return __uint_as_float((hi << 16) | M[((size_t)x * 16 + c + 3) * nyz + jk]);
+  if (c < 10) return __uint_as_float(((unsigned)M[((size_t)x * 16 + c) * nyz + jk]) << 16) + ref[c];
+  return ((const float*)(M + ((size_t)x * 16 + 10) * nyz))[(size_t)(c - 10) * nyz + jk];
 }
 // Field c of a metric plane resource r at byte column voff (mpb bytes a
 // field); ref is the field's reference value in the deviation form.
 template <class MT>
 __device__ __forceinline__ float bldmc(Rsrc r, uint32_t voff, int c, uint32_t mpb, float ref) {
   if constexpr (std::is_same<MT, MetBf16Delta>::value) {
-    if (c >= 10) {
-      const unsigned hi = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)(c * mpb), kStream);
-      const unsigned lo = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)((c + 3) * mpb), kStream);
-      return __uint_as_float((hi << 16) | lo);
-    }
+    if (c >= 10) return bld<kStream>(r, 2 * voff, (uint32_t)(10 + 2 * (c - 10)) * mpb);  // FP32 word
     return bldm<MT>(r, voff, c * mpb) + ref;
   }
   return bldm<MT>(r, voff, c * mpb);

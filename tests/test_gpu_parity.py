@@ -226,8 +226,7 @@ def test_metric_bf16_delta_storage():
     """MetricBF16=2 stores the metric words (fields 0-9) as the bfloat16 of
     their deviation from a reference value per field (the midrange of the
     field over every 4th stored point along each axis, and the last, in
-    FP32) and lambda, mu, rho exact, as the high and low 16 bits of their
-    FP32 words (fields 10-12 and 13-15).  Decoded, each metric word lies
+    FP32) and lambda, mu, rho as FP32 planes.  Decoded, each metric word lies
     within one bfloat16 rounding of its deviation (2^-8 |m - ref|, plus FP32
     rounding) of the FP32-stored word, and the media words equal the
     FP32-stored ones bit for bit.  A 40-step slip-weakening rupture on a
@@ -238,7 +237,7 @@ def test_metric_bf16_delta_storage():
     s, o = make_pair(case)
     d = o.dims()
     shp = (d["nx"], d["nz"], d["ny"])
-    m2 = s.get("M_bf16").astype(np.uint32).reshape((16,) + shp)
+    m2 = s.get("M_bf16").astype(np.uint32).reshape((10,) + shp)
     t = dyn.Solver(dict(case["params"], MetricBF16=0))
     cases.apply_to_solver(case, t)
     t.setup()
@@ -246,16 +245,17 @@ def test_metric_bf16_delta_storage():
     t.close()
     ix = [np.r_[np.arange(0, n, 4), n - 1] for n in shp]
     smp = m0[:10][:, ix[0]][:, :, ix[1]][:, :, :, ix[2]].reshape(10, -1)
-    ref = (0.5 * (smp.min(1) + smp.max(1))).astype(np.float32).reshape(10, 1, 1, 1)
-    dec = np.ascontiguousarray(m2[:10] << 16).view(np.float32) + ref
+    ref = s.get("M_ref")  # midrange of the FP64 words, in FP32
+    np.testing.assert_allclose(ref, 0.5 * (smp.min(1) + smp.max(1)), rtol=1e-6, atol=1e-9)
+    ref = ref.reshape(10, 1, 1, 1)
+    dec = np.ascontiguousarray(m2 << 16).view(np.float32) + ref
     err = np.abs(dec.astype(np.float64) - m0[:10])
     bound = 2.0 ** -8 * 1.0001 * np.abs(m0[:10] - ref) + 2.0 ** -21 * np.abs(m0[:10])
     assert np.all(err <= bound), float((err - bound).max())
     plain = (((m0[:10].astype(np.float32).view(np.uint32) + 0x8000) >> 16) << 16).view(np.float32)
     print("max metric error: deviation form", float(err.max()), "plain bfloat16 (about)",
           float(np.abs(plain.astype(np.float64) - m0[:10]).max()))
-    media = np.ascontiguousarray((m2[10:13] << 16) | m2[13:16]).view(np.float32)
-    np.testing.assert_array_equal(media, m0[10:13].astype(np.float32))
+    np.testing.assert_array_equal(s.get("M_media").reshape((3,) + shp), m0[10:13].astype(np.float32))
     for it in range(40):
         s.step(it)
         o.step(it)
