@@ -57,6 +57,7 @@ def run_against_fixture(name, **overrides):
     report = {
         "case": name, "steps": nt,
         "slip_rel_l2": rel(slip, ref["slip"]),
+        "slip_rel_l2_ruptured_in_both": rel(slip[both], ref["slip"][both]) if both.any() else 0.0,
         "rupture_time_rms_over_duration": float(np.sqrt(np.mean((t0[both] - ref["init_t0"][both]) ** 2)) / dur)
         if both.any() else 0.0,
         "ruptured_nodes": int(both.sum()), "ruptured_set_agreement": float(both.sum() / max(either.sum(), 1)),
