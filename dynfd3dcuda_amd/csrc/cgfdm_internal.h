@@ -183,6 +183,7 @@ struct Ctx {
   int wave_nwork = 0;                  // all work items
   bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
+  float* met_ref_dev = nullptr;     // the same on the device (fault kernels index it at run time)
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
   void* mlp_dev = nullptr;          // MlpDev of the learned Newton guess (NewtonMode 2)

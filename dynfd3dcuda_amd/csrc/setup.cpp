@@ -424,6 +424,8 @@ int host_setup(Ctx& c, std::string& err) {
         c.met_ref[q] = (float)(0.5 * (lo[q] + hi[q]));
         ref.r[q] = c.met_ref[q];
       }
+      c.met_ref_dev = dmalloc<float>(c, 10);
+      ok = c.met_ref_dev && upload(c.met_ref_dev, std::vector<float>(c.met_ref, c.met_ref + 10));
     }
     if (ok) {
       const long total = (long)nx * nz * ny;
