@@ -183,7 +183,10 @@ struct Ctx {
   int wave_nwork = 0;                  // all work items
   bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
-  float* met_ref_dev = nullptr;     // the same on the device (fault kernels index it at run time)
+  // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the
+  // fault kernels read, and that copy's base moved back by its first plane
+  float* met_fault = nullptr;
+  const float* met_fault_base = nullptr;
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
   void* mlp_dev = nullptr;          // MlpDev of the learned Newton guess (NewtonMode 2)

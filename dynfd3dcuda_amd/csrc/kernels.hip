@@ -21,9 +21,7 @@ struct FArgs {
   FaultDev F;
   const float* __restrict__ W;   // stage input wavefield (read)
   float* __restrict__ Wi0;       // wavefield receiving fault2wave
-  const float* __restrict__ M;   // metric (fp32 path; bf16 handled via Mb)
-  const uint16_t* __restrict__ Mb;
-  int bf16;   // 0 FP32 words; 1 bfloat16 words; 2 bfloat16 metric deviations, FP32 media words
+  const float* __restrict__ M;   // FP32 metric and media words (the volume, or the copy around the fault)
   const float* mVx; const float* mVy;
   float* slabH;
   const float* S; float* T;      // for the slab RK
@@ -32,11 +30,10 @@ struct FArgs {
   int ni, nj, nk, nx, ny, nz, i0;
   long nyz;        // words of a 2-D fault array (ny*nz)
   long pyz;        // words of one field of one plane of a volume array (py*nz, padded rows)
-  long wps, mps;   // plane strides of the wave (9 pyz) and metric (13 or 16 pyz) volume arrays
+  long wps, mps;   // plane strides of the wave (9 pyz) and metric (13 pyz) volume arrays
   int py, jpad;    // row pitch and front pad of volume rows
   int NY, NZ;
   float DH, rDH, DT;
-  const float* __restrict__ mref;   // 10 metric reference values on the device (bfloat16-deviation form)
   double DTd;
   int it, irk, FX, FY, FZ;
   int freenode, friction, stress_type, cerjan;
@@ -49,29 +46,16 @@ struct FArgs {
   double newton_tol, mlp_gate;
 };
 
-// Metric and media reads, for the storage form MB (MetricBF16: 0 FP32
-// words, 1 bfloat16 words, 2 bfloat16 metric deviations and FP32 media);
-// a compile-time form keeps the FP32 kernels free of the other forms' code.
-template <int MB>
-__device__ __forceinline__ float LM(const FArgs& a, long i) {
-  if constexpr (MB == 0) return a.M[i];
-  else return __uint_as_float(((unsigned)a.Mb[i]) << 16);
-}
-// Metric word c (0-9) at plane offset p.  In the bfloat16-deviation form it
-// is stored as its deviation from its field's reference value mref[c].
-template <int MB>
-__device__ __forceinline__ float LMF(const FArgs& a, int c, long p) {
-  const long i = c * a.pyz + p;
-  if constexpr (MB == 2) return __uint_as_float(((unsigned)a.Mb[i]) << 16) + a.mref[c];
-  else return LM<MB>(a, i);
-}
-// Media word c (10 lambda, 11 mu, 12 rho) of padded plane x at column pc.  In
-// the bfloat16-deviation form the slots 10-15 of a plane hold the three
-// media words as FP32 planes.
-template <int MB>
+// Metric and media reads of the fault kernels: FP32 words, from the metric
+// volume with FP32 storage, or from the FP32 copy of the planes around the
+// fault that the bfloat16 storage forms keep (a.M is then that copy's base
+// moved back by its first plane, so full-volume plane indices address it).
+__device__ __forceinline__ float LM(const FArgs& a, long i) { return a.M[i]; }
+// Metric word c (0-9) at plane offset p.
+__device__ __forceinline__ float LMF(const FArgs& a, int c, long p) { return a.M[c * a.pyz + p]; }
+// Media word c (10 lambda, 11 mu, 12 rho) of padded plane x at column pc.
 __device__ __forceinline__ float LMEDIA(const FArgs& a, int c, long x, long pc) {
-  if constexpr (MB == 2) return ((const float*)(a.Mb + x * a.mps + 10 * a.pyz))[(c - 10) * a.pyz + pc];
-  else return LM<MB>(a, c * a.pyz + pc + x * a.mps);
+  return a.M[c * a.pyz + pc + x * a.mps];
 }
 __device__ float Gt_func(float t, float T) {  // device.cu:134-143
   if (t < T) {
@@ -87,11 +71,10 @@ __device__ float Fr_func(float r, float R) { return r < R ? expf(r * r / (r * r 
 // (wavefields) or i*mps + c*pyz + jk (metric).  2-D fault arrays keep the
 // reference's [nz][ny] rows and are indexed with pf = j + k*ny.
 // Conservative traction J*(grad xi . sigma) at plane i, volume column jk along metric row 'row'.
-template <int MB>
 __device__ __forceinline__ void trac(const FArgs& a, const float* W, int i, long jk, int row, float* o) {
   const long nyz = a.pyz, pm = jk + i * a.mps, pw = jk + i * a.wps;
-  float e0 = LMF<MB>(a, 3 * row, pm), e1 = LMF<MB>(a, 3 * row + 1, pm), e2 = LMF<MB>(a, 3 * row + 2, pm);
-  float J = LMF<MB>(a, 9, pm);
+  float e0 = LMF(a, 3 * row, pm), e1 = LMF(a, 3 * row + 1, pm), e2 = LMF(a, 3 * row + 2, pm);
+  float J = LMF(a, 9, pm);
   float txx = W[3 * nyz + pw], tyy = W[4 * nyz + pw], tzz = W[5 * nyz + pw];
   float txy = W[6 * nyz + pw], txz = W[7 * nyz + pw], tyz = W[8 * nyz + pw];
   o[0] = J * (e0 * txx + e1 * txy + e2 * txz);
@@ -149,7 +132,6 @@ __device__ int nr_search(double x, double psi, double A, double Vhat, double coe
 // trial_ratestate.cu:800-880): T1 on the fault plane with the optional
 // viscosity term, the stage traction averages, rupture time, then
 // fault2wave (transform.cu:224-403) into the stage input at i0.
-template <int MB>
 __device__ __forceinline__ void fault_tail(const FArgs& a, const FaultDev& F, long q, long pf, long pc,
                                            const float* Tau, float Tau_n, bool ifchange, float ratecheck,
                                            const float* Trial, float jacvec, const float* s1, const float* s2) {
@@ -182,10 +164,10 @@ __device__ __forceinline__ void fault_tail(const FArgs& a, const FaultDev& F, lo
   // ---- fault2wave into the stage input at i0
   {
     const long p = pc + i0 * mps, pw = pc + i0 * wps;
-    float mt[3][3] = {{LMF<MB>(a, 0, p), LMF<MB>(a, 3, p), LMF<MB>(a, 6, p)},
-                      {LMF<MB>(a, 1, p), LMF<MB>(a, 4, p), LMF<MB>(a, 7, p)},
-                      {LMF<MB>(a, 2, p), LMF<MB>(a, 5, p), LMF<MB>(a, 8, p)}};
-    float rj = 1.0f / LMF<MB>(a, 9, p);
+    float mt[3][3] = {{LMF(a, 0, p), LMF(a, 3, p), LMF(a, 6, p)},
+                      {LMF(a, 1, p), LMF(a, 4, p), LMF(a, 7, p)},
+                      {LMF(a, 2, p), LMF(a, 5, p), LMF(a, 8, p)}};
+    float rj = 1.0f / LMF(a, 9, p);
     float inv[3][3];
     inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
     inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
@@ -220,7 +202,6 @@ __device__ __forceinline__ void fault_tail(const FArgs& a, const FaultDev& F, lo
 
 // fault_pre: state_rk (rk4.cu:791-938) + trial_sw (trial_slipweakening.cu:23-732)
 // or trial_rs (trial_ratestate.cu:59-893) + fault2wave (transform.cu:224-403).
-template <int MB>
 __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -280,9 +261,9 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     float jac = 1.f;
     for (int l = 1; l <= 3; l++) {
       const int ii = i0 + (2 * m - 1) * l;
-      jac = LMF<MB>(a, 9, pc + ii * mps);
+      jac = LMF(a, 9, pc + ii * mps);
       float o[3];
-      trac<MB>(a, a.W, ii, pc, 0, o);
+      trac(a, a.W, ii, pc, 0, o);
       long pl = (3 + (2 * m - 1) * l) * nyz + pf;
       F.T11[pl] = o[0]; F.T12[pl] = o[1]; F.T13[pl] = o[2];
     }
@@ -310,9 +291,9 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
   }
   float t = irk == 0 ? a.it * DT : (irk == 3 ? (a.it + 1) * DT : (a.it + 0.5f) * DT);
   long pm = pc + i0 * mps;
-  float n[3] = {LMF<MB>(a, 0, pm), LMF<MB>(a, 1, pm), LMF<MB>(a, 2, pm)};
+  float n[3] = {LMF(a, 0, pm), LMF(a, 1, pm), LMF(a, 2, pm)};
   float n0 = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-  float jacvec = LMF<MB>(a, 9, pm) * n0;
+  float jacvec = LMF(a, 9, pm) * n0;
   for (int c = 0; c < 3; c++) n[c] /= n0;
   float dV[3];
   for (int c = 0; c < 3; c++) dV[c] = F.mW[c * 2 * nyz + nyz + pf] - F.mW[c * 2 * nyz + pf];
@@ -417,7 +398,7 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
     Cf[7 * n2 + q] = (float)log10(fmax(Rn, 1e-30));
     return;
   }
-  fault_tail<MB>(a, F, q, pf, pc, Tau, Tau_n, ifchange, ratecheck, Trial, jacvec, s1, s2);
+  fault_tail(a, F, q, pf, pc, Tau, Tau_n, ifchange, ratecheck, Trial, jacvec, s1, s2);
 }
 
 // Rate-and-state trial after the hand-over from fault_pre_kernel
@@ -427,7 +408,6 @@ __global__ void __launch_bounds__(256) fault_pre_kernel(FArgs a) {
 // trial direction, Tau[0] = str_init_x (trial_ratestate.cu:797) and the
 // shared tail.  Every lane of a wavefront stays active up to the guess, which
 // evaluates the network with matrix-core instructions for the whole wave.
-template <int MB>
 __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -486,7 +466,7 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
   Tau[0] = F.str_init_x[q];  // trial_ratestate.cu:797
   const float Tau_n = Tau[0] * n[0] + Tau[1] * n[1] + Tau[2] * n[2];
   const float Trial[3] = {0.0f, 0.0f, 0.0f};
-  fault_tail<MB>(a, F, q, pf, pc, Tau, Tau_n, true, V, Trial, jacvec, s1, s2);
+  fault_tail(a, F, q, pf, pc, Tau, Tau_n, true, V, Trial, jacvec, s1, s2);
 }
 
 __device__ __forceinline__ void mv3(const float* A, const float* x, float* y) {
@@ -542,7 +522,6 @@ __device__ __forceinline__ float dxv_off(const FArgs& a, int nn, int c, long pc,
 // layer (blockIdx.z): 0-5 the velocity rates of planes i0-3..-1, +1..+3,
 // 6-11 the stress rates of the same planes, 12-13 the split-node velocity
 // rates of sides 0 and 1, 14-15 their stress rates.
-template <int MB>
 __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -569,11 +548,11 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
       float v1[3][7], v2[3][7], v3[3][7];
       for (int l = -3; l <= 3; l++) {
         float o[3];
-        trac<MB>(a, W, i + l, pc, 0, o);
+        trac(a, W, i + l, pc, 0, o);
         for (int c = 0; c < 3; c++) v1[c][l + 3] = o[c];
-        trac<MB>(a, W, i, pc + l, 1, o);
+        trac(a, W, i, pc + l, 1, o);
         for (int c = 0; c < 3; c++) v2[c][l + 3] = o[c];
-        trac<MB>(a, W, i, pc + (long)l * py, 2, o);
+        trac(a, W, i, pc + (long)l * py, 2, o);
         for (int c = 0; c < 3; c++) v3[c][l + 3] = o[c];
       }
       v1[0][nn + 3] = F.T11[3 * nyz + pf];
@@ -592,7 +571,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
           for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
         }
       const long p = pc + i * mps;
-      float rrj = 1.0f / (LMEDIA<MB>(a, 12, i, pc) * LMF<MB>(a, 9, p));
+      float rrj = 1.0f / (LMEDIA(a, 12, i, pc) * LMF(a, 9, p));
       for (int c = 0; c < 3; c++) {
         float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
         slabx(i, c) = (dxt + dyt + dzt) * rrj;
@@ -619,10 +598,10 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
         else if (a.freenode && km == 3) dz[c] = d24(&W[c * pyz + pw], py, FZ) * rDH;
         else dz[c] = drp(&W[c * pyz + pw], py, FZ) * rDH;
       }
-      float lam = LMEDIA<MB>(a, 10, i, pc), mu = LMEDIA<MB>(a, 11, i, pc), l2m = lam + 2.0f * mu;
-      float xi[3] = {LMF<MB>(a, 0, p), LMF<MB>(a, 1, p), LMF<MB>(a, 2, p)};
-      float et[3] = {LMF<MB>(a, 3, p), LMF<MB>(a, 4, p), LMF<MB>(a, 5, p)};
-      float zt[3] = {LMF<MB>(a, 6, p), LMF<MB>(a, 7, p), LMF<MB>(a, 8, p)};
+      float lam = LMEDIA(a, 10, i, pc), mu = LMEDIA(a, 11, i, pc), l2m = lam + 2.0f * mu;
+      float xi[3] = {LMF(a, 0, p), LMF(a, 1, p), LMF(a, 2, p)};
+      float et[3] = {LMF(a, 3, p), LMF(a, 4, p), LMF(a, 5, p)};
+      float zt[3] = {LMF(a, 6, p), LMF(a, 7, p), LMF(a, 8, p)};
       slabx(i, 3) = l2m * xi[0] * dx[0] + lam * xi[1] * dx[1] + lam * xi[2] * dx[2] + l2m * et[0] * dy[0] + lam * et[1] * dy[1] +
                     lam * et[2] * dy[2] + l2m * zt[0] * dz[0] + lam * zt[1] * dz[1] + lam * zt[2] * dz[2];
       slabx(i, 4) = lam * xi[0] * dx[0] + l2m * xi[1] * dx[1] + lam * xi[2] * dx[2] + lam * et[0] * dy[0] + l2m * et[1] * dy[1] +
@@ -638,7 +617,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   if (part < 14) {  // split-node velocity rates of side part-12 (fault_dvelo.cu:260-385)
     const int m = part - 12;
     const float* TT[3] = {F.T11, F.T12, F.T13};
-    float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LMF<MB>(a, 9, pc + i0 * mps));
+    float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LMF(a, 9, pc + i0 * mps));
     for (int c = 0; c < 3; c++) {
       const float* T = TT[c];
       float dxt = m == 0 ? rDH * (A0M * T[3 * nyz + pf] - A1 * T[2 * nyz + pf] - A2 * T[1 * nyz + pf] - A3 * T[0 * nyz + pf])
@@ -730,7 +709,6 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
 
 // fault_post: RK of the fault-slab planes + fault_rk_cu + fault_rk_stage2_cu
 // (rk4.cu:466-789) + fault2wave (transform.cu:224-403) + Cerjan.
-template <int MB>
 __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -788,10 +766,10 @@ __global__ void __launch_bounds__(256) fault_post_kernel(FArgs a) {
   if (uni) return;
   // fault2wave into the stage output at i0
   const long p = pc + i0 * mps, pw = pc + i0 * wps;
-  float mt[3][3] = {{LMF<MB>(a, 0, p), LMF<MB>(a, 3, p), LMF<MB>(a, 6, p)},
-                    {LMF<MB>(a, 1, p), LMF<MB>(a, 4, p), LMF<MB>(a, 7, p)},
-                    {LMF<MB>(a, 2, p), LMF<MB>(a, 5, p), LMF<MB>(a, 8, p)}};
-  float rj = 1.0f / LMF<MB>(a, 9, p);
+  float mt[3][3] = {{LMF(a, 0, p), LMF(a, 3, p), LMF(a, 6, p)},
+                    {LMF(a, 1, p), LMF(a, 4, p), LMF(a, 7, p)},
+                    {LMF(a, 2, p), LMF(a, 5, p), LMF(a, 8, p)}};
+  float rj = 1.0f / LMF(a, 9, p);
   float inv[3][3];
   inv[0][0] = mt[1][1] * mt[2][2] - mt[2][1] * mt[1][2];
   inv[0][1] = mt[2][1] * mt[0][2] - mt[0][1] * mt[2][2];
@@ -933,7 +911,6 @@ __global__ void smooth_ext_kernel(FArgs a, float* __restrict__ Wv) {
 // half-volume Gaussian average of the node's own plane and the conservative
 // tractions J (sigma . grad eta), J (sigma . grad zeta) of the three volume
 // planes on its side.
-template <int MB>
 __global__ void __launch_bounds__(256) smooth_avg_kernel(FArgs a, const float* __restrict__ Wv) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y * blockDim.y + threadIdx.y;
@@ -958,11 +935,11 @@ __global__ void __launch_bounds__(256) smooth_avg_kernel(FArgs a, const float* _
           const long pm = pc + i * a.mps, pw = pc + i * a.wps;
           float E[3][3];  // columns grad xi, grad eta, grad zeta
           for (int r = 0; r < 3; r++)
-            for (int cc = 0; cc < 3; cc++) E[r][cc] = LMF<MB>(a, 3 * cc + r, pm);
+            for (int cc = 0; cc < 3; cc++) E[r][cc] = LMF(a, 3 * cc + r, pm);
           const float sxx = Wv[3 * pyz + pw], syy = Wv[4 * pyz + pw], szz = Wv[5 * pyz + pw];
           const float sxy = Wv[6 * pyz + pw], sxz = Wv[7 * pyz + pw], syz = Wv[8 * pyz + pw];
           const float S[3][3] = {{sxx, sxy, sxz}, {sxy, syy, syz}, {sxz, syz, szz}};
-          const float w = LMF<MB>(a, 9, pm) * wp[(l * 7 + aa) * 7 + b];
+          const float w = LMF(a, 9, pm) * wp[(l * 7 + aa) * 7 + b];
           for (int r = 0; r < 3; r++) {
             const float t1 = S[r][0] * E[0][1] + S[r][1] * E[1][1] + S[r][2] * E[2][1];
             const float t2 = S[r][0] * E[0][2] + S[r][1] * E[1][2] + S[r][2] * E[2][2];
@@ -1000,26 +977,15 @@ __global__ void fill_kernel(float* p, long n, float v) {
   for (; i < n; i += st) p[i] = v;
 }
 
-// Launch kernel template K in the metric storage form of the run.
-#define MB_LAUNCH(ctx, K, grid, block, stream, ...)                                              \
-  do {                                                                                           \
-    if ((ctx).p.MetricBF16 == 2) hipLaunchKernelGGL(K<2>, grid, block, 0, stream, __VA_ARGS__);  \
-    else if ((ctx).p.MetricBF16) hipLaunchKernelGGL(K<1>, grid, block, 0, stream, __VA_ARGS__);  \
-    else hipLaunchKernelGGL(K<0>, grid, block, 0, stream, __VA_ARGS__);                          \
-  } while (0)
-
 FArgs make_fargs(Ctx& c) {
   FArgs a{};
   a.F = c.f;
-  a.bf16 = c.p.MetricBF16;
-  a.mref = c.met_ref_dev;
-  a.M = a.bf16 ? nullptr : (const float*)c.met;
-  a.Mb = a.bf16 ? (const uint16_t*)c.met : nullptr;
+  a.M = c.p.MetricBF16 ? c.met_fault_base : (const float*)c.met;
   a.mVx = c.matVx2Vz; a.mVy = c.matVy2Vz;
   a.slabH = c.slabH;
   for (int x = 0; x < 3; x++) a.cj[x] = c.cj[x];
   a.ni = c.d.ni; a.nj = c.d.nj; a.nk = c.d.nk; a.nx = c.d.nx; a.ny = c.d.ny; a.nz = c.d.nz; a.i0 = c.d.i0;
-  a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = (c.p.MetricBF16 == 2 ? 16 : 13) * c.d.pyz;
+  a.nyz = c.d.nyz; a.pyz = c.d.pyz; a.wps = 9 * c.d.pyz; a.mps = 13 * c.d.pyz;
   a.py = c.d.py; a.jpad = c.d.jpad;
   for (int q = 0; q < 4; q++) a.G[q] = c.p.Fault_grid[q];
   a.mlp = c.p.NewtonMode == 2 ? (const MlpDev*)c.mlp_dev : nullptr;
@@ -1054,14 +1020,14 @@ static FArgs trial_args(Ctx& c, const float* Win, int it, int irk, int fx, int f
 void launch_fault_pre(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz) {
   FArgs a = trial_args(c, Win, it, irk, fx, fy, fz);
   c.prof.begin(c.stream, PK_FAULT_PRE);
-  MB_LAUNCH(c, fault_pre_kernel, fgrid(c), dim3(64, 4, 1), c.stream, a);
+  hipLaunchKernelGGL(fault_pre_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 
 void launch_fault_rs(Ctx& c, const float* Win, int it, int irk, int fx, int fy, int fz) {
   FArgs a = trial_args(c, Win, it, irk, fx, fy, fz);
   c.prof.begin(c.stream, PK_FAULT_RS);
-  MB_LAUNCH(c, fault_rs_kernel, fgrid(c), dim3(64, 4, 1), c.stream, a);
+  hipLaunchKernelGGL(fault_rs_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 
@@ -1072,7 +1038,7 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz) {
   c.prof.begin(c.stream, PK_FAULT_DERIV);
   dim3 g = fgrid(c);
   g.z = 16;  // per-plane parts and the split-node parts
-  MB_LAUNCH(c, fault_deriv_kernel, g, dim3(64, 4, 1), c.stream, a);
+  hipLaunchKernelGGL(fault_deriv_kernel, g, dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 
@@ -1083,7 +1049,7 @@ void launch_fault_post(Ctx& c, const StageBufs& b, int irk) {
   c.prof.begin(c.stream, PK_FAULT_POST);
   dim3 g = fgrid(c);
   g.z = 7;  // six slab planes and the split-node part
-  MB_LAUNCH(c, fault_post_kernel, g, dim3(64, 4, 1), c.stream, a);
+  hipLaunchKernelGGL(fault_post_kernel, g, dim3(64, 4, 1), 0, c.stream, a);
   c.prof.end(c.stream);
 }
 
@@ -1105,7 +1071,7 @@ void launch_fault_smooth(Ctx& c) {
   FArgs a = make_fargs(c);
   if (c.p.FreeSurface)
     hipLaunchKernelGGL(smooth_ext_kernel, dim3((c.d.nj + 255) / 256), dim3(256), 0, c.stream, a, c.cur);
-  MB_LAUNCH(c, smooth_avg_kernel, fgrid(c), dim3(64, 4, 1), c.stream, a, (const float*)c.cur);
+  hipLaunchKernelGGL(smooth_avg_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a, (const float*)c.cur);
   hipLaunchKernelGGL(smooth_blend_kernel, fgrid(c), dim3(64, 4, 1), 0, c.stream, a);
 }
 

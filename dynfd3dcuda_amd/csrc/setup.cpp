@@ -147,27 +147,28 @@ __device__ inline uint16_t metric_store(double v, uint16_t*) {  // bf16, round t
 struct MetRef {
   double r[10];
 };
+// The planes ib .. ib+np-1 are written, plane ib first.
 template <class OT, int NF = 13>
 __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, long pyz, int py, int jpad,
-                                                           unsigned long long* vpmax_bits, MetRef ref) {
+                                                           unsigned long long* vpmax_bits, MetRef ref, int ib, int np) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  const long total = (long)m.nx * m.nz * m.ny;
+  const long total = (long)np * m.nz * m.ny;
   double vp = 0.0;
   if (idx < total) {
     const int j = (int)(idx % m.ny);
     const long r = idx / m.ny;
-    const int k = (int)(r % m.nz), i = (int)(r / m.nz);
+    const int k = (int)(r % m.nz), i = ib + (int)(r / m.nz), io = i - ib;
     double v[13];
     metric_point(m, mirror_idx(i, m.nx), mirror_idx(j, m.ny), mirror_idx(k, m.nz), v);
     const long base = (long)k * py + jpad + j;
     if constexpr (NF == 16) {
       for (int c = 0; c < 10; c++) {
-        out[((long)i * 16 + c) * pyz + base] = metric_store(v[c] - ref.r[c], out);
+        out[((long)io * 16 + c) * pyz + base] = metric_store(v[c] - ref.r[c], out);
       }
-      float* media = reinterpret_cast<float*>(out + ((long)i * 16 + 10) * pyz);
+      float* media = reinterpret_cast<float*>(out + ((long)io * 16 + 10) * pyz);
       for (int c = 10; c < 13; c++) media[(long)(c - 10) * pyz + base] = (float)v[c];
     } else {
-      for (int c = 0; c < 13; c++) out[((long)i * 13 + c) * pyz + base] = metric_store(v[c], out);
+      for (int c = 0; c < 13; c++) out[((long)io * 13 + c) * pyz + base] = metric_store(v[c], out);
     }
     if (i >= 3 && i < m.nx - 3 && j >= 3 && j < m.ny - 3 && k >= 3 && k < m.nz - 3)
       vp = sqrt((v[10] + 2 * v[11]) / v[12]);
@@ -424,20 +425,31 @@ int host_setup(Ctx& c, std::string& err) {
         c.met_ref[q] = (float)(0.5 * (lo[q] + hi[q]));
         ref.r[q] = c.met_ref[q];
       }
-      c.met_ref_dev = dmalloc<float>(c, 10);
-      ok = c.met_ref_dev && upload(c.met_ref_dev, std::vector<float>(c.met_ref, c.met_ref + 10));
     }
     if (ok) {
       const long total = (long)nx * nz * ny;
       const unsigned blocks = (unsigned)((total + 255) / 256);
       if (p.MetricBF16 == 2)
-        hipLaunchKernelGGL((metric_media_kernel<uint16_t, 16>), dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref);
+        hipLaunchKernelGGL((metric_media_kernel<uint16_t, 16>), dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
       else if (p.MetricBF16)
-        hipLaunchKernelGGL(metric_media_kernel<uint16_t>, dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref);
+        hipLaunchKernelGGL(metric_media_kernel<uint16_t>, dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
       else
-        hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks), dim3(256), 0, 0, md, (float*)met, pyz, d.py, d.jpad, dvp, ref);
+        hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks), dim3(256), 0, 0, md, (float*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
+      if (p.MetricBF16 && p.Rupture) {
+        // FP32 metric and media of the planes the fault kernels read (i0-6..i0+6):
+        // the fault normal and traction stay FP32 under bfloat16 storage
+        const int fb = std::max(d.i0 - 6, 0), fe = std::min(d.i0 + 6, nx - 1), fnp = fe - fb + 1;
+        c.met_fault = dmalloc<float>(c, (size_t)fnp * 13 * pyz);
+        if (c.met_fault) {
+          c.met_fault_base = c.met_fault - (long)fb * 13 * pyz;
+          const unsigned fblocks = (unsigned)(((long)fnp * nz * ny + 255) / 256);
+          hipLaunchKernelGGL(metric_media_kernel<float>, dim3(fblocks), dim3(256), 0, 0, md, c.met_fault, pyz, d.py, d.jpad,
+                             dvp, ref, fb, fnp);
+        }
+        ok = c.met_fault != nullptr;
+      }
       unsigned long long bits = 0;
-      ok = hipGetLastError() == hipSuccess && hipMemcpy(&bits, dvp, sizeof bits, hipMemcpyDeviceToHost) == hipSuccess;
+      ok = ok && hipGetLastError() == hipSuccess && hipMemcpy(&bits, dvp, sizeof bits, hipMemcpyDeviceToHost) == hipSuccess;
       double vpm;
       std::memcpy(&vpm, &bits, sizeof vpm);
       c.vpmax = vpm;

@@ -81,35 +81,25 @@ def test_rupture_outputs_match_oracle(name):
 
 
 @pytest.mark.gpu
-def test_rupture_outputs_bf16_metric():
-    """The slip-weakening run with the metric and media stored in BF16 (FP32
-    arithmetic), against the same FP64 fixture.  Each stored word carries up
-    to 2^-8 relative rounding (lambda, mu, rho included, so wave speeds move
-    by up to 0.4%); measured: final slip 7e-3, rupture time 4.7e-3 of the
-    duration, bounds 2e-2 and 1e-2.  (The rate-and-state case, nucleating
-    from creep, shifts its nucleation time under that change of the medium
-    by a tenth of the run, so its final slip is not comparable; the 1e-3
-    targets hold for FP32 storage.)"""
-    report = run_against_fixture("sw_curved", MetricBF16=1)
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("name", sorted(ac.CASES))
+def test_rupture_outputs_bf16_metric(name, mode):
+    """The whole runs with bfloat16 metric storage (FP32 arithmetic), against
+    the same FP64 fixtures and to the same bounds as FP32 storage.
+    MetricBF16=1 stores the 13 metric and media words in bfloat16;
+    MetricBF16=2 stores the metric words as bfloat16 deviations from a
+    reference value per field and the media words in FP32.  In both, the
+    fault kernels read an FP32 copy of the planes around the fault, so the
+    fault normal and the traction on the fault carry no bfloat16 rounding
+    (rounded, the normal tilts by about 1e-4 rad on the curved grids, which
+    the rate-state solve turns into a 20% slip difference after 40 steps).
+    Measured: final slip 4.9e-4 to 7.2e-4 and rupture time 1.1e-4 to
+    7.5e-4 with MetricBF16=1; 1.7e-5 to 3.6e-5 and up to 1.4e-4 with
+    MetricBF16=2 (bound 1e-4 on its final slip)."""
+    report = run_against_fixture(name, MetricBF16=mode)
     assert report["ruptured_nodes"] > 100
-    assert report["slip_rel_l2"] <= 2e-2, report
-    assert report["rupture_time_rms_over_duration"] <= 1e-2, report
-    assert report["ruptured_set_agreement"] >= 0.99, report
-
-
-@pytest.mark.gpu
-@pytest.mark.parametrize("name,slip_tol,time_tol", [("sw_curved", 1e-2, 5e-3), ("tp_curved", 5e-3, 1e-2)])
-def test_rupture_outputs_bf16_metric_deviations(name, slip_tol, time_tol):
-    """MetricBF16=2: the metric words as bfloat16 deviations from a reference
-    value per field, lambda, mu, rho exact (FP32 arithmetic), against the
-    FP64 fixtures.  On these grids the stored metric words are then within
-    1.2e-4 (absolute) of their FP32 values against 3.9e-3 with every word in
-    bfloat16; measured: slip-weakening final slip 4.7e-3 and rupture time
-    2.5e-3 of the duration (against 7e-3 and 4.7e-3 with every word in
-    bfloat16), thermal pressurization 2.3e-3 and 4.7e-3 (against 1.2e-2 and
-    1.3e-2).  Bounds about twice the measured values."""
-    report = run_against_fixture(name, MetricBF16=2)
-    assert report["ruptured_nodes"] > 100
-    assert report["slip_rel_l2"] <= slip_tol, report
-    assert report["rupture_time_rms_over_duration"] <= time_tol, report
-    assert report["ruptured_set_agreement"] >= 0.99, report
+    assert report["slip_rel_l2"] <= (1e-3 if mode == 1 else 1e-4), report
+    assert report["rupture_time_rms_over_duration"] <= 1e-3, report
+    assert report["ruptured_set_agreement"] >= 0.999, report
+    assert report["peak_rate_rel_l2"] <= 1e-2, report
+    assert report["seismogram_rel_l2"] <= 1e-2, report
