@@ -241,12 +241,13 @@ def curved_rs_mlp_400x300x200(seed=2, nt=4000, scale=1, mlp_path=None):
 
 def large_rs_mlp_bf16_1600x1000x500(seed=4, nt=8000):
     """Large, mixed precision: 1600x1000x500 at 25 m, rate-state + learned
-    guess, bf16 metric/media storage."""
+    guess, the metric stored as bfloat16 deviations with FP32 media
+    (MetricBF16=2)."""
     c = curved_rs_mlp_400x300x200(seed=seed, nt=nt)
     rng = np.random.default_rng(seed)
     NX, NY, NZ, DH, DT = 1600, 1000, 500, 25.0, 0.0015
     p = _params(NX, NY, NZ, DH, DT, nt, Friction_type=1, RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12,
-                RS_L=0.02, RS_fw=0.2, smooth_load_T=1.0, MetricBF16=1, NewtonMode=2,
+                RS_L=0.02, RS_fw=0.2, smooth_load_T=1.0, MetricBF16=2, NewtonMode=2,
                 NewtonTol=1e-10, NewtonFallbackIter=20)
     geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
     return {"name": "large_rs_mlp_bf16_1600x1000x500", "params": p, "geom": geom,
