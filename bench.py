@@ -308,7 +308,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the solver runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the whole domain lives in one MI355X's HBM; see the sharding design notes)", file=sys.stderr)
         sys.exit(2)
 
