@@ -76,11 +76,10 @@ FieldRef field(Ctx& c, const std::string& n) {
   if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; r.planes = nx; }
-  else if (n == "M_bf16") {  // MetricBF16 = 2: the 10 metric deviations from M_ref
-    const int nf = c.p.MetricBF16 == 2 ? 16 : 13, nout = c.p.MetricBF16 == 2 ? 10 : 13;
-    r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = (long)nout * nxyz; r.dtype = 3; r.nf = nf; r.planes = nx;
-    r.fn = nout;
+  else if (n == "M_bf16" && c.p.MetricBF16 == 2) {  // 5 words, each two metric deviations from M_ref
+    r.ptr = c.met; r.count = 5 * nxyz; r.dtype = 2; r.nf = 8; r.planes = nx; r.fn = 5;
   }
+  else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; r.nf = 13; r.planes = nx; }
   else if (n == "M_media") {  // MetricBF16 = 2: lambda, mu, rho, the FP32 planes in slots 10-15
     f32(c.p.MetricBF16 == 2 ? c.met : nullptr, 3 * nxyz); r.nf = 8; r.planes = nx; r.f0 = 5; r.fn = 3;
   }

@@ -141,9 +141,10 @@ __device__ inline uint16_t metric_store(double v, uint16_t*) {  // bf16, round t
 // The 13 metric and media words of every stored point (ghosts mirrored), in
 // the device layout, and the largest P-wave speed of the interior as the
 // bits of a positive double.  One thread per point, j fastest.  NF = 16 is
-// the bfloat16-deviation form: fields 0-9 hold the bfloat16 of each metric
-// word less its field's reference value ref.r[c], and the slots 10-15 of a
-// plane hold lambda, mu, rho as three FP32 planes.
+// the bfloat16-deviation form: the bfloat16 of each metric word less its
+// field's reference value ref.r[c], two to a 32-bit word (slots c and c+1,
+// field c in the low half), and the slots 10-15 of a plane hold lambda, mu,
+// rho as three FP32 planes.
 struct MetRef {
   double r[10];
 };
@@ -162,8 +163,9 @@ __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, 
     metric_point(m, mirror_idx(i, m.nx), mirror_idx(j, m.ny), mirror_idx(k, m.nz), v);
     const long base = (long)k * py + jpad + j;
     if constexpr (NF == 16) {
-      for (int c = 0; c < 10; c++) {
-        out[((long)io * 16 + c) * pyz + base] = metric_store(v[c] - ref.r[c], out);
+      for (int c = 0; c < 10; c += 2) {  // two deviations per 32-bit word, field c in the low half
+        const uint32_t lo = metric_store(v[c] - ref.r[c], out), hi = metric_store(v[c + 1] - ref.r[c + 1], out);
+        reinterpret_cast<uint32_t*>(out + ((long)io * 16 + c) * pyz)[base] = lo | (hi << 16);
       }
       float* media = reinterpret_cast<float*>(out + ((long)io * 16 + 10) * pyz);
       for (int c = 10; c < 13; c++) media[(long)(c - 10) * pyz + base] = (float)v[c];

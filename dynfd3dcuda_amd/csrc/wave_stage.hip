@@ -90,11 +90,13 @@ __device__ __forceinline__ float mat_(const float* M, int c, int x, uint32_t jk,
 __device__ __forceinline__ float mat_(const uint16_t* M, int c, int x, uint32_t jk, uint32_t nyz, const float*) {
   return __uint_as_float(((unsigned)M[((size_t)x * 13 + c) * nyz + jk]) << 16);
 }
-// Metric in bfloat16 with the media words exact (16 fields per plane):
-// fields 0-9 hold the bfloat16 of each metric word's deviation from a
-// reference value of its field (ref[c], the midrange of the field over the
-// grid), so the rounding scales with the field's variation rather than its
-// size; the slots 10-15 hold lambda, mu and rho as three FP32 planes.
+// Metric in bfloat16 with the media words exact (16 16-bit slots per
+// plane): the bfloat16 of each metric word's deviation from a reference
+// value of its field (ref[c], the midrange of the field over the grid), so
+// the rounding scales with the field's variation rather than its size,
+// packed two to a 32-bit word (slots 2p and 2p+1 form 32-bit plane p, field
+// 2p in the low half); the slots 10-15 hold lambda, mu and rho as three
+// FP32 planes.
 struct MetBf16Delta {
   uint16_t h;
 };
@@ -104,7 +106,10 @@ constexpr uint32_t met_nf() {
 }
 __device__ __forceinline__ float mat_(const MetBf16Delta* Ms, int c, int x, uint32_t jk, uint32_t nyz, const float* ref) {
   const uint16_t* M = &Ms->h;
-  if (c < 10) return __uint_as_float(((unsigned)M[((size_t)x * 16 + c) * nyz + jk]) << 16) + ref[c];
+  if (c < 10) {
+    const uint32_t u = ((const uint32_t*)(M + ((size_t)x * 16 + (c & ~1)) * nyz))[jk];
+    return __uint_as_float((c & 1) ? (u & 0xffff0000u) : (u << 16)) + ref[c];
+  }
   return ((const float*)(M + ((size_t)x * 16 + 10) * nyz))[(size_t)(c - 10) * nyz + jk];
 }
 // Field c of a metric plane resource r at byte column voff (mpb bytes a
@@ -113,7 +118,8 @@ template <class MT>
 __device__ __forceinline__ float bldmc(Rsrc r, uint32_t voff, int c, uint32_t mpb, float ref) {
   if constexpr (std::is_same<MT, MetBf16Delta>::value) {
     if (c >= 10) return bld<kStream>(r, 2 * voff, (uint32_t)(10 + 2 * (c - 10)) * mpb);  // FP32 word
-    return bldm<MT>(r, voff, c * mpb) + ref;
+    const uint32_t u = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(c & ~1) * mpb));  // pair of c
+    return __uint_as_float((c & 1) ? (u & 0xffff0000u) : (u << 16)) + ref;
   }
   return bldm<MT>(r, voff, c * mpb);
 }

@@ -147,8 +147,9 @@ int cgfdm_fault_smooth(cgfdm_ctx* ctx);
 /* Host copies of named fields (cgfdm_field_info reports each one's element
  * count and type, and rejects an unknown name): "W" (current
  * wavefield, 9 x nx*ny*nz, float), "hW", "M" (13 fields; MetricBF16 = 0),
- * "M_bf16" (MetricBF16 = 1: 13 bfloat16 fields; = 2: the 10 bfloat16 metric
- * deviations, word = deviation + "M_ref"[field]), "M_media" (MetricBF16 = 2:
+ * "M_bf16" (MetricBF16 = 1: 13 bfloat16 fields; = 2: 5 int32 fields, each
+ * two bfloat16 metric deviations, field 2p in the low half of field p, word =
+ * deviation + "M_ref"[field]), "M_media" (MetricBF16 = 2:
  * lambda, mu, rho, float), "M_ref" (10 floats, read only), "FW" (split nodes),
  * fault 2-D outputs "Vs1" "Vs2" "Tn" "Ts1" "Ts2" "slip" "State" "init_t0"
  * "rup_sensor" ..., PML aux "PMLx1".."PMLz2".  dtype: 0 float32, 1 float64,

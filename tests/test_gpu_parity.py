@@ -238,7 +238,8 @@ def test_metric_bf16_delta_storage():
     s, o = make_pair(case)
     d = o.dims()
     shp = (d["nx"], d["nz"], d["ny"])
-    m2 = s.get("M_bf16").astype(np.uint32).reshape((10,) + shp)
+    pairs = s.get("M_bf16").view(np.uint32).reshape((5,) + shp)  # field 2p low, 2p+1 high
+    m2 = np.stack([pairs & 0xffff, pairs >> 16], axis=1).reshape((10,) + shp)
     t = dyn.Solver(dict(case["params"], MetricBF16=0))
     cases.apply_to_solver(case, t)
     t.setup()
