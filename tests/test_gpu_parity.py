@@ -225,9 +225,10 @@ def test_metric_bf16_storage():
 
 def test_metric_bf16_delta_storage():
     """MetricBF16=2 stores the metric words (fields 0-9) as the bfloat16 of
-    their deviation from a reference value per field (the midrange of the
+    their deviation from a reference value per field, two to a 32-bit word
+    (exported as 5 int32 fields, field 2p in the low half; the midrange of the
     field over every 4th stored point along each axis, and the last, in
-    FP32) and lambda, mu, rho as FP32 planes.  Decoded, each metric word lies
+    FP32, is the reference) and lambda, mu, rho as FP32 planes.  Decoded, each metric word lies
     within one bfloat16 rounding of its deviation (2^-8 |m - ref|, plus FP32
     rounding) of the FP32-stored word, and the media words equal the
     FP32-stored ones bit for bit.  A 40-step slip-weakening rupture on a
