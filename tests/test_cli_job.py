@@ -55,9 +55,12 @@ def read_nc(path):
         return {k: v[:].copy() for k, v in f.variables.items()}, dict(f.dimensions)
 
 
-def test_cli_job_directory(tmp_path):
+@pytest.mark.parametrize("metric_bf16", [0, 2])
+def test_cli_job_directory(tmp_path, metric_bf16):
+    """The job as the reference runs it, with the metric in FP32 and in the
+    bfloat16-deviation form (MetricBF16=2)."""
     job = str(tmp_path)
-    p = write_job(job, "OUT")
+    p = write_job(job, "OUT", case=cases.sw_case(curved=True, MetricBF16=metric_bf16))
     r = subprocess.run([dyn.CLI_PATH, "params.json"], cwd=job, capture_output=True, text=True, timeout=300)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
