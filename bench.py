@@ -2,41 +2,48 @@
 """Benchmark of the CGFDM dynamic-rupture time step on one MI355X.
 
 One step = one full RK4 time step of the whole job on the GPU: rupture
-sensor, and per stage the fault trial (fault_pre), the fused wave stage
-(interior / absorbing-band / free-surface instantiations + PML auxiliary
-update), the split-node derivatives (fault_deriv) and the fault RK
-(fault_post).  Inputs are generated with the seeded synthetic generators of
-dynfd3dcuda_amd.configs and are resident in HBM before timing starts.
+sensor, and per stage the fault trial (fault_pre, and for rate and state the
+FP64 Newton solve fault_rs), the fused wave stage (interior / absorbing-band /
+free-surface instantiations + PML auxiliary update), the split-node
+derivatives (fault_deriv) and the fault RK (fault_post).  Inputs are
+generated with the seeded synthetic generators of dynfd3dcuda_amd.configs and
+are resident in HBM before timing starts.
 
-Default workload: the curved-grid slip-weakening configuration, 400 x 300 x
-200 points at 50 m (topography, bent fault, 3-layer medium, 20-layer ADE-PML,
-FP32).  Prints one JSON line:
+Default workload: the large mixed-precision configuration, 1600 x 1000 x 500
+points at 25 m (800 M points): topography, bent fault, 3-layer medium,
+20-layer ADE-PML, rate-and-state aging law with the learned Newton initial
+guess (the committed network trained on four seeds of the 400x300x200
+rate-state workload, none of them the timed one), metric stored as bfloat16
+deviations with FP32 media (MetricBF16=2), FP32 wavefield, FP64 friction and
+slip/state accumulation.  Prints one JSON line:
   value          grid-point updates per second (pt-steps/s) of the whole step
   roofline       the dominant stage kernel (the stage-kernel class with the
                  largest GPU time in an untimed window of PROFILE_STEPS steps
                  with every class timed), timed with HIP events on the
                  solver's stream around its own launches over the timed
                  region; achieved = algorithmic bytes per launch
-                 (points x B_alg/4, B_alg = 768 B/pt-step FP32, 672 B with
-                 BF16 metric) / average launch time; traffic = HBM bytes per
-                 launch from the committed rocprofv3 FETCH/WRITE summary when
-                 present
-  cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, same
-                 workload, a few steps
-  mixed          the same workload with the metric and media stored in BF16
-  mixed_deviation  the same with the metric as BF16 deviations, media exact
-                 (FP32 arithmetic): throughput and the step's fraction of the
-                 672 B/pt-step roofline
+                 (points x B_alg/4; B_alg = 768 B/pt-step FP32, 696 B with
+                 the deviation-form metric) / average launch time; traffic =
+                 HBM bytes per launch from the committed rocprofv3
+                 FETCH/WRITE summary of the same workload; stream ceilings =
+                 a hand-written streaming kernel's GB/s for read:write mixes
+  cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, on the
+                 100 x 1000 x 500 sub-slab around the fault of the same
+                 workload (the full grid needs ~300 GB of host memory in FP64)
+  fp32_metric    the same workload with the metric and media in FP32
+                 (MetricBF16=0): plain vs mixed form on the same GPU
+  side           the 400x300x200 slip-weakening workload: throughput and its
+                 dominant kernel's roofline
   kernel_ms_per_step  every kernel class's GPU time per step, from the
                  untimed window
-  rate_state     rate-and-state fault step: the reference's Newton rule,
-                 plain Newton and the learned initial guess, same grid with
-                 the aging law: the
-                 plain run's Newton samples train the network on the GPU,
-                 then both runs are timed over the same window of steps
-                 inside the rupture (GPU time per RK stage of the Newton
-                 kernel and of the whole fault part, iterations per solve)
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload curved_sw_400x300x200] [--no-rs]
+  rate_state     the rate-state fault step three ways on the 400x300x200
+                 rate-state workload: the reference's Newton rule, plain
+                 Newton, and the learned guess (committed network), timed
+                 over a window from the nucleation ramp through the rupture
+                 front's passage, with the share of solves whose wavefront
+                 evaluated the network, the share that started from its
+                 guess and the per-wavefront largest-iteration histogram
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-rs] ...
 """
 import argparse
 import json
@@ -52,6 +59,10 @@ METRIC = "grid-pt updates/s + HBM roofline %; RS fault step time MLP-Newton vs p
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak
 PROFILE_STEPS = 5      # untimed steps with every kernel class timed (the per-class breakdown)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stage_traffic_{workload}.txt")
+DEFAULT_WORKLOAD = "large_rs_mlp_bf16_1600x1000x500"
+SIDE_WORKLOAD = "curved_sw_400x300x200"
+CPU_SUB_NX = 100       # fault-normal planes of the CPU baseline's sub-slab of the large workload
+NEWTON_MLP = os.path.join(ROOT, "dynfd3dcuda_amd", "data", "newton_guess_4x64x64x1.f32")
 
 
 def b_alg(params):
@@ -68,7 +79,9 @@ def workload_name(name, p):
                                      "20-layer ADE-PML",
             "curved_rs_400x300x200": "curved grid, rate-and-state aging law, plain Newton",
             "curved_rs_mlp_400x300x200": "curved grid, rate-and-state aging law, learned Newton guess",
-            "large_rs_mlp_bf16_1600x1000x500": "large grid, rate-and-state, BF16 metric/media"}
+            "large_rs_mlp_bf16_1600x1000x500": "curved grid + topography, bent fault, 3-layer medium, 20-layer "
+                                               "ADE-PML, rate-and-state aging law with the learned Newton guess, "
+                                               "metric as BF16 deviations + FP32 media"}
     return f"{name}: {p['NX']}x{p['NY']}x{p['NZ']} at {p['DH']:g} m, {what.get(name, '')}"
 
 
@@ -102,8 +115,7 @@ def hip_runtime():
 
 def copy_ceiling(nbytes=1 << 30, reps=10):
     """Device-to-device copy rate (GB/s of bytes read + written) of one
-    buffer of `nbytes`, timed with HIP events: the practical ceiling that a
-    streaming kernel on this GPU can reach."""
+    buffer of `nbytes` with the runtime's copy, timed with HIP events."""
     import ctypes
     hip = hip_runtime()
     if hip is None:
@@ -142,6 +154,16 @@ def copy_ceiling(nbytes=1 << 30, reps=10):
     return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9 if ms.value > 0 else None
 
 
+def stream_ceilings():
+    """GB/s of the library's streaming probe kernel (non-temporal 16-byte
+    loads and stores, 1 GiB arrays) for read:write array mixes."""
+    import dynfd3dcuda_amd as dyn
+    out = {}
+    for nr, nw in ((1, 0), (1, 1), (2, 1), (3, 1)):
+        out[f"{nr}:{nw}"] = dyn.stream_probe(nr, nw)
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -152,10 +174,10 @@ def cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(case, budget_s):
-    """Time the FP64 oracle on the same workload (same grid and inputs) for a
-    few steps on the host cores.  The oracle is the checker of the test suite
-    and is used here only as the reported CPU baseline."""
+def cpu_baseline(case, budget_s, what):
+    """Time the FP64 oracle on `case` for a bounded number of steps on the
+    host cores.  The oracle is the checker of the test suite and is used here
+    only as the reported CPU baseline."""
     from oracle.binding import Oracle
     p = case["params"]
     orc = Oracle(p)
@@ -173,7 +195,7 @@ def cpu_baseline(case, budget_s):
     t0 = time.perf_counter()
     orc.step(0)  # warm-up
     t1 = time.perf_counter() - t0
-    n = int(max(1, min(20, budget_s // max(t1, 1e-3))))
+    n = int(max(1, min(50, budget_s // max(t1, 1e-3))))
     t0 = time.perf_counter()
     for it in range(1, 1 + n):
         orc.step(it)
@@ -182,34 +204,9 @@ def cpu_baseline(case, budget_s):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": npts / dt, "unit": "grid-pt updates/s", "cores": cores, "kind": "port",
             "effective_GBps": npts / dt * b_alg(p) / 1e9,
-            "sample": f"FP64 oracle, full {p['NX']}x{p['NY']}x{p['NZ']} grid of the same workload, {n} steps after 1 "
-                      f"warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), {cpu_model()}"}
-
-
-def mixed_leg(case, steps, warmup, mode=1):
-    """Same workload with MetricBF16=mode (1: metric and media words in
-    bfloat16; 2: metric words as bfloat16 deviations, media words exact):
-    throughput of the whole step."""
-    import dynfd3dcuda_amd as dyn
-    from dynfd3dcuda_amd import configs
-    p = dict(case["params"], MetricBF16=mode)
-    s = dyn.Solver(p)
-    configs.apply(case, s)
-    s.setup()
-    d = s.dims()
-    for it in range(warmup):
-        s.step(it)
-    s.synchronize()
-    t0 = time.perf_counter()
-    for it in range(warmup, warmup + steps):
-        s.step(it)
-    s.synchronize()
-    t = time.perf_counter() - t0
-    s.close()
-    v = d["ni"] * d["nj"] * d["nk"] * steps / t
-    ba = b_alg(p)
-    return {"metric_storage": "bf16" if mode == 1 else "bf16_deviation_exact_media", "value": v, "unit": "grid-pt updates/s", "ms_per_step": 1e3 * t / steps,
-            "B_alg_bytes_per_pt_step": ba, "step_hbm_roofline_frac": v * ba / (PEAK_HBM_GBS * 1e9)}
+            "sample": f"FP64 oracle (FP64 metric and wavefield), {what}: {p['NX']}x{p['NY']}x{p['NZ']} points, "
+                      f"{n} steps after 1 warm-up ({dt:.2f} s/step; setup {t_setup:.1f} s not timed), "
+                      f"{cores} threads, {cpu_model()}"}
 
 
 def overrides(args):
@@ -221,65 +218,130 @@ def overrides(args):
     return out
 
 
-def rate_state_leg(args):
-    """The rate-state solve three ways on the rate-state configuration: the
-    reference's stopping rule (NewtonMode 0: |dx|/|x| < 1e-12, at most 100
-    iterations, trial_ratestate.cu:22-57), plain Newton on the residual rule
-    (NewtonMode 1) and the learned guess (NewtonMode 2, same residual rule
-    and fallback)."""
+def timed_run(case, steps, warmup, workload):
+    """Set up `case`, run `warmup` steps, an untimed window of PROFILE_STEPS
+    steps with every kernel class timed, then `steps` timed steps with HIP
+    events around the dominant stage-kernel class only."""
     import numpy as np
     import dynfd3dcuda_amd as dyn
-    from dynfd3dcuda_amd import configs, mlp
+    from dynfd3dcuda_amd import configs
+    p = case["params"]
+    solver = dyn.Solver(p)
+    configs.apply(case, solver)
+    t0 = time.perf_counter()
+    solver.setup()
+    t_setup = time.perf_counter() - t0
+    d = solver.dims()
+    npts = d["ni"] * d["nj"] * d["nk"]
+    print(f"bench: {workload}: setup of {npts} points in {t_setup:.1f} s", file=sys.stderr, flush=True)
+    for it in range(warmup):
+        solver.step(it)
+    solver.synchronize()
+    it0 = warmup
+    solver.profile(True)
+    for it in range(it0, it0 + PROFILE_STEPS):
+        solver.step(it)
+    solver.synchronize()
+    prof_all = solver.profile_report()
+    solver.profile(False)
+    stage = {k: v for k, v in prof_all.items() if k.startswith("stage_") and v[1] > 0}
+    dom = max(stage, key=lambda k: stage[k][0])
+    it0 += PROFILE_STEPS
+    rs = p.get("Friction_type", 0) >= 1 and p.get("Rupture", 1)
+    if rs:
+        solver.put("newton_counts", np.zeros(5 * d["nj"] * d["nk"], np.int32))
+    solver.profile_classes([dom])
+    solver.profile(True)
+    t0 = time.perf_counter()
+    for it in range(it0, it0 + steps):
+        solver.step(it)
+    solver.synchronize()
+    t = time.perf_counter() - t0
+    prof = solver.profile_report()
+    solver.profile(False)
+    solver.profile_classes(None)
+    newton = None
+    if rs:
+        cnt = solver.get("newton_counts").reshape(5, -1).astype(np.float64).sum(axis=1)
+        newton = {"solves": int(cnt[0]), "iters_per_solve": cnt[1] / max(cnt[0], 1), "fallbacks": int(cnt[2]),
+                  "learned_start_share": cnt[3] / max(cnt[0], 1), "network_wave_share": cnt[4] / max(cnt[0], 1)}
+    finite = bool(np.all(np.isfinite(solver.get("Vs1"))) and np.all(np.isfinite(solver.get("Ts1"))))
+    solver.close()
+    value = npts * steps / t
+    ba = b_alg(p)
+    ms, n, pts = prof[dom]
+    avg_s = ms / n / 1e3
+    achieved = pts * ba / 4.0 / avg_s / 1e9  # one launch = one RK stage of its points
+    return {"value": value, "ms_per_step": 1e3 * t / steps, "npts": npts, "B_alg": ba, "setup_s": t_setup,
+            "step_frac": value * ba / (PEAK_HBM_GBS * 1e9), "dom": dom, "achieved": achieved,
+            "avg_launch_ms": avg_s * 1e3, "points_per_launch": pts, "traffic": read_traffic(workload, dom),
+            "kernel_ms_per_step": {k: v[0] / PROFILE_STEPS for k, v in prof_all.items() if v[1] > 0},
+            "stage_points_per_launch": {k: v[2] for k, v in stage.items()}, "newton": newton,
+            "fault_outputs_finite": finite}
+
+
+def rate_state_leg(args, mlp_path):
+    """The rate-state solve three ways on the 400x300x200 rate-state
+    workload: the reference's stopping rule (NewtonMode 0: |dx|/|x| < 1e-12,
+    at most 100 iterations, trial_ratestate.cu:22-57), plain Newton on the
+    residual rule (NewtonMode 1) and the learned guess (NewtonMode 2, same
+    residual rule and fallback), each timed over the same window of steps."""
+    import numpy as np
+    import dynfd3dcuda_amd as dyn
+    from dynfd3dcuda_amd import configs
     lead, window = args.rs_lead, args.rs_steps
     fault = ("fault_pre", "fault_rs", "fault_deriv", "fault_post")
     res = {}
-    params = None
     for mode in (0, 1, 2):
         case = configs.curved_rs_400x300x200(nt=lead + window + 2)
         case["params"].update({"NewtonMode": mode, "NewtonTol": 1e-10, "NewtonFallbackIter": 20})
+        if mode == 2:
+            case["params"]["NewtonMLP"] = mlp_path
         case["params"].update(overrides(args))
         s = dyn.Solver(case["params"])
         configs.apply(case, s)
-        if mode == 2:
-            s.set_newton_mlp(params)
         s.setup()
         d = s.dims()
-        t0 = time.perf_counter()
-        if mode == 1:
-            F, T = mlp.collect(s, 0, lead, every=args.rs_sample_every, keep=args.rs_samples)
-        else:
-            for it in range(lead):
-                s.step(it)
+        n2 = d["nj"] * d["nk"]
+        for it in range(lead):
+            s.step(it)
         s.synchronize()
-        t_lead = time.perf_counter() - t0
-        s.put("newton_counts", np.zeros(3 * d["nj"] * d["nk"], np.int32))
+        slide0 = int(np.sum(np.hypot(s.get("Vs1"), s.get("Vs2")) > 1e-3))
+        s.put("newton_counts", np.zeros(5 * n2, np.int32))
+        s.put("newton_wave_hist", np.zeros(32, np.int32))
         s.profile(True)
         for it in range(lead, lead + window):
             s.step(it)
         s.synchronize()
         prof = s.profile_report()
         s.profile(False)
-        cnt = s.get("newton_counts").reshape(3, -1).astype(np.float64)
+        cnt = s.get("newton_counts").reshape(5, -1).astype(np.float64)
+        hist = s.get("newton_wave_hist").astype(np.int64)
         nst = 4 * window
         per_node = cnt[1] / np.maximum(cnt[0], 1)
+        solves = max(cnt[0].sum(), 1)
+        nz = np.nonzero(hist)[0]
         res[mode] = {"newton_kernel_ms_per_stage": prof["fault_rs"][0] / nst,
                      "fault_ms_per_stage": sum(prof[k][0] for k in fault) / nst,
-                     "iters_per_solve": cnt[1].sum() / max(cnt[0].sum(), 1),
+                     "iters_per_solve": cnt[1].sum() / solves,
                      "max_node_mean_iters": float(per_node.max()),
                      "fallbacks": int(cnt[2].sum()),
-                     "sliding_nodes": int(np.sum(np.hypot(s.get("Vs1"), s.get("Vs2")) > 1e-3)),
-                     "lead_seconds": round(t_lead, 1)}
-        if mode == 1:
-            t0 = time.perf_counter()
-            params, hist = mlp.train(F, T, epochs=args.rs_epochs, batch=4096, lr=3e-3, seed=0)
-            res["train"] = {"samples": int(len(T)), "epochs": args.rs_epochs, "seconds": round(time.perf_counter() - t0, 2),
-                            "loss_first": float(hist[0]), "loss_last": float(hist[-1])}
-            fault_nodes = d["nj"] * d["nk"]
+                     "learned_start_share": cnt[3].sum() / solves,
+                     "network_wave_share": cnt[4].sum() / solves,
+                     "wave_max_iters_hist": {int(b): int(hist[b]) for b in nz},
+                     "sliding_nodes_first": slide0,
+                     "sliding_nodes_last": int(np.sum(np.hypot(s.get("Vs1"), s.get("Vs2")) > 1e-3))}
         s.close()
-    pl, ml = res[1], res[2]
-    return {"workload": f"curved_rs_400x300x200: the curved 400x300x200 grid with the aging law, {fault_nodes} fault nodes; steps {lead}..{lead + window - 1}",
-            "reference_rule": res[0], "plain": pl, "learned": ml, "train": res["train"],
-            "fault_step_speedup_vs_reference_rule": res[0]["fault_ms_per_stage"] / ml["fault_ms_per_stage"],
+    ref, pl, ml = res[0], res[1], res[2]
+    share = pl["newton_kernel_ms_per_stage"] / pl["fault_ms_per_stage"]
+    return {"workload": f"curved_rs_400x300x200 (seed 2): the curved 400x300x200 grid with the aging law, {n2} fault "
+                        f"nodes; steps {lead}..{lead + window - 1}, from the nucleation ramp through the rupture "
+                        f"front's passage",
+            "network": os.path.relpath(mlp_path, ROOT),
+            "reference_rule": ref, "plain": pl, "learned": ml,
+            "newton_share_of_fault_step": share,
+            "amdahl_bound_fault_step_speedup": 1.0 / (1.0 - share),
+            "fault_step_speedup_vs_reference_rule": ref["fault_ms_per_stage"] / ml["fault_ms_per_stage"],
             "newton_kernel_speedup": pl["newton_kernel_ms_per_stage"] / ml["newton_kernel_ms_per_stage"],
             "fault_step_speedup": pl["fault_ms_per_stage"] / ml["fault_ms_per_stage"]}
 
@@ -287,22 +349,22 @@ def rate_state_leg(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="curved_sw_400x300x200",
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD,
                     help="synthetic workload, one of dynfd3dcuda_amd.configs.WORKLOADS")
     ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--mlp", default=NEWTON_MLP, help="learned-guess network (float32 blob) for NewtonMode 2")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="override a params.json key of the workload (repeatable)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU-baseline timing budget")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0, help="CPU-baseline timing budget")
     ap.add_argument("--no-rs", action="store_true", help="skip the rate-state plain-vs-learned leg")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the BF16-metric leg")
-    ap.add_argument("--rs-lead", type=int, default=600, help="steps before the rate-state timing window")
-    ap.add_argument("--rs-steps", type=int, default=100, help="rate-state timing window (steps)")
-    ap.add_argument("--rs-sample-every", type=int, default=4, help="training-sample stride (steps)")
-    ap.add_argument("--rs-samples", type=int, default=2_000_000, help="training samples kept")
-    ap.add_argument("--rs-epochs", type=int, default=8, help="training epochs")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-metric leg")
+    ap.add_argument("--no-side", action="store_true", help="skip the 400x300x200 slip-weakening leg")
+    ap.add_argument("--no-ceilings", action="store_true", help="skip the copy and streaming-kernel ceilings")
+    ap.add_argument("--rs-lead", type=int, default=200, help="steps before the rate-state timing window")
+    ap.add_argument("--rs-steps", type=int, default=800, help="rate-state timing window (steps)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run a 1 GiB device-to-device copy (counter calibration for rocprofv3)")
     args = ap.parse_args()
@@ -323,7 +385,6 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "copies": 3}))
         return
 
-    import dynfd3dcuda_amd as dyn
     from dynfd3dcuda_amd import configs
 
     if args.workload not in configs.WORKLOADS:
@@ -335,84 +396,78 @@ def main():
     case = gen(**kw)
     p = case["params"]
     p.update(overrides(args))
-    note = None
     if p.get("NewtonMode") == 2 and not p.get("NewtonMLP"):
-        # no network given: the step runs with plain Newton (same residual rule)
-        p["NewtonMode"] = 1
-        note = "NewtonMode 1 (no learned-guess network given)"
-    solver = dyn.Solver(p)
-    configs.apply(case, solver)
-    t_setup = time.perf_counter()
-    solver.setup()
-    d = solver.dims()
-    npts = d["ni"] * d["nj"] * d["nk"]
-    print(f"bench: setup of {npts} points in {time.perf_counter() - t_setup:.1f} s", file=sys.stderr, flush=True)
-
-    for it in range(args.warmup):
-        solver.step(it)
-    solver.synchronize()
-    # an untimed window with every kernel class timed: the per-class
-    # breakdown, and the stage-kernel class with the largest GPU time
-    nprof = PROFILE_STEPS
-    it0 = args.warmup
-    solver.profile(True)
-    for it in range(it0, it0 + nprof):
-        solver.step(it)
-    solver.synchronize()
-    prof_all = solver.profile_report()
-    solver.profile(False)
-    stage = {k: v for k, v in prof_all.items() if k.startswith("stage_") and v[1] > 0}
-    dom = max(stage, key=lambda k: stage[k][0])
-    # the timed region: HIP events around that class's launches only (events
-    # around every launch would add their own packets to each step)
-    it0 += nprof
-    solver.profile_classes([dom])
-    solver.profile(True)
-    t0 = time.perf_counter()
-    for it in range(it0, it0 + args.steps):
-        solver.step(it)
-    solver.synchronize()
-    t = time.perf_counter() - t0
-    prof = solver.profile_report()
-    solver.profile(False)
-    solver.profile_classes(None)
-
-    value = npts * args.steps / t
-    ba = b_alg(p)
-    ms, n, pts = prof[dom]
-    avg_s = ms / n / 1e3
-    bytes_per_launch = pts * ba / 4.0  # one launch = one RK stage of its points
-    achieved = bytes_per_launch / avg_s / 1e9
-    traffic = read_traffic(args.workload, dom)
+        if not os.path.exists(args.mlp):
+            raise SystemExit(f"bench.py: NewtonMode 2 needs the learned-guess network; {args.mlp} is missing")
+        p["NewtonMLP"] = args.mlp
+    r = timed_run(case, args.steps, args.warmup, args.workload)
     out = {
-        "metric": METRIC, "value": value, "unit": "grid-pt updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t / args.steps,
+        "metric": METRIC, "value": r["value"], "unit": "grid-pt updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (seeded generators, dynfd3dcuda_amd.configs)",
-        "config": {"workload": workload_name(args.workload, p) + (f"; {note}" if note else ""),
-                   "grid_points": npts, "B_alg_bytes_per_pt_step": ba,
-                   "step_hbm_roofline_frac": value * ba / (PEAK_HBM_GBS * 1e9),
-                   "stage_points_per_launch": {k: v[2] for k, v in stage.items()}},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
-                     "points_per_launch": pts, "avg_launch_ms": avg_s * 1e3},
-        "kernel_ms_per_step": {k: v[0] / nprof for k, v in prof_all.items() if v[1] > 0},
+        "config": {"workload": workload_name(args.workload, p), "grid_points": r["npts"],
+                   "metric_storage": {0: "fp32", 1: "bf16", 2: "bf16_deviation_fp32_media"}[p.get("MetricBF16", 0)],
+                   "newton": {0: "reference rule", 1: "plain guess, residual rule",
+                              2: "learned guess, residual rule"}.get(p.get("NewtonMode", 0), "")
+                   if p.get("Friction_type", 0) >= 1 else None,
+                   "network": os.path.relpath(p["NewtonMLP"], ROOT) if p.get("NewtonMLP") else None,
+                   "B_alg_bytes_per_pt_step": r["B_alg"], "step_hbm_roofline_frac": r["step_frac"],
+                   "setup_s": round(r["setup_s"], 1), "stage_points_per_launch": r["stage_points_per_launch"],
+                   "newton_in_timed_steps": r["newton"], "fault_outputs_finite": r["fault_outputs_finite"]},
+        "roofline": {"bound": "hbm", "kernel": r["dom"], "achieved": r["achieved"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": r["achieved"] / PEAK_HBM_GBS, "traffic": r["traffic"],
+                     "points_per_launch": r["points_per_launch"], "avg_launch_ms": r["avg_launch_ms"]},
+        "kernel_ms_per_step": r["kernel_ms_per_step"],
         "cpu_baseline": None,
-        "mixed": None,
-        "mixed_deviation": None,
+        "fp32_metric": None,
+        "side": None,
         "rate_state": None,
     }
-    solver.close()
-    ceil = copy_ceiling()
-    out["roofline"]["copy_ceiling_GBps"] = ceil
-    out["roofline"]["frac_of_copy_ceiling"] = achieved / ceil if ceil else None
-    if not args.no_mixed:
-        out["mixed"] = mixed_leg(case, args.steps, args.warmup)
-        out["mixed_deviation"] = mixed_leg(case, args.steps, args.warmup, mode=2)
+    print(f"bench: main line {r['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    if not args.no_ceilings:
+        ceil = copy_ceiling()
+        streams = stream_ceilings()
+        out["roofline"]["copy_ceiling_GBps"] = ceil
+        out["roofline"]["stream_ceilings_GBps"] = streams
+        out["roofline"]["frac_of_read2_write1_ceiling"] = r["achieved"] / streams["2:1"]
+    if not args.no_fp32 and p.get("MetricBF16", 0):
+        c32 = gen(**kw)
+        c32["params"].update(overrides(args))
+        c32["params"].update({"MetricBF16": 0, "NewtonMLP": p.get("NewtonMLP", "")})
+        r32 = timed_run(c32, args.steps, args.warmup, args.workload + "_fp32")
+        out["fp32_metric"] = {"metric_storage": "fp32", "value": r32["value"], "ms_per_step": r32["ms_per_step"],
+                              "B_alg_bytes_per_pt_step": r32["B_alg"], "step_hbm_roofline_frac": r32["step_frac"],
+                              "dominant_kernel": r32["dom"], "dominant_frac": r32["achieved"] / PEAK_HBM_GBS,
+                              "mixed_speedup": r32["ms_per_step"] / r["ms_per_step"]}
+        print(f"bench: fp32-metric leg {r32['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    if not args.no_side and args.workload != SIDE_WORKLOAD:
+        cs = configs.WORKLOADS[SIDE_WORKLOAD](nt=50 + PROFILE_STEPS + 10)
+        rs_ = timed_run(cs, 50, 5, SIDE_WORKLOAD)
+        out["side"] = {"workload": workload_name(SIDE_WORKLOAD, cs["params"]), "value": rs_["value"],
+                       "ms_per_step": rs_["ms_per_step"], "steps": 50, "warmup": 5,
+                       "step_hbm_roofline_frac": rs_["step_frac"],
+                       "roofline": {"kernel": rs_["dom"], "achieved": rs_["achieved"],
+                                    "frac": rs_["achieved"] / PEAK_HBM_GBS, "traffic": rs_["traffic"],
+                                    "points_per_launch": rs_["points_per_launch"],
+                                    "avg_launch_ms": rs_["avg_launch_ms"]},
+                       "kernel_ms_per_step": rs_["kernel_ms_per_step"]}
+        print(f"bench: side leg {rs_['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
     if not args.no_rs:
-        out["rate_state"] = rate_state_leg(args)
+        if not os.path.exists(args.mlp):
+            raise SystemExit(f"bench.py: the rate-state leg needs the learned-guess network; {args.mlp} is missing")
+        out["rate_state"] = rate_state_leg(args, args.mlp)
+        print("bench: rate-state leg done", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_seconds)
+        if args.workload == DEFAULT_WORKLOAD:
+            sub = configs.large_rs_mlp_bf16_1600x1000x500(sub_nx=CPU_SUB_NX, nt=60)
+            what = (f"the {CPU_SUB_NX}x1000x500 sub-slab around the fault of the same workload (same fault, "
+                    f"stress, media and topography rows; 50 M of its 800 M points)")
+        else:
+            sub = gen(**kw)
+            sub["params"].update(overrides(args))
+            what = "the full grid of the same workload"
+        out["cpu_baseline"] = cpu_baseline(sub, args.cpu_seconds, what)
     print(json.dumps(out), flush=True)
 
 

@@ -33,6 +33,7 @@ EXPORTED = [
     "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
     "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
     "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress", "cgfdm_profile_classes",
+    "cgfdm_stream_probe",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
@@ -110,6 +111,8 @@ def load_library(path=None):
     lib.cgfdm_fault_smooth.argtypes = [c_p]
     lib.cgfdm_thermpress.argtypes = [c_p]
     lib.cgfdm_profile_classes.argtypes = [c_p, ctypes.c_char_p]
+    lib.cgfdm_stream_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTED[1:]:
         getattr(lib, name).restype = ctypes.c_int
     if path is None:
@@ -118,6 +121,17 @@ def load_library(path=None):
 
 
 _DT = {0: np.float32, 1: np.float64, 2: np.int32, 3: np.uint16}
+
+
+def stream_probe(nread, nwrite, bytes_per_array=1 << 30, reps=10):
+    """GB/s of a kernel that reads `nread` and writes `nwrite` arrays once
+    each (non-temporal 16-byte accesses; forms 1:0, 1:1, 2:1, 3:1, 2:2)."""
+    lib = load_library()
+    v = ctypes.c_double()
+    rc = lib.cgfdm_stream_probe(nread, nwrite, bytes_per_array, reps, ctypes.byref(v))
+    if rc != 0:
+        raise CgfdmError(rc, lib.cgfdm_last_error().decode())
+    return v.value
 
 
 class Solver:

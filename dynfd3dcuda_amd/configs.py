@@ -239,21 +239,35 @@ def curved_rs_mlp_400x300x200(seed=2, nt=4000, scale=1, mlp_path=None):
     return c
 
 
-def large_rs_mlp_bf16_1600x1000x500(seed=4, nt=8000):
+def large_rs_mlp_bf16_1600x1000x500(seed=4, nt=8000, sub_nx=None, mlp_path=None):
     """Large, mixed precision: 1600x1000x500 at 25 m, rate-state + learned
     guess, the metric stored as bfloat16 deviations with FP32 media
-    (MetricBF16=2)."""
-    c = curved_rs_mlp_400x300x200(seed=seed, nt=nt)
+    (MetricBF16=2).
+
+    sub_nx: keep only the sub_nx fault-normal planes around the fault
+    (NX = sub_nx; the same fault, stress, media and the matching rows of the
+    same topography): the sub-slab on which the FP64 oracle can run, e.g.
+    100x1000x500 (50 M of the 800 M points)."""
     rng = np.random.default_rng(seed)
     NX, NY, NZ, DH, DT = 1600, 1000, 500, 25.0, 0.0015
+    geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
+    topo = sum_of_gaussians_topography(NX, NY, DH, rng)
+    stress = _rs_stress(NY, NZ, DH, rng, frame=fault_frame(geom, DH))
+    name = "large_rs_mlp_bf16_1600x1000x500"
+    if sub_nx is not None:
+        if not (0 < sub_nx <= NX and sub_nx % 2 == 0):
+            raise ValueError("sub_nx must be an even number of planes in 2..1600")
+        lo = NX // 2 - sub_nx // 2
+        topo = topo[lo:lo + sub_nx].copy()
+        NX = sub_nx
+        name = f"{name}_sub{sub_nx}x{NY}x{NZ}"
     p = _params(NX, NY, NZ, DH, DT, nt, Friction_type=1, RS_V0=1e-6, RS_f0=0.6, RS_Vini=1e-12,
                 RS_L=0.02, RS_fw=0.2, smooth_load_T=1.0, MetricBF16=2, NewtonMode=2,
                 NewtonTol=1e-10, NewtonFallbackIter=20)
-    geom = bent_fault_geometry(NY, NZ, DH, 20.0, rng)
-    return {"name": "large_rs_mlp_bf16_1600x1000x500", "params": p, "geom": geom,
-            "media1d": three_layer_media(),
-            "topo": sum_of_gaussians_topography(NX, NY, DH, rng),
-            "stress": _rs_stress(NY, NZ, DH, rng, frame=fault_frame(geom, DH))}
+    if mlp_path:
+        p["NewtonMLP"] = mlp_path
+    return {"name": name, "params": p, "geom": geom, "media1d": three_layer_media(), "topo": topo,
+            "stress": stress}
 
 
 WORKLOADS = {f.__name__: f for f in (flat_sw_100x200x100, curved_sw_400x300x200, curved_rs_400x300x200,

@@ -127,7 +127,10 @@ FieldRef field(Ctx& c, const std::string& n) {
     // rate-state Newton samples of the last stage: features f0..f3 of the
     // learned guess and the solved log10 slip rate, [5][nk][nj]
     else if (n == "RS_samples") f32(F.rs_carryf ? F.rs_carryf + 4 * n2 : nullptr, 5 * n2);
-    else if (n == "newton_counts") i32(F.newton_stats, 3 * n2);  // solves, iterations, fallbacks per node
+    // per node: solves, iterations, fallbacks, learned-guess starts, network evaluations
+    else if (n == "newton_counts") i32(F.newton_stats, 5 * n2);
+    // wavefronts by their largest Newton iteration count (last bin: that many or more)
+    else if (n == "newton_wave_hist") i32(F.newton_stats ? F.newton_stats + 5 * n2 : nullptr, kNewtonHistBins);
     else if (n == "Vs2") f32(F.Vs2, n2);
     else if (n == "hslip") f32(F.hslip, n2);
     else if (n == "slip") f64(F.slip, n2);
@@ -666,6 +669,13 @@ int cgfdm_mlp_train(float* params, const float* feat, const float* target, long 
     return fail(CGFDM_EPARAM, "bad argument");
   std::string err;
   if (mlp_train_host(params, feat, target, n, epochs, batch, lr, seed, loss_hist, err)) return fail(CGFDM_EHIP, err);
+  return 0;
+}
+
+int cgfdm_stream_probe(int nread, int nwrite, size_t bytes_per_array, int reps, double* gbps) {
+  if (!gbps) return fail(CGFDM_EPARAM, "null argument");
+  std::string err;
+  if (stream_probe_host(nread, nwrite, bytes_per_array, reps, gbps, err)) return fail(CGFDM_EPARAM, err);
   return 0;
 }
 

@@ -8,6 +8,10 @@
 
 namespace cgfdm {
 
+// bins of the per-wavefront Newton iteration histogram (the last bin holds
+// counts of that many iterations or more)
+constexpr int kNewtonHistBins = 32;
+
 // ---------------------------------------------------------------------------
 // Parameters: the key set and defaults of get_params (set_params.cu:56-298)
 // plus runtime forms of the reference's compile-time feature macros.
@@ -97,7 +101,10 @@ struct FaultDev {
   double *slip, *mslip, *tslip, *State, *mState, *tState;
   float *hslip, *Vs1, *Vs2, *init_t0, *friction, *rsD_y, *rsD_z, *rup_sensor;
   int *init_t0_flag, *rup_index_y, *rup_index_z, *united, *faultgrid, *flag_rup;
-  unsigned int* newton_stats;        // [3][nk][nj] per-node counts: solves, iterations, fallbacks
+  // [5][nk][nj] per-node counts (solves, iterations, fallbacks, solves started
+  // from the learned guess, solves whose wavefront evaluated the network),
+  // then kNewtonHistBins counts of wavefronts by their largest iteration count
+  unsigned int* newton_stats;
   float *Ws;                         // [9][2][nz][ny] smoothed T2*/T3* (FaultSmooth)
   double *TP_T = nullptr, *TP_P = nullptr;  // Friction_type 3: [TP_n][nk][nj] temperature and pore-pressure rise
   double *TP_hy = nullptr;                  // [nk][nj] hydraulic diffusivity
@@ -222,6 +229,7 @@ int mlp_upload(Ctx& c, const float* blob, long n);  // nonzero on error
 int mlp_forward_host(const float* blob, const float* feat, long n, float* out, int mode, std::string& err);
 int mlp_grad_host(const float* blob, const float* feat, const float* target, long n, float* grad, float* loss,
                   std::string& err);
+int stream_probe_host(int nread, int nwrite, size_t bytes_per_array, int reps, double* gbps, std::string& err);
 int mlp_train_host(float* blob, const float* feat, const float* target, long n, int epochs, int batch, float lr,
                    unsigned seed, float* loss_hist, std::string& err);
 void launch_fill(Ctx& c, float* p, long n, float v);

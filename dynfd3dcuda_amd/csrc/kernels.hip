@@ -420,6 +420,7 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
   const float* Cf = F.rs_carryf;
   const double st = C[q], RS_a = C[n2 + q], Vtn = C[2 * n2 + q], coef = C[3 * n2 + q], x_plain = C[4 * n2 + q];
   double x0 = x_plain;
+  bool consulted = false;
   if (a.mlp) {
     // 2 V0 e^(-psi/a) sinh(x_plain) is the previous slip rate Rn, so the
     // plain guess's residual costs no transcendental.  The network is
@@ -430,12 +431,21 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
     const double f_pln = Rn + coef * x_plain - Vtn;
     const bool poor = valid && fabs(f_pln) > a.mlp_gate * fmax(Vtn, 1e-300);
     if (__any(poor)) {
+      consulted = true;
       float feat[4];
       for (int f = 0; f < 4; f++) feat[f] = valid ? Cf[(4 + f) * n2 + q] : 0.0f;
       const float lv = mlp_guess_wave(a.mlp, feat);  // log10 of the predicted slip rate
       if (poor) x0 = choose_guess(lv, x_plain, f_pln, st, RS_a, Vtn, coef);
     }
   }
+  // Newton on the valid lanes; the wave's largest iteration count goes to a
+  // histogram (one atomic per wave, lane 0) before the other lanes leave
+  double xnew = x0;
+  int fb = 0, iters = 0;
+  if (valid) iters = nr_search(x0, st, RS_a, Vtn, coef, a.newton_mode, a.newton_tol, a.newton_fallback, x_plain, &xnew, &fb);
+  int wmax = iters;
+  for (int off = 32; off > 0; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
+  if (__any(valid) && __lane_id() == 0) atomicAdd(&F.newton_stats[5 * n2 + min(wmax, kNewtonHistBins - 1)], 1u);
   if (!valid) return;
   // C[5] holds the effective normal stress; the traction vector takes the
   // total one (trial_ratestate.cu:785-788)
@@ -448,12 +458,11 @@ __global__ void __launch_bounds__(256) fault_rs_kernel(FArgs a) {
   const long pf = j1 + (long)k1 * a.ny, pv = pf * 3, pc = a.jpad + j1 + (long)k1 * a.py;
   const float s1[3] = {F.vec_s1[pv], F.vec_s1[pv + 1], F.vec_s1[pv + 2]};
   const float s2[3] = {F.vec_s2[pv], F.vec_s2[pv + 1], F.vec_s2[pv + 2]};
-  double xnew;
-  int fb;
-  const int iters = nr_search(x0, st, RS_a, Vtn, coef, a.newton_mode, a.newton_tol, a.newton_fallback, x_plain, &xnew, &fb);
   F.newton_stats[q] += 1u;  // per-node counters: no contention between nodes
   F.newton_stats[n2 + q] += (unsigned)iters;
   F.newton_stats[2 * n2 + q] += fb ? 1u : 0u;
+  F.newton_stats[3 * n2 + q] += x0 != x_plain ? 1u : 0u;  // started from the learned guess
+  F.newton_stats[4 * n2 + q] += consulted ? 1u : 0u;      // its wave evaluated the network
   const double RS_V0 = 1e-6;
   float V = (float)(2.0 * RS_V0 * exp(-st / RS_a) * sinh(xnew));
   const double Tm = RS_a * xnew * fabs(Tau_eff);

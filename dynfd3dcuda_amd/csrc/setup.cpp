@@ -770,7 +770,7 @@ int host_setup(Ctx& c, std::string& err) {
     F.friction = dmalloc<float>(c, n2); F.rsD_y = dmalloc<float>(c, n2); F.rsD_z = dmalloc<float>(c, n2); F.rup_sensor = dmalloc<float>(c, n2);
     F.init_t0_flag = dmalloc<int>(c, n2); F.flag_rup = dmalloc<int>(c, n2);
     UPI(rup_index_y, riy); UPI(rup_index_z, riz); UPI(united, uni); UPI(faultgrid, fg);
-    F.newton_stats = dmalloc<unsigned int>(c, 3 * n2);
+    F.newton_stats = dmalloc<unsigned int>(c, 5 * n2 + kNewtonHistBins);
     if (p.Friction_type >= 1) {
       F.rs_carry = dmalloc<double>(c, 10 * n2);
       F.rs_carryf = dmalloc<float>(c, 9 * n2);

@@ -184,6 +184,14 @@ int cgfdm_profile(cgfdm_ctx* ctx, int enable);
 int cgfdm_profile_classes(cgfdm_ctx* ctx, const char* names);
 int cgfdm_profile_get(cgfdm_ctx* ctx, const char* name, double* total_ms, long* launches, long* points_per_launch);
 
+/* Streaming-bandwidth probe (extension, no counterpart in the reference;
+ * measurement only): a kernel reads `nread` and writes `nwrite` device arrays
+ * of `bytes_per_array` once each with non-temporal 16-byte accesses, `reps`
+ * times; *gbps is (nread + nwrite) * bytes_per_array * reps / time in GB/s.
+ * Forms: 1:0, 1:1, 2:1, 3:1, 2:2.  The 2:1 form has the stage kernels'
+ * read:write mix, so it is their practical HBM ceiling. */
+int cgfdm_stream_probe(int nread, int nwrite, size_t bytes_per_array, int reps, double* gbps);
+
 /* Learned initial guess of the rate-and-state Newton solve (extension, no
  * counterpart in the reference).  With NewtonMode 2 the solve of
  * trial_ratestate.cu:686-702 starts from the better (smaller residual) of the

@@ -41,6 +41,8 @@ def load():
     lib.orc_fault_trial.argtypes = [P] + [ctypes.c_int] * 5
     lib.orc_fault_deriv.argtypes = [P] + [ctypes.c_int] * 3
     lib.orc_rup_sensor.argtypes = [P]
+    lib.orc_state_rk.argtypes = [P, ctypes.c_int]
+    lib.orc_set_mutation.argtypes = [ctypes.c_int]
     lib.orc_thermpress.argtypes = [P]
     lib.orc_smooth.argtypes = [P]
     lib.orc_wave_deriv.argtypes = [P] + [ctypes.c_int] * 3
@@ -56,10 +58,17 @@ def load():
     lib.orc_put.argtypes = [P, ctypes.c_char_p, P, L]
     for n in ["orc_set_param", "orc_set_geometry", "orc_set_coord3d", "orc_set_topography", "orc_set_media1d", "orc_set_media3d",
               "orc_set_stress", "orc_setup", "orc_step", "orc_stage", "orc_rup_sensor", "orc_smooth", "orc_wave_deriv",
-              "orc_pml_deriv", "orc_wave_rk", "orc_apply_cerjan", "orc_nr_search", "orc_get", "orc_put"]:
+              "orc_pml_deriv", "orc_wave_rk", "orc_apply_cerjan", "orc_nr_search", "orc_get", "orc_put",
+              "orc_state_rk", "orc_set_mutation"]:
         getattr(lib, n).restype = ctypes.c_int
     _lib = lib
     return lib
+
+
+def set_mutation(m):
+    """Plant mistake number m in the oracle (0: none).  Pin tests use it to
+    show that their check fails on a plausible error."""
+    load().orc_set_mutation(int(m))
 
 
 class Oracle:
@@ -141,6 +150,9 @@ class Oracle:
 
     def rup_sensor(self):
         self.lib.orc_rup_sensor(self.h)
+
+    def state_rk(self, irk):
+        self.lib.orc_state_rk(self.h, irk)
 
     def thermpress(self):
         self.lib.orc_thermpress(self.h)

@@ -35,6 +35,13 @@ typedef double real;
 
 namespace {
 
+// Planted mistakes for the pin tests (tests/test_oracle_pins*.py).  0 (the
+// default) runs the restatement as written; a test sets a number with
+// orc_set_mutation, checks that its pin then FAILS, and sets it back to 0.
+// Each MUT(n) site below names the mistake it plants.
+int g_mutation = 0;
+inline bool MUT(int n) { return g_mutation == n; }
+
 // DRP/MacCormack coefficients, srcd/macdrp.h:23-40.
 const real c_1 = 0.29164704779069994;
 const real c_2 = 0.66674514216519998;
@@ -531,8 +538,10 @@ struct Oracle {
           if (freenode && k == nk - 1) {
             long ij = (j + 3 + (long)(i + 3) * ny) * 9;
             for (int a = 0; a < 3; a++) {
-              dz[a] = matVx2Vz[ij + 3 * a + 0] * dx[0] + matVx2Vz[ij + 3 * a + 1] * dx[1] + matVx2Vz[ij + 3 * a + 2] * dx[2]
-                    + matVy2Vz[ij + 3 * a + 0] * dy[0] + matVy2Vz[ij + 3 * a + 1] * dy[1] + matVy2Vz[ij + 3 * a + 2] * dy[2];
+              const std::vector<real>& MX = MUT(11) ? matVy2Vz : matVx2Vz;  // MUT(11): maps swapped
+              const std::vector<real>& MY = MUT(11) ? matVx2Vz : matVy2Vz;
+              dz[a] = MX[ij + 3 * a + 0] * dx[0] + MX[ij + 3 * a + 1] * dx[1] + MX[ij + 3 * a + 2] * dx[2]
+                    + MY[ij + 3 * a + 0] * dy[0] + MY[ij + 3 * a + 1] * dy[1] + MY[ij + 3 * a + 2] * dy[2];
             }
           } else if (freenode && k == nk - 2) {
             for (int a = 0; a < 3; a++) dz[a] = rDH * opL22(Vv[a] + p, ny, FZ);
@@ -584,7 +593,7 @@ struct Oracle {
             }
             for (int a = 0; a < 3; a++) {
               v3[a][ik + 3] = 0.0;
-              for (int l = 0; l < 3 - ik; l++) v3[a][6 - l] = -v3[a][l + 2 * ik];
+              for (int l = 0; l < 3 - ik; l++) v3[a][6 - l] = (MUT(10) ? 1.0 : -1.0) * v3[a][l + 2 * ik];  // MUT(10): even image
             }
             for (int a = 0; a < 3; a++) {
               real d1 = FX * rDH * opL(&v1[a][3], FX, 1);
@@ -620,7 +629,7 @@ struct Oracle {
           int e0 = axis * 3;
           real xix = M[(e0 + 0) * nxyz + p], xiy = M[(e0 + 1) * nxyz + p], xiz = M[(e0 + 2) * nxyz + p];
           int li = axis == 0 ? gi : axis == 1 ? gj : gk;
-          real b1 = Bv[li], d1 = Dv[li], ad1 = -(Av[li] + d1);
+          real b1 = Bv[li], d1 = Dv[li], ad1 = -(Av[li] + (MUT(15) ? -d1 : d1));  // MUT(15): sign of d in the decay
           real lam = M[10 * nxyz + p], mu = M[11 * nxyz + p], rrho = 1.0 / M[12 * nxyz + p], l2m = lam + 2 * mu;
           real rb1 = 1.0 / b1;
           real Dr[9];
@@ -649,7 +658,7 @@ struct Oracle {
             Ph[f][3 * ps + pa] += c0 * (l2m * ztx * Dz1[0] + lam * zty * Dz1[1] + lam * ztz * Dz1[2]);
             Ph[f][4 * ps + pa] += c0 * (lam * ztx * Dz1[0] + l2m * zty * Dz1[1] + lam * ztz * Dz1[2]);
             Ph[f][5 * ps + pa] += c0 * (lam * ztx * Dz1[0] + lam * zty * Dz1[1] + l2m * ztz * Dz1[2]);
-            Ph[f][6 * ps + pa] += c0 * mu * (zty * Dz1[0] + ztx * Dz1[1]);
+            Ph[f][6 * ps + pa] += c0 * mu * (MUT(16) ? ztx * Dz1[0] + zty * Dz1[1] : zty * Dz1[0] + ztx * Dz1[1]);  // MUT(16)
             Ph[f][7 * ps + pa] += c0 * mu * (ztz * Dz1[0] + ztx * Dz1[2]);
             Ph[f][8 * ps + pa] += c0 * mu * (ztz * Dz1[1] + zty * Dz1[2]);
           }
@@ -783,7 +792,9 @@ struct Oracle {
           mm3(Tv, Vy_free[0], o); for (int a = 0; a < 9; a++) matP2Mf[2][j * 9 + a] = o[a / 3][a % 3];
           for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) Tv[r][c] = Vx_free[1][r][c];
           inv3(Tv);
-          mm3(Tv, Vx_free[0], o); for (int a = 0; a < 9; a++) matM2Pf[0][j * 9 + a] = o[a / 3][a % 3];
+          if (MUT(18)) mm3(Vx_free[0], Tv, o);  // MUT(18): product in the wrong order
+          else mm3(Tv, Vx_free[0], o);
+          for (int a = 0; a < 9; a++) matM2Pf[0][j * 9 + a] = o[a / 3][a % 3];
           mm3(Tv, Vy_free[0], o); for (int a = 0; a < 9; a++) matM2Pf[1][j * 9 + a] = o[a / 3][a % 3];
           mm3(Tv, Vy_free[1], o); for (int a = 0; a < 9; a++) matM2Pf[2][j * 9 + a] = o[a / 3][a % 3];
         }
@@ -865,8 +876,8 @@ struct Oracle {
         long q = F2(j, k);
         real D1 = rsD_y[q] - rsD_y[q + 1], D2 = rsD_y[q] - rsD_y[q - 1];
         real mg = 0.5 * (D1 * D1 + D2 * D2);
-        D1 = rsD_z[q] - rsD_y[q + nj];  // rup_sensor.cu:75-76 pairs Dz with Dy
-        D2 = rsD_z[q] - rsD_y[q - nj];
+        D1 = rsD_z[q] - (MUT(17) ? rsD_z : rsD_y)[q + nj];  // rup_sensor.cu:75-76 pairs Dz with Dy (MUT(17): Dz with Dz)
+        D2 = rsD_z[q] - (MUT(17) ? rsD_z : rsD_y)[q - nj];
         mg += 0.5 * (D1 * D1 + D2 * D2);
         rup_sensor[q] = mg / (Ts1[q] * Ts1[q] + 1e-16) + 1e-16;
         if (united[q]) rup_sensor[q] = 0;
@@ -892,9 +903,9 @@ struct Oracle {
         real RS_a = Fa[q], RS_b = Fb[q], RS_Vw = FVw[q];
         real RS_f0 = f0, RS_V0 = V0, RS_L = FL[q];
         if (Friction_type == 1) {
-          hState[q] = RS_b * RS_V0 / RS_L * (std::exp((RS_f0 - st) / RS_b) - V / RS_V0);
+          hState[q] = RS_b * RS_V0 / RS_L * (std::exp((RS_f0 - st) / RS_b) + (MUT(13) ? 1.0 : -1.0) * V / RS_V0);  // MUT(13)
         } else if (Friction_type == 2 || Friction_type == 3) {
-          real flv = RS_f0 - (RS_b - RS_a) * std::log(V / RS_V0);
+          real flv = RS_f0 - (MUT(14) ? RS_a - RS_b : RS_b - RS_a) * std::log(V / RS_V0);  // MUT(14): a and b swapped
           real fss = fw + (flv - fw) / std::pow(1. + std::pow(V / RS_Vw, 8), 0.125);
           real psiss = RS_a * (std::log(std::sinh(fss / RS_a)) + std::log(2. * (RS_V0 / V)));
           hState[q] = -V / RS_L * (st - psiss);
@@ -950,7 +961,8 @@ struct Oracle {
       std::vector<real>* TT[3] = {&T11, &T12, &T13};
       for (int a = 0; a < 3; a++) {
         std::vector<real>& T = *TT[a];
-        real r = m == 0 ? a_1 * T[2 * nyz + pf] + a_2 * T[1 * nyz + pf] + a_3 * T[0 * nyz + pf]
+        // MUT(2): a_2 reads plane i0-1 instead of i0-2
+        real r = m == 0 ? a_1 * T[2 * nyz + pf] + a_2 * T[(MUT(2) ? 2 : 1) * nyz + pf] + a_3 * T[0 * nyz + pf]
                         : a_1 * T[4 * nyz + pf] + a_2 * T[5 * nyz + pf] + a_3 * T[6 * nyz + pf];
         R[m][a] = 0.5 * ((2 * m - 1) * r + (DyT[a] + DzT[a]) * DH) * DH2;
       }
@@ -972,7 +984,7 @@ struct Oracle {
       T12[p1] = (Tau[1] - str_init_y[q]) * jacvec;
       T13[p1] = (Tau[2] - str_init_z[q]) * jacvec;
     } else {
-      T11[p1] = Trial[0]; T12[p1] = Trial[1]; T13[p1] = Trial[2];
+      T11[p1] = Trial[0]; T12[p1] = Trial[MUT(19) ? 2 : 1]; T13[p1] = Trial[2];  // MUT(19): component index
     }
     real vis = viscosity * DT;
     real h1 = (T11[p1] - mT11[p1]) / DT, h2 = (T12[p1] - mT12[p1]) / DT, h3 = (T13[p1] - mT13[p1]) / DT;
@@ -1017,7 +1029,8 @@ struct Oracle {
         real dt1 = DT;
         real Trial[3];
         for (int a = 0; a < 3; a++)
-          Trial[a] = -(Mr[0] * Mr[1] * dV[a] / dt1 + Mr[0] * R[1][a] - Mr[1] * R[0][a]) / (DH2 * (Mr[0] * a_0p - a_0m * Mr[1])) * 2.0;
+          Trial[a] = -(Mr[0] * Mr[1] * dV[a] / dt1 + Mr[0] * R[1][a] + (MUT(1) ? 1.0 : -1.0) * Mr[1] * R[0][a]) /
+                     (DH2 * (Mr[0] * a_0p - a_0m * Mr[1])) * 2.0;  // MUT(1): sign of the minus-side term
         long pv = pf * 3;
         long pm = P(i0, j1, k1);
         real n[3] = {M[pm], M[nxyz + pm], M[2 * nxyz + pm]};
@@ -1138,7 +1151,7 @@ struct Oracle {
         friction[q] = RS_a * xnew;
         Tau_n -= Pf;  // back to the total normal stress, trial_ratestate.cu:785-788
         Ttn = std::max(Ttn, 1e-8);
-        real Tau_s1 = Tt[1] / Ttn * T, Tau_s2 = Tt[2] / Ttn * T;
+        real Tau_s1 = (MUT(9) ? Tt[2] : Tt[1]) / Ttn * T, Tau_s2 = Tt[2] / Ttn * T;  // MUT(9): s2 component for s1
         V = std::max(V, 1e-30);
         real Tau[3];
         for (int a = 0; a < 3; a++) Tau[a] = Tau_n * n[a] + Tau_s1 * s1[a] + Tau_s2 * s2[a];
@@ -1167,7 +1180,8 @@ struct Oracle {
         }
         inv3(mt);
         real sg[3][3];
-        mm3(tr, mt, sg);
+        if (MUT(7)) mm3(mt, tr, sg);  // MUT(7): product in the wrong order
+        else mm3(tr, mt, sg);
         for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) sg[a][b] *= rj;
         for (int a = 0; a < 3; a++) W[a * nxyz + p] = (FW[a * 2 * nyz + pf] + FW[a * 2 * nyz + nyz + pf]) * 0.5;
         W[3 * nxyz + p] = sg[0][0];
@@ -1214,7 +1228,7 @@ struct Oracle {
           for (int a = 0; a < 3; a++) {
             real* v = v1[a];
             if (n == 2) v[6] = 2.0 * v[5] - v[4];
-            if (n == 1) { v[5] = 2.0 * v[4] - v[3]; v[6] = 2.0 * v[4] - v[2]; }
+            if (n == 1) { v[5] = MUT(3) ? v[4] : 2.0 * v[4] - v[3]; v[6] = 2.0 * v[4] - v[2]; }  // MUT(3): constant extrapolation
             if (n == -1) { v[0] = 2.0 * v[2] - v[4]; v[1] = 2.0 * v[2] - v[3]; }
             if (n == -2) v[0] = 2.0 * v[1] - v[2];
           }
@@ -1239,7 +1253,8 @@ struct Oracle {
           real DxT[3], DyT[3], DzT[3];
           for (int a = 0; a < 3; a++) {
             std::vector<real>& T = *TT[a];
-            DxT[a] = m == 0 ? rDH * (a_0m * T[3 * nyz + pf] - a_1 * T[2 * nyz + pf] - a_2 * T[1 * nyz + pf] - a_3 * T[0 * nyz + pf])
+            // MUT(4): the minus side takes the plus side's fault coefficient
+            DxT[a] = m == 0 ? rDH * ((MUT(4) ? a_0p : a_0m) * T[3 * nyz + pf] - a_1 * T[2 * nyz + pf] - a_2 * T[1 * nyz + pf] - a_3 * T[0 * nyz + pf])
                             : rDH * (a_0p * T[3 * nyz + pf] + a_1 * T[4 * nyz + pf] + a_2 * T[5 * nyz + pf] + a_3 * T[6 * nyz + pf]);
             real v2[7], v3[7];
             for (int l = -3; l <= 3; l++) {
@@ -1280,7 +1295,7 @@ struct Oracle {
         for (int a = 0; a < 3; a++) {
           if (FX == 1) {
             DxV[7][a] = opLF(wv(a, i0 + 3), nyz) * rDH;
-            DxV[6][a] = opL24F(wv(a, i0 + 2), nyz) * rDH;
+            DxV[6][a] = (MUT(5) ? -1.0 : 1.0) * opL24F(wv(a, i0 + 2), nyz) * rDH;  // MUT(5): sign
             DxV[5][a] = (wv(a, i0 + 1)[nyz] - wv(a, i0 + 1)[0]) * rDH;
             DxV[4][a] = (wv(a, i0 + 1)[0] - fv(a, 1)[0]) * rDH;
             // minus side uses the split-node value in place of i0
@@ -1311,7 +1326,7 @@ struct Oracle {
           real o1[3], o2[3], o3[3];
           if (FX == 1) {
             mv(&matP2Mf[0][j1 * 9], DxV[4], o1); mv(&matP2Mf[1][j1 * 9], DyV[4], o2); mv(&matP2Mf[2][j1 * 9], DyV[3], o3);
-            for (int a = 0; a < 3; a++) DxV[3][a] = o1[a] + o2[a] - o3[a];
+            for (int a = 0; a < 3; a++) DxV[3][a] = o1[a] + o2[a] + (MUT(12) ? 1.0 : -1.0) * o3[a];  // MUT(12): sign
           } else {
             mv(&matM2Pf[0][j1 * 9], DxV[3], o1); mv(&matM2Pf[1][j1 * 9], DyV[3], o2); mv(&matM2Pf[2][j1 * 9], DyV[4], o3);
             for (int a = 0; a < 3; a++) DxV[4][a] = o1[a] + o2[a] - o3[a];
@@ -1345,7 +1360,7 @@ struct Oracle {
         for (int m = 0; m < 2; m++) {
           for (int row = 0; row < 2; row++) {
             const real* A = &D[m][row + 1][0][pidx];
-            const real* B = &D[m][row + 1][1][pidx];
+            const real* B = &D[m][row + 1][MUT(6) ? 2 : 1][pidx];  // MUT(6): DyV takes D_a3
             const real* Cc = &D[m][row + 1][2][pidx];
             for (int a = 0; a < 3; a++) {
               real v = A[3 * a] * DxV[3 + m][0] + A[3 * a + 1] * DxV[3 + m][1] + A[3 * a + 2] * DxV[3 + m][2]
@@ -1417,7 +1432,7 @@ struct Oracle {
         }
       }
     // stage2 runs on every node, united or not (rk4.cu:519-567)
-    real rkb1 = rkb / DT;
+    real rkb1 = (MUT(8) ? rka : rkb) / DT;  // MUT(8): stage weight of the wavefield update
     for (int k = 0; k < nk; k++)
       for (int j = 0; j < nj; j++) {
         long q = F2(j, k);
@@ -1651,7 +1666,7 @@ struct Oracle {
     if (n == "Tn") return &Tn; if (n == "Ts1") return &Ts1; if (n == "Ts2") return &Ts2;
     if (n == "tTn") return &tTn; if (n == "tTs1") return &tTs1; if (n == "tTs2") return &tTs2;
     if (n == "Vs1") return &Vs1; if (n == "Vs2") return &Vs2; if (n == "slip") return &slip;
-    if (n == "hslip") return &hslip; if (n == "State") return &State; if (n == "init_t0") return &init_t0;
+    if (n == "hslip") return &hslip; if (n == "State") return &State; if (n == "hState") return &hState; if (n == "init_t0") return &init_t0;
     if (n == "friction") return &friction; if (n == "rup_sensor") return &rup_sensor;
     if (n == "TP_T") return &TP_T; if (n == "TP_P") return &TP_P; if (n == "TP_hy") return &TP_hy;
     if (n == "str_init_x") return &str_init_x; if (n == "str_init_y") return &str_init_y; if (n == "str_init_z") return &str_init_z;
@@ -1737,6 +1752,8 @@ int orc_fault_deriv(void* h, int fx, int fy, int fz) {
   return 0;
 }
 int orc_rup_sensor(void* h) { ((Oracle*)h)->rup_sensor_calc(); return 0; }
+int orc_state_rk(void* h, int irk) { ((Oracle*)h)->state_rk(irk); return 0; }
+int orc_set_mutation(int m) { g_mutation = m; return 0; }
 int orc_thermpress(void* h) { ((Oracle*)h)->thermpress(); return 0; }
 int orc_smooth(void* h) { ((Oracle*)h)->smooth_gauss(); return 0; }
 // kernel-level entry points for unit parity tests
