@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libcgfdm.so")
 CLI_PATH = os.path.join(HERE, "cgfdm")
 
 # error codes (include/cgfdm.h)
-EPARAM, EFILE, EJSON, ENETCDF, ECFL, EHIP, ESTATE, ESIZE = range(1, 9)
+EPARAM, EFILE, EJSON, ENETCDF, ECFL, EHIP, ESTATE, ESIZE, ENONFINITE = range(1, 10)
 
 # FD operator direction table (srcd/main.cu:86-94); negated after each stage
 FD_FLAGS = [(-1, -1, -1), (1, 1, -1), (1, 1, 1), (-1, -1, 1),
@@ -33,7 +33,7 @@ EXPORTED = [
     "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
     "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
     "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress", "cgfdm_profile_classes",
-    "cgfdm_stream_probe",
+    "cgfdm_stream_probe", "cgfdm_check_field", "cgfdm_print_params",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
@@ -111,6 +111,9 @@ def load_library(path=None):
     lib.cgfdm_fault_smooth.argtypes = [c_p]
     lib.cgfdm_thermpress.argtypes = [c_p]
     lib.cgfdm_profile_classes.argtypes = [c_p, ctypes.c_char_p]
+    lib.cgfdm_check_field.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong),
+                                      ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
+    lib.cgfdm_print_params.argtypes = [c_p]
     lib.cgfdm_stream_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTED[1:]:
@@ -307,6 +310,15 @@ class Solver:
 
     def close_outputs(self):
         self._check(self.lib.cgfdm_close_outputs(self.h))
+
+    def check_field(self, name):
+        """Device-side check of an FP32 field: {"nonfinite", "ghost_nonzero", "maxabs"}."""
+        a, b, m = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        self._check(self.lib.cgfdm_check_field(self.h, name.encode(), ctypes.byref(a), ctypes.byref(b), ctypes.byref(m)))
+        return {"nonfinite": a.value, "ghost_nonzero": b.value, "maxabs": m.value}
+
+    def print_params(self):
+        self._check(self.lib.cgfdm_print_params(self.h))
 
     def set_newton_mlp(self, params):
         """Network for NewtonMode 2 (float32 blob of 4555 values)."""

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["params.cpp", "setup.cpp", "netcdf_classic.cpp", "kernels.hip", "wave_stage.hip", "mlp_train.hip", "stream_probe.hip", "capi.cpp"]
+SOURCES = ["params.cpp", "setup.cpp", "netcdf_classic.cpp", "kernels.hip", "wave_stage.hip", "mlp_train.hip", "stream_probe.hip", "field_check.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled for all device code
 DEVFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -5,6 +5,7 @@
 #include "cgfdm_internal.h"
 #include "mlp.h"
 #include "netcdf_classic.h"
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -672,6 +673,58 @@ int cgfdm_mlp_train(float* params, const float* feat, const float* target, long 
   return 0;
 }
 
+int cgfdm_check_field(cgfdm_ctx* h, const char* name, long long* nonfinite, long long* ghost_nonzero, double* maxabs) {
+  if (!h || !name) return fail(CGFDM_EPARAM, "null argument");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "before setup");
+  FieldRef r = field(c, name);
+  if (!r.ptr || r.host) return fail(CGFDM_EPARAM, std::string("unknown or unavailable field ") + name);
+  if (r.dtype != 0) return fail(CGFDM_EPARAM, std::string("not an FP32 field: ") + name);
+  std::string err;
+  const int fn = r.fn ? r.fn : r.nf;
+  const int rc = r.nf ? field_check_host(c, (const float*)r.ptr, r.nf, r.f0, fn, 0, nonfinite, ghost_nonzero, maxabs, err)
+                      : field_check_host(c, (const float*)r.ptr, 0, 0, 1, (long)r.count, nonfinite, ghost_nonzero, maxabs, err);
+  return rc ? fail(CGFDM_EHIP, err) : 0;
+}
+
+int cgfdm_print_params(cgfdm_ctx* h) {
+  if (!h) return fail(CGFDM_EPARAM, "null context");
+  const Params& p = h->c.p;
+  std::printf("# Parameters setting:\n"
+              "* Tmax           = %g (sec)\n"
+              "* DT             = %g (sec)\n"
+              "* DH             = %g (m)\n"
+              "* NX, NY, NZ, NT = %d %d %d %d\n"
+              "* PX, PY, PZ     = %d %d %d\n"
+              "* Fault_grid     = %d %d %d %d\n"
+              "* Asp_grid       = %d %d %d %d\n"
+              "* vp             = %g/%g\n"
+              "* vs             = %g/%g\n"
+              "* rho            = %g/%g\n"
+              "* PML_N          = %d\n"
+              "* PML_velocity   = %g\n"
+              "* PML_fc         = %g\n"
+              "* PML_bmax       = %g\n"
+              "* RupThres       = %g\n\n",
+              p.TMAX, p.DT, p.DH, p.NX, p.NY, p.NZ, p.NT, p.PX, p.PY, p.PZ, p.Fault_grid[0], p.Fault_grid[1],
+              p.Fault_grid[2], p.Fault_grid[3], p.Asp_grid[0], p.Asp_grid[1], p.Asp_grid[2], p.Asp_grid[3], p.bi_vp1,
+              p.bi_vp2, p.bi_vs1, p.bi_vs2, p.bi_rho1, p.bi_rho2, p.PML_N, p.PML_velocity, p.PML_fc, p.PML_bmax,
+              p.RupThres);
+  std::printf("# Friction type: %d\n", p.Friction_type);
+  if (p.Friction_type == 0)
+    std::printf("* Dc             = %g (m)\n* C0             = %g (Pa)\n* mu_s           = %g\n* mu_d           = %g\n\n",
+                p.Dc, p.C0, p.mu_s, p.mu_d);
+  else if (p.Friction_type >= 1 && p.Friction_type <= 3)
+    std::printf("* RS_V0          = %g (m/s)\n* RS_Vini        = %g (m/s)\n* RS_f0          = %g\n* RS_fw          = %g\n"
+                "* RS_L           = %g (m)\n* TP_n           = %d\n\n",
+                p.V0, p.Vini, p.f0, p.fw, p.L, p.TP_n);
+  std::printf("* smooth_load_T   = %g (s)\n", p.smooth_load_T);
+  if (!p.Media1D.empty()) std::printf("Media1D = %s\n", p.Media1D.c_str());
+  if (!p.Media3D.empty()) std::printf("Media3D = %s\n", p.Media3D.c_str());
+  std::fflush(stdout);
+  return 0;
+}
+
 int cgfdm_stream_probe(int nread, int nwrite, size_t bytes_per_array, int reps, double* gbps) {
   if (!gbps) return fail(CGFDM_EPARAM, "null argument");
   std::string err;
@@ -790,6 +843,7 @@ static void d2h(void* dst, const void* src, size_t bytes, hipError_t* e) {
 
 int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
   hipError_t herr = hipSuccess;  // first failed copy, reported after the records
+  std::string nonfinite;         // first NaN/Inf of the fault record, reported after it
   if (!h) return fail(CGFDM_EPARAM, "null context");
   Ctx& c = h->c;
   if (!c.out) {
@@ -820,6 +874,14 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
         d2h(buf.data(), r.ptr, n2 * 4, &herr);
       }
       o->fault.put_record(o->fv[v], rec, buf.data());
+      // the first non-finite word of the record's fields (reported after the
+      // record is written: the file then shows where the run went wrong)
+      if (nonfinite.empty() && herr == hipSuccess)
+        for (long q = 0; q < n2; q++)
+          if (!std::isfinite(buf[q])) {
+            nonfinite = std::string(names[v]) + " at node j=" + std::to_string(q % nj) + " k=" + std::to_string(q / nj);
+            break;
+          }
     }
     d2h(buf.data(), c.f.init_t0, n2 * 4, &herr);
     o->fault.put_var(o->f_init_t0, buf.data());
@@ -877,6 +939,9 @@ int cgfdm_write_outputs(cgfdm_ctx* h, int it) {
   put_slice(o->xz, o->wxz, 1, jy, ni, nk);
   put_slice(o->yz, o->wyz, 0, ix, nk, nj);
   if (herr != hipSuccess) return fail(CGFDM_EHIP, std::string("output copy failed: ") + hipGetErrorString(herr));
+  if (!nonfinite.empty())
+    return fail(CGFDM_ENONFINITE, "non-finite value in fault record " + std::to_string(rec) + " (step " +
+                                      std::to_string(it) + "): " + nonfinite + "; the run has become unstable");
   return 0;
 }
 

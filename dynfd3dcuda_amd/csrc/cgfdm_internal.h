@@ -48,6 +48,7 @@ struct Params {
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
   int StageStreams = 0;  // 1: interior and free-surface launches of a stage on side streams
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
+  int WaveNarrowInterior = 0;  // 1: interior tiles 32 x 8 in blocks of 256 threads
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
@@ -189,6 +190,7 @@ struct Ctx {
   float* img_top = nullptr;     // free surface: image velocity RHS of the three top rows [3][3][ni][nj]
   int wave_nwork = 0;                  // all work items
   bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
+  bool wave_narrow_in = false;      // interior tiles 32 x 8 (WaveNarrowInterior)
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the
   // fault kernels read, and that copy's base moved back by its first plane
@@ -201,6 +203,7 @@ struct Ctx {
   Profiler prof;
   PmlDev pml{};
   FaultDev f{};
+  unsigned long long* check_counts = nullptr;  // counters of cgfdm_check_field (3 words)
   std::vector<void*> allocations;
   // host copies kept from setup
   std::vector<double> hostC;    // coordinates [3][nxyz] (used for outputs)
@@ -229,6 +232,8 @@ int mlp_upload(Ctx& c, const float* blob, long n);  // nonzero on error
 int mlp_forward_host(const float* blob, const float* feat, long n, float* out, int mode, std::string& err);
 int mlp_grad_host(const float* blob, const float* feat, const float* target, long n, float* grad, float* loss,
                   std::string& err);
+int field_check_host(Ctx& c, const float* ptr, int nf, int f0, int fn, long flat_n, long long* nonfinite,
+                     long long* ghost_nonzero, double* maxabs, std::string& err);
 int stream_probe_host(int nread, int nwrite, size_t bytes_per_array, int reps, double* gbps, std::string& err);
 int mlp_train_host(float* blob, const float* feat, const float* target, long n, int epochs, int batch, float lr,
                    unsigned seed, float* loss_hist, std::string& err);

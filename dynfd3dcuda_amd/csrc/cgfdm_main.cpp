@@ -2,7 +2,10 @@
 // progress output (srcd/main.cu:123-580, srcd/set_params.cu:108-132):
 //   cgfdm [params.json]
 // exit -1 (255) for params file / JSON errors, 2 for netCDF errors, 110 for
-// the CFL check; prints "> %6d, total %d, %8.4lf s" every 10 steps.
+// the CFL check; prints the parameter block (set_params.cu:299-388, after
+// reading params.json as main.cu:156 does) and "> %6d, total %d, %8.4lf s"
+// every 10 steps.  A run whose fault record turns non-finite stops with
+// exit 1 after writing that record (the reference writes on silently).
 #include "../../include/cgfdm.h"
 #include <chrono>
 #include <cstdio>
@@ -17,6 +20,7 @@ int main(int argc, char** argv) {
     std::printf("%s\n", cgfdm_last_error());
     return -1;
   }
+  cgfdm_print_params(ctx);
   rc = cgfdm_setup(ctx);
   if (rc) {
     std::printf("%s\n", cgfdm_last_error());

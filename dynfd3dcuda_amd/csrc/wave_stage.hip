@@ -652,6 +652,8 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     pr.begin(si, PK_STAGE_INTERIOR);
     if (c.wave_tall)
       hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0, TJ, 2 * NTHR>), dim3(a.nwork), dim3(TJ, 2 * TK, 1), 0, si, a);
+    else if (c.wave_narrow_in)
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0, TJN, NTHRN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, si, a);
     else
       hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, si, a);
     pr.end(si);
@@ -841,6 +843,10 @@ int upload_wave_cold(Ctx& c) {
   // threads, which re-read the k halo rows for twice as many points)
   const bool tall = p.WaveTallInterior != 0;
   c.wave_tall = tall;
+  // WaveNarrowInterior: interior tiles of TJN x TKN in blocks of NTHRN
+  // threads, four blocks (barrier groups) per CU instead of two
+  const bool narrow_in = p.WaveNarrowInterior != 0 && !tall;
+  c.wave_narrow_in = narrow_in;
   for (int ax = 1; ax < 3; ax++) {
     const int t = ax == 1 ? TJ : (tall ? 2 * TK : TK), e = n3[ax] - lo[ax] - hi[ax];
     if (e > t) (ax == 2 && surf ? lo[ax] : hi[ax]) += e % t;
@@ -848,6 +854,7 @@ int upload_wave_cold(Ctx& c) {
   const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
   const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
   const std::vector<Seg> ski = tall ? axis_segments(n3[2], lo[2], hi[2], 2 * TK) : sk;  // interior tiles' rows
+  const std::vector<Seg> sji = narrow_in ? axis_segments(n3[1], lo[1], hi[1], TJN) : sj;  // and columns
   const int nk = n3[2];
   auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
   std::vector<Seg> sjt = sj;  // j tiles of the free-surface rows
@@ -864,7 +871,7 @@ int upload_wave_cold(Ctx& c) {
     if (is_top(b)) tiles[2] += (int)sjt.size();
   int tiles_in = 0;  // interior blocks per chunk
   for (const Seg& b : ski)
-    for (const Seg& a : sj) tiles_in += a.in && b.in;
+    for (const Seg& a : sji) tiles_in += a.in && b.in;
   const int ni = n3[0], ia = std::min(lo[0], ni), ib = std::max(ia, ni - hi[0]);
   // planes per block of each kind: short chunks (several thousand blocks per
   // launch) balance the CUs better than the longer chunks would save on the
@@ -904,7 +911,7 @@ int upload_wave_cold(Ctx& c) {
   for (const Seg& b : sk)
     if (is_top(b)) ktop = std::min(ktop, b.s);
   for (int i0 = ia; i0 < ib; i0 += ich_in)
-    for (const Seg& b : ski) for (const Seg& a : sj) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
+    for (const Seg& b : ski) for (const Seg& a : sji) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
   for (int i0 = 0; i0 < ni; i0 += ich_band)
     for (const Seg& b : sk)
       for (const Seg& a : sj) {

@@ -32,7 +32,9 @@ enum {
   CGFDM_ECFL = 5,      /* DT >= 1.3 hmin / vpmax (ref: MPI_Abort 110)        */
   CGFDM_EHIP = 6,      /* HIP runtime / allocation failure                   */
   CGFDM_ESTATE = 7,    /* call out of order (e.g. step before setup)         */
-  CGFDM_ESIZE = 8      /* buffer size does not match the field               */
+  CGFDM_ESIZE = 8,     /* buffer size does not match the field               */
+  CGFDM_ENONFINITE = 9 /* NaN or Inf in an output record (extension: the
+                          reference writes such records without notice)      */
 };
 
 /* Message of the last error on this thread (never NULL). */
@@ -183,6 +185,19 @@ int cgfdm_profile(cgfdm_ctx* ctx, int enable);
  * "" for all, the default); the other launches then carry no events. */
 int cgfdm_profile_classes(cgfdm_ctx* ctx, const char* names);
 int cgfdm_profile_get(cgfdm_ctx* ctx, const char* name, double* total_ms, long* launches, long* points_per_launch);
+
+/* Device-side check of an FP32 field (extension, no counterpart in the
+ * reference): *nonfinite = NaN/Inf words, *ghost_nonzero = nonzero words in
+ * the three ghost layers of a volume field ("W", "hW"; 0 for 2-D and fault
+ * fields, which have no ghost layers in their layout here or are checked
+ * whole), *maxabs = largest finite magnitude of the interior.  Reads the
+ * field once on the device; no host copy of the volume. */
+int cgfdm_check_field(cgfdm_ctx* ctx, const char* name, long long* nonfinite, long long* ghost_nonzero, double* maxabs);
+
+/* Print the parameter block of the reference's print_params
+ * (set_params.cu:299-388) to stdout, as the reference does after reading
+ * params.json (main.cu:156). */
+int cgfdm_print_params(cgfdm_ctx* ctx);
 
 /* Streaming-bandwidth probe (extension, no counterpart in the reference;
  * measurement only): a kernel reads `nread` and writes `nwrite` device arrays

@@ -160,3 +160,33 @@ def test_stage_flags_table():
             assert tuple(x.value for x in f) == tuple(sign * v for v in table[it % 8])
     assert L.cgfdm_stage_flags(0, 4, *[ctypes.byref(x) for x in f]) == dyn.EPARAM
     assert L.cgfdm_stage_flags(-1, 0, *[ctypes.byref(x) for x in f]) == dyn.EPARAM
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "params_*.json"))))
+def test_print_params_block(path, capfd):
+    """cgfdm_print_params writes the reference's parameter block
+    (set_params.cu:299-388) for each of the reference's job files: the
+    values it read, in the reference's line format."""
+    js = json.loads(open(path).read())
+    rc, h = create(open(path).read())
+    assert rc == 0
+    try:
+        capfd.readouterr()
+        assert lib().cgfdm_print_params(h) == 0
+        out = capfd.readouterr().out
+    finally:
+        lib().cgfdm_destroy(h)
+    g = lambda x: "%g" % x  # noqa: E731
+    nt = int(np.float32(np.float32(js["TMAX"]) / np.float32(js["DT"])))
+    lines = out.splitlines()
+    assert lines[0] == "# Parameters setting:"
+    assert "* Tmax           = %s (sec)" % g(js["TMAX"]) in lines
+    assert "* NX, NY, NZ, NT = %d %d %d %d" % (js["NX"], js["NY"], js["NZ"], nt) in lines
+    assert "* Fault_grid     = %d %d %d %d" % tuple(js["Fault_grid"]) in lines
+    assert "* PML_N          = %d" % js.get("PML_N", 20) in lines
+    ft = js.get("Friction_type", 0)
+    assert "# Friction type: %d" % ft in lines
+    if ft == 0:
+        assert "* mu_s           = %s" % g(js["mu_s"]) in lines
+        assert "* Dc             = %s (m)" % g(js["Dc"]) in lines
+    assert lines[-1].startswith("* smooth_load_T") or lines[-1].startswith("Media")
