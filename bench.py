@@ -358,7 +358,7 @@ def main():
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="override a params.json key of the workload (repeatable)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=25.0, help="CPU-baseline timing budget")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0, help="CPU-baseline timing budget")
     ap.add_argument("--no-rs", action="store_true", help="skip the rate-state plain-vs-learned leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-metric leg")
     ap.add_argument("--no-side", action="store_true", help="skip the 400x300x200 slip-weakening leg")

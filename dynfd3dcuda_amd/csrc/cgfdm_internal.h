@@ -48,7 +48,7 @@ struct Params {
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
   int StageStreams = 0;  // 1: interior and free-surface launches of a stage on side streams
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
-  int WaveNarrowInterior = 0;  // 1: interior tiles 32 x 8 in blocks of 256 threads
+  int WaveInteriorKernel = 1;  // interior tiles: 0 general kernel; 1 pipelined kernel (64 x 8); 2 pipelined, 64 x 6
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
@@ -190,7 +190,8 @@ struct Ctx {
   float* img_top = nullptr;     // free surface: image velocity RHS of the three top rows [3][3][ni][nj]
   int wave_nwork = 0;                  // all work items
   bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
-  bool wave_narrow_in = false;      // interior tiles 32 x 8 (WaveNarrowInterior)
+  bool wave_pipe = false;           // pipelined interior kernel (WaveInteriorKernel)
+  int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the
   // fault kernels read, and that copy's base moved back by its first plane

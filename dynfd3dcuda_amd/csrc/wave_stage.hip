@@ -40,6 +40,7 @@ constexpr int TK = NTHR / TJ;
 constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j x 8 k)
 constexpr int NTHRN = 256;  // threads per block of the narrow tiles
 constexpr int TKN = NTHRN / TJN;
+constexpr int TK6 = 6;     // rows of the 64 x 6 interior tiles (WaveInteriorKernel 2)
 
 
 // Volume arrays are plane-interleaved with padded rows: word (field c,
@@ -511,6 +512,233 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
   }
 }
 
+
+// Raw metric words of one point (MetricBF16=2: five 32-bit pairs of bfloat16
+// deviations and the FP32 lambda, mu, rho; FP32: the 12 words xix..ztz, lam,
+// mu, rho; bfloat16: the same 12 as 16-bit words), held undecoded while they
+// wait for their plane.
+template <class MT>
+constexpr int met_raw_n() { return std::is_same<MT, MetBf16Delta>::value ? 8 : 12; }
+template <class MT>
+__device__ __forceinline__ void met_raw_load(Rsrc r, uint32_t voff, uint32_t mpb, uint32_t (&w)[met_raw_n<MT>()]) {
+  if constexpr (std::is_same<MT, MetBf16Delta>::value) {
+#pragma unroll
+    for (int p = 0; p < 5; p++) w[p] = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(2 * p) * mpb));
+#pragma unroll
+    for (int m = 0; m < 3; m++) w[5 + m] = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(10 + 2 * m) * mpb));
+  } else if constexpr (sizeof(MT) == 2) {
+#pragma unroll
+    for (int q = 0; q < 12; q++)
+      w[q] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)((q < 9 ? q : q + 1) * mpb), kStream) << 16;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 12; q++) w[q] = __float_as_uint(bld<kStream>(r, voff, (uint32_t)(q < 9 ? q : q + 1) * mpb));
+  }
+}
+// Field c (0..8 metric, 10..12 media) of raw metric words.
+template <class MT>
+__device__ __forceinline__ float met_raw_get(const uint32_t (&w)[met_raw_n<MT>()], int c, const float* ref) {
+  if constexpr (std::is_same<MT, MetBf16Delta>::value) {
+    if (c >= 10) return __uint_as_float(w[5 + c - 10]);
+    const uint32_t u = w[c >> 1];
+    return __uint_as_float((c & 1) ? (u & 0xffff0000u) : (u << 16)) + ref[c];
+  } else {
+    return __uint_as_float(w[c < 9 ? c : c - 1]);
+  }
+}
+
+// Interior stage kernel with every load of plane i+1 issued before plane i
+// is computed: the points of the interior box (no absorbing layer, no free
+// surface), where the general kernel's per-plane metric and RK-register
+// loads would each cost a memory round trip that the whole block waits for
+// at its barrier.  Per plane the block issues, in this order, the halo and
+// the new queue plane of plane i+1, the metric of plane i+1, and the S and T
+// words of plane i; vector-memory loads return in issue order, so waiting
+// for the S and T words at the RK combine leaves nothing outstanding of the
+// earlier ones, and those were issued a whole compute phase earlier.  The
+// arithmetic is the general kernel's (same operators, same order of the
+// sums), so results are identical to it bit for bit.
+template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB>
+__global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a) {
+  constexpr int TJ = TJ_, TK = NT_ / TJ_;
+  constexpr int LJ = TJ + 4, LK = TK + 4;
+  constexpr int NH = 4 * LJ + 4 * TK;
+  static_assert(NH <= NT_, "one halo cell per thread");
+  constexpr int HJL = FY > 0 ? 1 : 3;
+  constexpr int HKL = FZ > 0 ? 1 : 3;
+  constexpr int QLO = FX > 0 ? -1 : -3;
+  constexpr int CTR = -QLO;
+  constexpr int NMW = met_raw_n<MT>();
+  __shared__ float lds[2][9][LK][LJ];
+
+  const int item = xcd_remap((int)blockIdx.x, a.nwork);
+  const int4 wk = a.work[2 * item], we = a.work[2 * item + 1];
+  const int j0 = wk.x, k0 = wk.y, ibeg = wk.z, iend = wk.w;
+  const int tj = threadIdx.x, tk = threadIdx.y, tid = tj + tk * TJ;
+  const int ni = a.ni, nj = a.nj, nk = a.nk;
+  const int j = j0 + tj, k = k0 + tk;
+  const bool act = tj >= we.z && tj < we.x && tk >= we.w && tk < we.y;
+  const int nxp = ni + 6;
+  const uint32_t nyz = a.nyz;
+  const uint32_t py = a.py, jpad = a.jpad;
+  const uint32_t col = jpad + (uint32_t)(min(max(j, -3), nj + 2) + 3) + (uint32_t)(min(max(k, -3), nk + 2) + 3) * py;
+  const uint32_t col4 = col << 2;
+  const uint32_t mcol = col * (uint32_t)sizeof(MT);
+  const uint32_t pb = nyz * 4, mpb = nyz * (uint32_t)sizeof(MT);
+  const float* __restrict__ Win = a.Win;
+  const int irk = a.irk;
+
+  int hlk = 0, hlj = 0;
+  uint32_t hcol = col;
+  if (tid < NH) {
+    if (tid < 4 * LJ) {
+      const int r = tid / LJ;
+      hlk = r < HKL ? r : r + TK;
+      hlj = tid % LJ;
+    } else {
+      const int h2 = tid - 4 * LJ;
+      hlk = HKL + h2 / 4;
+      const int cc = h2 % 4;
+      hlj = cc < HJL ? cc : cc + TJ;
+    }
+    const int gj = min(max(j0 + hlj - HJL, -3), nj + 2), gk = min(max(k0 + hlk - HKL, -3), nk + 2);
+    hcol = jpad + (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * py;
+  }
+  const uint32_t hcol4 = hcol << 2;
+
+  float q[5][9];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    const Rsrc r = plane_rsrc(Win, ibeg + QLO + m + 3, 9, nyz);
+#pragma unroll
+    for (int c = 0; c < 9; c++) q[m][c] = bld(r, col4, c * pb);
+  }
+  float hv[9];
+  {
+    const Rsrc r = plane_rsrc(Win, ibeg + 3, 9, nyz);
+#pragma unroll
+    for (int c = 0; c < 9; c++) hv[c] = bld(r, hcol4, c * pb);
+  }
+  uint32_t mw[NMW];
+  met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw);
+#pragma unroll
+  for (int c = 0; c < 9; c++) lds[0][c][tk + HKL][tj + HJL] = q[CTR][c];
+  if (tid < NH) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) lds[0][c][hlk][hlj] = hv[c];
+  }
+  __syncthreads();
+
+  for (int i = ibeg; i < iend; ++i) {
+    const int b = (i - ibeg) & 1;
+    const bool more = i + 1 < iend;  // uniform per block
+    // plane i+1: halo, queue plane, metric (clamped in-bounds past the chunk)
+    {
+      const Rsrc r = plane_rsrc(Win, min(i + 4, nxp - 1), 9, nyz);
+#pragma unroll
+      for (int c = 0; c < 9; c++) hv[c] = bld(r, hcol4, c * pb);
+    }
+    float nq[9];
+    {
+      const Rsrc r = plane_rsrc(Win, min(i + QLO + 8, nxp - 1), 9, nyz);
+#pragma unroll
+      for (int c = 0; c < 9; c++) nq[c] = bld(r, col4, c * pb);
+    }
+    uint32_t mwn[NMW];
+    met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn);
+    // plane i: the step-start (S) and accumulator (T) words of the RK combine
+    const bool slab = a.fault && i >= a.ii0 - 3 && i <= a.ii0 + 3 && i != a.ii0;
+    const Rsrc rs = plane_rsrc(a.S, i + 3, 9, nyz), rt = plane_rsrc(a.T, i + 3, 9, nyz);
+    float sv[9], tv[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) { sv[c] = 0.0f; tv[c] = 0.0f; }
+    if (!a.rhs_only && !slab && irk != 0) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) tv[c] = bld<kStream>(rt, col4, c * pb);
+      if (irk != 3) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) sv[c] = bld<kStream>(rs, col4, c * pb);
+      }
+    }
+
+    if (act) {
+      auto MET = [&](int c) { return met_raw_get<MT>(mw, c, a.mref); };
+      const float xix = MET(0), xiy = MET(1), xiz = MET(2);
+      const float etx = MET(3), ety = MET(4), etz = MET(5);
+      const float ztx = MET(6), zty = MET(7), ztz = MET(8);
+      const float lam = MET(10), mu = MET(11);
+      const float rrho = 1.0f / MET(12);
+      const float l2m = lam + 2.0f * mu;
+      const float rDH = a.rDH;
+      const float(*L)[LK][LJ] = lds[b];
+      const int lj = tj + HJL, lk = tk + HKL;
+      auto Dx = [&](int c) { return drp5<FX>(q[0][c], q[1][c], q[2][c], q[3][c], q[4][c]); };
+      auto Dy = [&](int c) {
+        return FY > 0 ? drp5<1>(L[c][lk][lj - 1], L[c][lk][lj], L[c][lk][lj + 1], L[c][lk][lj + 2], L[c][lk][lj + 3])
+                      : drp5<-1>(L[c][lk][lj - 3], L[c][lk][lj - 2], L[c][lk][lj - 1], L[c][lk][lj], L[c][lk][lj + 1]);
+      };
+      auto Dz = [&](int c) {
+        return FZ > 0 ? drp5<1>(L[c][lk - 1][lj], L[c][lk][lj], L[c][lk + 1][lj], L[c][lk + 2][lj], L[c][lk + 3][lj])
+                      : drp5<-1>(L[c][lk - 3][lj], L[c][lk - 2][lj], L[c][lk - 1][lj], L[c][lk][lj], L[c][lk + 1][lj]);
+      };
+      float h[9];
+#pragma unroll
+      for (int c = 0; c < 9; c++) h[c] = 0.0f;
+      {
+        // the general kernel's arithmetic with beta = 1: rb * rDH = rDH
+        const float s1 = 1.0f * rDH, s2 = 1.0f * rDH, s3 = 1.0f * rDH;
+        accum_axis(h, Dx, xix * s1, xiy * s1, xiz * s1, rrho, lam, l2m, mu, true);
+        accum_axis(h, Dy, etx * s2, ety * s2, etz * s2, rrho, lam, l2m, mu, true);
+        accum_axis(h, Dz, ztx * s3, zty * s3, ztz * s3, rrho, lam, l2m, mu, true);
+      }
+      if (a.rhs_only) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) bst(plane_rsrc(a.rhs, i + 3, 9, nyz), col4, c * pb, h[c]);
+      } else if (slab) {
+        const Rsrc r = plane_rsrc(a.slabH, i - (a.ii0 - 3), 9, nyz);
+#pragma unroll
+        for (int c = 0; c < 9; c++) bst(r, col4, c * pb, h[c]);
+      } else {
+        // RK_Syn (rk4.cu:20-33) with the damping factor 1 of the interior
+        const Rsrc ro = plane_rsrc(a.Wout, i + 3, 9, nyz);
+        const float rka = a.rka, rkb = a.rkb;
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          float w;
+          if (irk == 0) {
+            bst(rt, col4, c * pb, q[CTR][c] + rkb * h[c]);
+            w = q[CTR][c] + rka * h[c];
+          } else if (irk == 3) {
+            w = tv[c] + rkb * h[c];
+          } else {
+            bst(rt, col4, c * pb, tv[c] + rkb * h[c]);
+            w = sv[c] + rka * h[c];
+          }
+          bst(ro, col4, c * pb, w * 1.0f);
+        }
+      }
+    }
+
+    if (more) {
+#pragma unroll
+      for (int c = 0; c < 9; c++) lds[b ^ 1][c][tk + HKL][tj + HJL] = q[CTR + 1][c];
+      if (tid < NH) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) lds[b ^ 1][c][hlk][hlj] = hv[c];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int c = 0; c < 9; c++) q[m][c] = q[m + 1][c];
+#pragma unroll
+    for (int c = 0; c < 9; c++) q[4][c] = nq[c];
+#pragma unroll
+    for (int m = 0; m < NMW; m++) mw[m] = mwn[m];
+    __syncthreads();
+  }
+}
+
 // ADE-PML auxiliary fields (pmlx.cu:126-210, pmly.cu, pmlz.cu): for every
 // point of the band of face f, the plain one-sided derivative along the
 // face's axis drives aux' = -(alpha + d/beta) aux + (d/beta) * RHS_axis, plus
@@ -650,10 +878,12 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   if (n[0] > 0) {
     a.work = w; a.nwork = n[0];
     pr.begin(si, PK_STAGE_INTERIOR);
-    if (c.wave_tall)
+    if (c.wave_tki == TK6)
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK6, 2>), dim3(a.nwork), dim3(TJ, TK6, 1), 0, si, a);
+    else if (c.wave_pipe)
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, si, a);
+    else if (c.wave_tall)
       hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0, TJ, 2 * NTHR>), dim3(a.nwork), dim3(TJ, 2 * TK, 1), 0, si, a);
-    else if (c.wave_narrow_in)
-      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0, TJN, NTHRN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, si, a);
     else
       hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0>), dim3(a.nwork), dim3(TJ, TK, 1), 0, si, a);
     pr.end(si);
@@ -843,18 +1073,20 @@ int upload_wave_cold(Ctx& c) {
   // threads, which re-read the k halo rows for twice as many points)
   const bool tall = p.WaveTallInterior != 0;
   c.wave_tall = tall;
-  // WaveNarrowInterior: interior tiles of TJN x TKN in blocks of NTHRN
-  // threads, four blocks (barrier groups) per CU instead of two
-  const bool narrow_in = p.WaveNarrowInterior != 0 && !tall;
-  c.wave_narrow_in = narrow_in;
+  // WaveInteriorKernel: 0 the general kernel on the interior tiles, 1 the
+  // pipelined interior kernel (64 x 8 tiles), 2 the pipelined kernel on
+  // 64 x 6 tiles (384 threads, two blocks per CU)
+  c.wave_pipe = !tall && p.WaveInteriorKernel >= 1;
+  const int tki = tall ? 2 * TK : (c.wave_pipe && p.WaveInteriorKernel == 2) ? TK6 : TK;  // interior tile rows
+  c.wave_tki = tki;
   for (int ax = 1; ax < 3; ax++) {
-    const int t = ax == 1 ? TJ : (tall ? 2 * TK : TK), e = n3[ax] - lo[ax] - hi[ax];
+    const int t = ax == 1 ? TJ : tki, e = n3[ax] - lo[ax] - hi[ax];
     if (e > t) (ax == 2 && surf ? lo[ax] : hi[ax]) += e % t;
   }
   const std::vector<Seg> sj = axis_segments(n3[1], lo[1], hi[1], TJ);
   const std::vector<Seg> sk = axis_segments(n3[2], lo[2], hi[2], TK);
-  const std::vector<Seg> ski = tall ? axis_segments(n3[2], lo[2], hi[2], 2 * TK) : sk;  // interior tiles' rows
-  const std::vector<Seg> sji = narrow_in ? axis_segments(n3[1], lo[1], hi[1], TJN) : sj;  // and columns
+  const std::vector<Seg> ski = tki != TK ? axis_segments(n3[2], lo[2], hi[2], tki) : sk;  // interior tiles' rows
+  const std::vector<Seg>& sji = sj;
   const int nk = n3[2];
   auto is_top = [&](const Seg& b) { return freenode && b.s + b.n > nk - 3; };
   std::vector<Seg> sjt = sj;  // j tiles of the free-surface rows
