@@ -33,7 +33,7 @@ EXPORTED = [
     "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
     "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
     "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress", "cgfdm_profile_classes",
-    "cgfdm_stream_probe", "cgfdm_check_field", "cgfdm_print_params",
+    "cgfdm_stream_probe", "cgfdm_check_field", "cgfdm_print_params", "cgfdm_mlp_forward_noalloc",
 ]
 
 # kernel classes timed by cgfdm_profile / cgfdm_profile_get
@@ -114,6 +114,7 @@ def load_library(path=None):
     lib.cgfdm_check_field.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong),
                                       ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
     lib.cgfdm_print_params.argtypes = [c_p]
+    lib.cgfdm_mlp_forward_noalloc.argtypes = [c_p, ctypes.c_long, c_p, ctypes.c_long, c_p, ctypes.c_int, c_p]
     lib.cgfdm_stream_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double)]
     for name in EXPORTED[1:]:

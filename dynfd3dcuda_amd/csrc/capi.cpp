@@ -657,6 +657,18 @@ int cgfdm_mlp_forward(const float* params, const float* feat, long n, float* out
   return 0;
 }
 
+int cgfdm_mlp_forward_noalloc(const float* params, long nparams, const float* feat, long n, float* out, int mode,
+                              void* stream) {
+  if (nparams != kMlpParams)
+    return fail(CGFDM_ESIZE, "cgfdm_mlp_forward_noalloc: the scratch holds a network of exactly " +
+                                 std::to_string(kMlpParams) + " parameters, got " + std::to_string(nparams));
+  if (n < 0 || n > 0x7fffffffL) return fail(CGFDM_ESIZE, "cgfdm_mlp_forward_noalloc: n must be in 0..2^31-1");
+  if (!params || (n > 0 && (!feat || !out)) || (mode != 0 && mode != 1)) return fail(CGFDM_EPARAM, "bad argument");
+  std::string err;
+  if (mlp_forward_noalloc_host(params, feat, n, out, mode, (hipStream_t)stream, err)) return fail(CGFDM_EHIP, err);
+  return 0;
+}
+
 int cgfdm_mlp_grad(const float* params, const float* feat, const float* target, long n, float* grad, float* loss) {
   if (!params || !feat || !target || !grad || !loss || n <= 0) return fail(CGFDM_EPARAM, "bad argument");
   std::string err;

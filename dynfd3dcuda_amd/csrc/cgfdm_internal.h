@@ -233,6 +233,8 @@ int mlp_upload(Ctx& c, const float* blob, long n);  // nonzero on error
 int mlp_forward_host(const float* blob, const float* feat, long n, float* out, int mode, std::string& err);
 int mlp_grad_host(const float* blob, const float* feat, const float* target, long n, float* grad, float* loss,
                   std::string& err);
+int mlp_forward_noalloc_host(const float* P, const float* feat, long n, float* out, int mode, hipStream_t s,
+                             std::string& err);
 int field_check_host(Ctx& c, const float* ptr, int nf, int f0, int fn, long flat_n, long long* nonfinite,
                      long long* ghost_nonzero, double* maxabs, std::string& err);
 int stream_probe_host(int nread, int nwrite, size_t bytes_per_array, int reps, double* gbps, std::string& err);

@@ -14,10 +14,19 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <random>
 #include <vector>
 
 namespace cgfdm {
+
+// Scratch of cgfdm_mlp_forward_noalloc: the packed network (an MlpDev) in
+// static device memory, one copy per GPU, held until the process ends, so
+// that call allocates nothing.  Every call on a GPU overwrites that GPU's
+// copy: calls that overlap on one GPU race.
+constexpr size_t kMlpScratchBytes = 32768;
+static_assert(sizeof(MlpDev) <= kMlpScratchBytes, "the scratch holds one packed network");
+static __device__ __align__(256) unsigned char scratch[kMlpScratchBytes];
 
 // offsets of the parameter blob
 constexpr int kOffW1 = 10, kOffB1 = kOffW1 + kMlpH * kMlpIn, kOffW2 = kOffB1 + kMlpH,
@@ -244,6 +253,40 @@ int mlp_upload(Ctx& c, const float* blob, long n) {
     return 1;
   }
   return hipMemcpy(c.mlp_dev, &h, sizeof(MlpDev), hipMemcpyHostToDevice) != hipSuccess;
+}
+
+namespace {
+constexpr int kScratchDevices = 64;
+void* g_scratch_dev[kScratchDevices] = {};  // device address of `scratch` on each GPU
+std::mutex g_scratch_mu;
+}  // namespace
+
+// The address of this GPU's `scratch`, looked up once per device with that
+// device current (the host symbol is only a stand-in for the device array).
+int mlp_scratch_address(void** ptr, std::string& err) {
+  int dev = 0;
+  if (hip_ok(hipGetDevice(&dev), "hipGetDevice", err)) return 1;
+  if (dev < 0 || dev >= kScratchDevices) { err = "device index out of range for the scratch table"; return 1; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)", err)) return 1;
+    g_scratch_dev[dev] = p;
+  }
+  *ptr = g_scratch_dev[dev];
+  return 0;
+}
+
+// cgfdm_mlp_forward with the caller's device arrays and the static scratch:
+// the same pack and forward kernels, queued on `s`, no allocation.
+int mlp_forward_noalloc_host(const float* P, const float* feat, long n, float* out, int mode, hipStream_t s,
+                             std::string& err) {
+  void* sp = nullptr;
+  if (mlp_scratch_address(&sp, err)) return 1;
+  MlpDev* m = (MlpDev*)sp;
+  pack_on(s, P, m);
+  if (n > 0) hipLaunchKernelGGL(mlp_forward_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, m, feat, n, out, mode);
+  return hip_ok(hipGetLastError(), "mlp forward launch", err);
 }
 
 int mlp_forward_host(const float* blob, const float* feat, long n, float* out, int mode, std::string& err) {

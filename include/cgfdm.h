@@ -225,6 +225,17 @@ int cgfdm_set_newton_mlp(cgfdm_ctx* ctx, const float* params, long n);
  * arrays): mode 0 with the wavefront matrix-instruction kernel the solver
  * uses, mode 1 with the scalar kernel. */
 int cgfdm_mlp_forward(const float* params, const float* feat, long n, float* out, int mode);
+/* cgfdm_mlp_forward that allocates nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  params (nparams =
+ * CGFDM_MLP_PARAMS floats), feat[n][4] and out[n] are DEVICE arrays of the
+ * caller; the packed network goes to a static device scratch of 32 KiB per
+ * GPU (held until the process ends).  Queued on `stream` (a hipStream_t, or
+ * NULL for the null stream) without synchronising.  Rejects a network of
+ * another size and n outside 0..2^31-1 (CGFDM_ESIZE).  Every call on a GPU
+ * shares that GPU's scratch, so calls that overlap on one GPU race. */
+int cgfdm_mlp_forward_noalloc(const float* params, long nparams, const float* feat, long n, float* out, int mode,
+                              void* stream);
 /* Gradient (CGFDM_MLP_PARAMS floats, zero for the standardisation entries)
  * and value of the loss mean_s (o_s - (t_s - ymean)/yscale)^2 on the GPU. */
 int cgfdm_mlp_grad(const float* params, const float* feat, const float* target, long n, float* grad, float* loss);
