@@ -6,6 +6,17 @@
 #include <stdint.h>
 
 namespace cgfdm {
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block L runs on
+// XCD L % 8).  Give XCD x the contiguous work range [x*per, ...), so blocks
+// resident on one XCD at the same time are neighbours in the work order.
+__device__ __forceinline__ int xcd_remap(int L, int n) {
+  const int per = (n + 7) >> 3;
+  const int x = L & 7, s = L >> 3;
+  const int full = n - 8 * (per - 1);  // XCDs that take 'per' items; the rest take per-1
+  return x < full ? x * per + s : full * per + (x - full) * (per - 1) + s;
+}
+
 namespace fd {
 
 // DRP/MacCormack coefficients (macdrp.h:23-40).

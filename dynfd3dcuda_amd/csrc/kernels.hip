@@ -527,13 +527,19 @@ __device__ __forceinline__ float dxv_off(const FArgs& a, int nn, int c, long pc,
 
 // fault_deriv: fault_dvelo_cu (fault_dvelo.cu:17-389) + fault_dstrs_f/b_cu
 // (fault_dstrs_f.cu:17-1254, fault_dstrs_b.cu) writing the slab RHS.  The
-// parts that are independent per slab plane run in parallel, one per block
-// layer (blockIdx.z): 0-5 the velocity rates of planes i0-3..-1, +1..+3,
+// parts that are independent per slab plane run in parallel, one per block:
+// 0-5 the velocity rates of planes i0-3..-1, +1..+3,
 // 6-11 the stress rates of the same planes, 12-13 the split-node velocity
-// rates of sides 0 and 1, 14-15 their stress rates.
-__global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int k = blockIdx.y * blockDim.y + threadIdx.y;
+// rates of sides 0 and 1, 14-15 their stress rates.  The 16 parts of one
+// 64 x 4 tile of fault nodes are consecutive work items, kept on one XCD:
+// they read the same slab planes around the tile, which then come from that
+// XCD's L2 instead of being fetched once per part.
+constexpr int kDerivParts = 16;
+__global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a, int tiles_j, int items) {
+  const int item = xcd_remap((int)blockIdx.x, items);
+  const int part = item % kDerivParts, tile = item / kDerivParts;
+  int j = (tile % tiles_j) * 64 + threadIdx.x;
+  int k = (tile / tiles_j) * 4 + threadIdx.y;
   if (j >= a.nj || k >= a.nk) return;
   const FaultDev& F = a.F;
   long q = j + (long)k * a.nj;
@@ -549,7 +555,6 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a) {
   float* slab = a.slabH;
   auto slabx = [&](int i, int c) -> float& { return slab[(long)(i - (i0 - 3)) * 9 * pyz + c * pyz + pc]; };
 
-  const int part = blockIdx.z;
   if (part < 6) {  // velocity rates at i0-3..-1, +1..+3 (fault_dvelo.cu:153-255)
     const int i = i0 + (part < 3 ? part - 3 : part - 2);
     {
@@ -1045,9 +1050,9 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz) {
   a.W = Win;
   a.FX = fx; a.FY = fy; a.FZ = fz;
   c.prof.begin(c.stream, PK_FAULT_DERIV);
-  dim3 g = fgrid(c);
-  g.z = 16;  // per-plane parts and the split-node parts
-  hipLaunchKernelGGL(fault_deriv_kernel, g, dim3(64, 4, 1), 0, c.stream, a);
+  const dim3 g = fgrid(c);
+  const int items = (int)(g.x * g.y) * kDerivParts;  // per-plane parts and the split-node parts of each tile
+  hipLaunchKernelGGL(fault_deriv_kernel, dim3(items), dim3(64, 4, 1), 0, c.stream, a, (int)g.x, items);
   c.prof.end(c.stream);
 }
 

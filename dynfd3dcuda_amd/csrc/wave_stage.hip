@@ -158,16 +158,6 @@ struct WaveArgs {
   int irk, freenode, cerjan, fault, rhs_only, pml;
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs (block L runs on
-// XCD L % 8).  Give XCD x the contiguous work range [x*per, ...), so blocks
-// resident on one XCD at the same time are spatial neighbours.
-__device__ __forceinline__ int xcd_remap(int L, int n) {
-  const int per = (n + 7) >> 3;
-  const int x = L & 7, s = L >> 3;
-  const int full = n - 8 * (per - 1);  // XCDs that take 'per' items; the rest take per-1
-  return x < full ? x * per + s : full * per + (x - full) * (per - 1) + s;
-}
-
 // Free surface, first pass: the conservative traction vectors
 // J (grad xi_ax . sigma) (wave.cu:398-440) of the six rows nk-6..nk-1 that
 // the image stencil reaches, for every padded plane and column, from the
@@ -558,11 +548,24 @@ __device__ __forceinline__ float met_raw_get(const uint32_t (&w)[met_raw_n<MT>()
 // earlier ones, and those were issued a whole compute phase earlier.  The
 // arithmetic is the general kernel's (same operators, same order of the
 // sums), so results are identical to it bit for bit.
-template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB>
+//
+// BAND: the same pipeline on the points of the absorbing layers (the general
+// kernel's KIND 1 tiles; no free-surface rows): the RHS of a point inside the
+// ADE-PML band of a face gets -aux/beta of that face, and the Cerjan sponge
+// damps the new value.  The aux words of plane i are issued with its S and T
+// words, one face per axis at most (the host uses this kernel only when the
+// two bands of an axis do not overlap), and added after the three axis sums
+// in the general kernel's order; 1/beta and the sponge factor of plane i+1
+// are fetched during plane i.
+template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB, bool BAND = false>
 __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a) {
   constexpr int TJ = TJ_, TK = NT_ / TJ_;
   constexpr int LJ = TJ + 4, LK = TK + 4;
-  constexpr int NH = 4 * LJ + 4 * TK;
+  // halo cells of one plane: the 4 k-halo rows over the tile's own columns
+  // and 4 j-halo cells per row; the corners of the LDS tile are neither
+  // written nor read (the y and z operators reach along one axis only), so
+  // each k-halo row costs the cache lines of the tile's own columns alone
+  constexpr int NH = 4 * TJ + 4 * TK;
   static_assert(NH <= NT_, "one halo cell per thread");
   constexpr int HJL = FY > 0 ? 1 : 3;
   constexpr int HKL = FZ > 0 ? 1 : 3;
@@ -570,6 +573,12 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   constexpr int CTR = -QLO;
   constexpr int NMW = met_raw_n<MT>();
   __shared__ float lds[2][9][LK][LJ];
+  // BAND: the absorbing layers' uniform descriptors, staged once per block
+  __shared__ const float* s_auxY[6];
+  __shared__ const float* s_pBx;
+  __shared__ const float* s_cjx;
+  __shared__ int s_on[6];
+  __shared__ int s_N;
 
   const int item = xcd_remap((int)blockIdx.x, a.nwork);
   const int4 wk = a.work[2 * item], we = a.work[2 * item + 1];
@@ -591,12 +600,12 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   int hlk = 0, hlj = 0;
   uint32_t hcol = col;
   if (tid < NH) {
-    if (tid < 4 * LJ) {
-      const int r = tid / LJ;
+    if (tid < 4 * TJ) {
+      const int r = tid / TJ;
       hlk = r < HKL ? r : r + TK;
-      hlj = tid % LJ;
+      hlj = HJL + tid % TJ;
     } else {
-      const int h2 = tid - 4 * LJ;
+      const int h2 = tid - 4 * TJ;
       hlk = HKL + h2 / 4;
       const int cc = h2 % 4;
       hlj = cc < HJL ? cc : cc + TJ;
@@ -605,6 +614,34 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     hcol = jpad + (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * py;
   }
   const uint32_t hcol4 = hcol << 2;
+
+  // BAND: per lane, the axes whose absorbing band (one side) holds the
+  // point's column (j, k), their 1/beta, and the sponge factor of the column
+  const bool pml = BAND && a.pml, cerjan = BAND && a.cerjan;
+  float rb2 = 1.0f, rb3 = 1.0f, cjk = 1.0f;
+  int N = 0;
+  bool inj = false, ink = false;
+  uint32_t offj = 0, offk = 0;
+  if constexpr (BAND) {
+    if (threadIdx.y == 0 && threadIdx.x < 6) {
+      s_auxY[threadIdx.x] = a.cold->auxY[threadIdx.x];
+      s_on[threadIdx.x] = a.cold->on[threadIdx.x];
+      if (threadIdx.x == 0) { s_N = a.cold->N; s_pBx = a.cold->pB[0]; s_cjx = a.cold->cj[0]; }
+    }
+    if (act && pml) { rb2 = 1.0f / a.cold->pB[1][j]; rb3 = 1.0f / a.cold->pB[2][k]; }
+    if (act && cerjan) cjk = fminf(a.cold->cj[1][j + 3], a.cold->cj[2][k + 3]);
+    __syncthreads();
+    if (pml) {
+      N = s_N;
+      if (s_on[2] && j < N) { inj = true; offj = 0; }
+      else if (s_on[3] && j >= nj - N) { inj = true; offj = nj - N; }
+      if (s_on[4] && k < N) { ink = true; offk = 0; }
+      else if (s_on[5] && k >= nk - N) { ink = true; offk = nk - N; }
+    }
+  }
+  // 1/beta of plane i along x and the sponge factor of plane i (uniform)
+  auto rb_x = [&](int i) { return pml ? 1.0f / s_pBx[i] : 1.0f; };
+  auto cj_x = [&](int i) { return cerjan ? s_cjx[i + 3] : 1.0f; };
 
   float q[5][9];
 #pragma unroll
@@ -621,6 +658,8 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   }
   uint32_t mw[NMW];
   met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw);
+  float rb1 = 1.0f, cjx = 1.0f;
+  if constexpr (BAND) { rb1 = rb_x(ibeg); cjx = cj_x(ibeg); }
 #pragma unroll
   for (int c = 0; c < 9; c++) lds[0][c][tk + HKL][tj + HJL] = q[CTR][c];
   if (tid < NH) {
@@ -646,6 +685,10 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     }
     uint32_t mwn[NMW];
     met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn);
+    float rb1n = 1.0f, cjxn = 1.0f;
+    if constexpr (BAND) {
+      if (more) { rb1n = rb_x(i + 1); cjxn = cj_x(i + 1); }
+    }
     // plane i: the step-start (S) and accumulator (T) words of the RK combine
     const bool slab = a.fault && i >= a.ii0 - 3 && i <= a.ii0 + 3 && i != a.ii0;
     const Rsrc rs = plane_rsrc(a.S, i + 3, 9, nyz), rt = plane_rsrc(a.T, i + 3, 9, nyz);
@@ -658,6 +701,34 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
       if (irk != 3) {
 #pragma unroll
         for (int c = 0; c < 9; c++) sv[c] = bld<kStream>(rs, col4, c * pb);
+      }
+    }
+    // BAND: the aux words of plane i of the face along each axis whose band
+    // holds the point (pmlx.cu:126-210 and the y/z variants)
+    float ya[3][9];
+    bool inx = false;
+    if constexpr (BAND) {
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+        for (int c = 0; c < 9; c++) ya[ax][c] = 0.0f;
+      if (pml) {
+        uint32_t offi = 0;
+        if (s_on[0] && i < N) { inx = true; offi = 0; }
+        else if (s_on[1] && i >= ni - N) { inx = true; offi = ni - N; }
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {
+          const bool in = ax == 0 ? inx : ax == 1 ? inj : ink;
+          if (!act || !in) continue;
+          const int f = 2 * ax + (ax == 0 ? offi != 0 : ax == 1 ? offj != 0 : offk != 0);
+          const uint32_t pi = ax == 0 ? i - offi : i, pj = ax == 1 ? j - offj : j, pk = ax == 2 ? k - offk : k;
+          const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
+          const size_t ps = (size_t)pni * pnj * pnk;
+          const uint32_t pa4 = (pj + pk * pnj + pi * pnj * pnk) << 2;
+          const float* Y = s_auxY[f];
+#pragma unroll
+          for (int c = 0; c < 9; c++) ya[ax][c] = ldo(Y + c * ps, pa4);
+        }
       }
     }
 
@@ -685,11 +756,24 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
 #pragma unroll
       for (int c = 0; c < 9; c++) h[c] = 0.0f;
       {
-        // the general kernel's arithmetic with beta = 1: rb * rDH = rDH
-        const float s1 = 1.0f * rDH, s2 = 1.0f * rDH, s3 = 1.0f * rDH;
+        // the general kernel's arithmetic (in the interior beta = 1: rb * rDH = rDH)
+        const float s1 = rb1 * rDH, s2 = rb2 * rDH, s3 = rb3 * rDH;
         accum_axis(h, Dx, xix * s1, xiy * s1, xiz * s1, rrho, lam, l2m, mu, true);
         accum_axis(h, Dy, etx * s2, ety * s2, etz * s2, rrho, lam, l2m, mu, true);
         accum_axis(h, Dz, ztx * s3, zty * s3, ztz * s3, rrho, lam, l2m, mu, true);
+      }
+      if constexpr (BAND) {
+        if (pml) {
+#pragma unroll
+          for (int ax = 0; ax < 3; ax++) {
+            const bool in = ax == 0 ? inx : ax == 1 ? inj : ink;
+            const float rbb = ax == 0 ? rb1 : ax == 1 ? rb2 : rb3;  // 1 / beta at the point
+            if (in) {
+#pragma unroll
+              for (int c = 0; c < 9; c++) h[c] += -ya[ax][c] * rbb;
+            }
+          }
+        }
       }
       if (a.rhs_only) {
 #pragma unroll
@@ -699,9 +783,10 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
 #pragma unroll
         for (int c = 0; c < 9; c++) bst(r, col4, c * pb, h[c]);
       } else {
-        // RK_Syn (rk4.cu:20-33) with the damping factor 1 of the interior
+        // RK_Syn (rk4.cu:20-33); the damping factor is 1 in the interior
         const Rsrc ro = plane_rsrc(a.Wout, i + 3, 9, nyz);
         const float rka = a.rka, rkb = a.rkb;
+        const float damp = cerjan ? fminf(cjx, cjk) : 1.0f;
 #pragma unroll
         for (int c = 0; c < 9; c++) {
           float w;
@@ -714,7 +799,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
             bst(rt, col4, c * pb, tv[c] + rkb * h[c]);
             w = sv[c] + rka * h[c];
           }
-          bst(ro, col4, c * pb, w * 1.0f);
+          bst(ro, col4, c * pb, w * damp);
         }
       }
     }
@@ -735,6 +820,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     for (int c = 0; c < 9; c++) q[4][c] = nq[c];
 #pragma unroll
     for (int m = 0; m < NMW; m++) mw[m] = mwn[m];
+    if constexpr (BAND) { rb1 = rb1n; cjx = cjxn; }
     __syncthreads();
   }
 }
@@ -903,11 +989,18 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     pr.begin(s, PK_STAGE_BAND);
     if (n[1] > 0) {
       a.work = w + 2 * o1; a.nwork = n[1];
-      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+      if (c.wave_band_pipe)
+        hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2, true>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
+      else
+        hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
     }
     if (n[2] > 0) {
       a.work = w + 2 * o2; a.nwork = n[2];
-      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1, TJN, NTHRN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, s, a);
+      if (c.wave_band_pipe)
+        hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJN, NTHRN, 2, true>), dim3(a.nwork), dim3(TJN, TKN, 1), 0,
+                           s, a);
+      else
+        hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1, TJN, NTHRN>), dim3(a.nwork), dim3(TJN, TKN, 1), 0, s, a);
     }
     pr.end(s);
   }
@@ -1077,6 +1170,14 @@ int upload_wave_cold(Ctx& c) {
   // pipelined interior kernel (64 x 8 tiles), 2 the pipelined kernel on
   // 64 x 6 tiles (384 threads, two blocks per CU)
   c.wave_pipe = !tall && p.WaveInteriorKernel >= 1;
+  // WaveBandKernel 1: the pipelined kernel on the band tiles too, which adds
+  // the aux terms of one face per axis: used when the two ADE-PML bands of
+  // every axis are disjoint (2 N <= the axis length)
+  bool disjoint = true;
+  for (int ax = 0; ax < 3; ax++)
+    if (c.pml.on[2 * ax] && c.pml.on[2 * ax + 1] && 2 * c.pml.N > (ax == 0 ? c.d.ni : ax == 1 ? c.d.nj : c.d.nk))
+      disjoint = false;
+  c.wave_band_pipe = p.WaveBandKernel >= 1 && disjoint;
   const int tki = tall ? 2 * TK : (c.wave_pipe && p.WaveInteriorKernel == 2) ? TK6 : TK;  // interior tile rows
   c.wave_tki = tki;
   for (int ax = 1; ax < 3; ax++) {
@@ -1111,16 +1212,22 @@ int upload_wave_cold(Ctx& c) {
   // 4 planes too (shorter chunks re-read the warm-up planes for 1-2 planes
   // of work; 0.58 against 0.52 ms per step on 400 x 300 x 200)
   auto chunk = [&](int span, int ntiles, int target, int minlen) {
-    if (p.WaveIChunk > 0) return p.WaveIChunk;
     span = std::max(span, 1);
     const int chunks = std::max(1, std::min(span, (target + std::max(ntiles, 1) - 1) / std::max(ntiles, 1)));
     return std::max((span + chunks - 1) / chunks, std::min(minlen, span));
   };
-  const int ich_in = chunk(ib - ia, tiles_in, tall ? 2048 : 4096, 4);
-  // the bands take twice the blocks (chunks of about 4 planes here: 2.68
-  // against 2.91 ms per step for about 7 on 400 x 300 x 200)
-  const int ich_band = p.WaveBandIChunk > 0 ? p.WaveBandIChunk
-                                            : chunk(ni, tiles[1] + tiles[0] * (ni - (ib - ia)) / std::max(ni, 1), 8192, 4);
+  // The pipelined interior kernel streams 32 planes per block: its loads
+  // overlap the compute, so a longer chunk mostly saves warm-up planes
+  // (1600 x 1000 x 500: 97.7 ms of interior per step against 106.2 for the
+  // ~300-plane chunks of a 4096-block target and 104.5 for 16 planes;
+  // 400 x 300 x 200: 2.44 against 2.70 ms for ~8 planes).
+  const int ich_in = p.WaveIChunk > 0 ? p.WaveIChunk
+                     : c.wave_pipe   ? std::min(std::max(ib - ia, 1), 32)
+                                     : chunk(ib - ia, tiles_in, tall ? 2048 : 4096, 4);
+  // band tiles stream 20 planes (1600 x 1000 x 500: 23.4 ms per step against
+  // 24.6, 25.0 and 32.9 for 16, 32 and 48 planes and 29.6 for the ~4 planes
+  // of an 8192-block target; 400 x 300 x 200: 2.55 against 2.68 ms)
+  const int ich_band = p.WaveBandIChunk > 0 ? p.WaveBandIChunk : std::min(std::max(ni, 1), 20);
   const int ich_top = p.WaveTopIChunk > 0 ? p.WaveTopIChunk : chunk(ni, tiles[2], 2048, 4);
   // lists: 0 interior, 1 absorbing bands (64-wide tiles), 2 absorbing bands
   // of at most 32 columns along j (32 x 16 tiles, half the idle lanes),

@@ -12,9 +12,10 @@
       kernels (points per stage from the JSON line bench.py printed in the
       same configuration)
 
-Kernel classes: stage_interior / stage_band / stage_surface (the three
-instantiations of the wave stage kernel; stage_surface includes the two
-free-surface precompute passes), pml_aux, fault_pre, fault_rs, fault_deriv,
+Kernel classes: stage_interior / stage_band / stage_surface (the
+instantiations of the wave stage kernel, stage_interior also the pipelined
+interior kernel; stage_surface includes the two free-surface precompute
+passes), pml_aux, fault_pre, fault_rs, fault_deriv,
 fault_post, rup_sensor, thermpress, setup_metric (the setup's metric
 pass); everything else is listed under its own name.
 """
@@ -38,6 +39,8 @@ def kernel_class(name):
     m = re.search(r"wave_stage_kernelI(?:Li-?\d+E|Lin1E){3}[ft]Li([012])E", name)  # mangled form
     if m:
         return STAGE_CLASSES[int(m.group(1))]
+    if "wave_interior_kernel" in name:
+        return "stage_interior"  # the pipelined interior kernel
     if "traction_top_kernel" in name or "image_top_kernel" in name:
         return "stage_surface"  # the free-surface precompute passes
     for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_rs_kernel", "fault_deriv_kernel", "fault_post_kernel"):

@@ -1,5 +1,6 @@
 """Seeded rupture cases for the accuracy comparison against the FP64 oracle
-(final slip, rupture time, peak slip rate, seismograms)."""
+(final slip, rupture time, peak slip rate, surface seismograms and on-fault
+station time series)."""
 import numpy as np
 
 from dynfd3dcuda_amd import configs
@@ -77,3 +78,24 @@ def receivers(d):
 def sample_receivers(W, d, rec):
     W = W.reshape(9, d["nx"], d["nz"], d["ny"])
     return np.array([[W[c, i + 3, k + 3, j + 3] for c in range(3)] for (i, j, k) in rec])
+
+
+FAULT_QUANTITIES = ("Vs1", "Vs2", "slip", "Ts1", "Ts2", "Tn")
+
+
+def fault_stations(d):
+    """Fault nodes (j, k) of the on-fault time series: the hypocentre of the
+    nucleation patch (configs._sw_stress: j = nj/2, k = nk - 1 - (nk - 30)/2),
+    four nodes along strike at 25 and 45 cells from it, one up dip near the
+    free surface and one down dip."""
+    nj, nk = d["nj"], d["nk"]
+    jc, kc = nj // 2, nk - 1 - int(round(0.5 * (nk - 30)))
+    return [(jc, kc), (jc - 25, kc), (jc + 25, kc), (jc - 45, kc), (jc + 45, kc), (jc, nk - 6), (jc, kc - 18)]
+
+
+def sample_fault(get, d, st):
+    """The FAULT_QUANTITIES at the stations: get(name) returns the [nk][nj]
+    fault array of a quantity, flattened."""
+    nj = d["nj"]
+    vals = [np.asarray(get(q)).reshape(-1) for q in FAULT_QUANTITIES]
+    return np.array([[v[k * nj + j] for v in vals] for (j, k) in st])

@@ -1,7 +1,8 @@
 """Writes the FP64-oracle reference outputs of the accuracy cases
 (tests/golden/accuracy_<case>.npz) that tests/test_rupture_accuracy.py compares the
-HIP solver with: final slip, rupture time (init_t0), peak slip rate and
-surface velocity seismograms at a few receivers.
+HIP solver with: final slip, rupture time (init_t0), peak slip rate,
+surface velocity seismograms at a few receivers and time series of slip
+rate, slip and tractions at a few fault stations.
 
     python tests/golden/make_accuracy_fixture.py [case ...]
 
@@ -30,21 +31,24 @@ def run(name):
     o.setup()
     d = o.dims()
     rec = ac.receivers(d)
+    st = ac.fault_stations(d)
     nt = case["nt"]
     seis = np.zeros((nt, len(rec), 3))
+    fault = np.zeros((nt, len(st), len(ac.FAULT_QUANTITIES)))
     peak = np.zeros(d["nj"] * d["nk"])
     t0 = time.time()
     for it in range(nt):
         o.step(it)
         W = o.get("W")
         seis[it] = ac.sample_receivers(W, d, rec)
+        fault[it] = ac.sample_fault(o.get, d, st)
         v = np.hypot(o.get("Vs1"), o.get("Vs2"))
         peak = np.maximum(peak, v)
         if it % 50 == 0:
             print(name, it, f"{time.time() - t0:.1f}s", flush=True)
     out = os.path.join(HERE, f"accuracy_{name}.npz")
     np.savez_compressed(out, slip=o.get("slip"), init_t0=o.get("init_t0"), peak_rate=peak, seis=seis,
-                        receivers=np.array(rec), nt=nt)
+                        receivers=np.array(rec), fault_series=fault, fault_stations=np.array(st), nt=nt)
     print("wrote", out, f"{time.time() - t0:.1f}s")
 
 

@@ -146,22 +146,30 @@ def test_stage_streams_same_result(extra):
         np.testing.assert_array_equal(a.get(name), b.get(name), err_msg=name)
 
 
-@pytest.mark.parametrize("layout,exact", [({"WaveBandRound": 1}, True), ({"WaveTallInterior": 1}, True),
-                                          ({"WaveBandRound": 1, "WaveTallInterior": 1}, True),
-                                          ({"WaveSurfaceTiles": 0}, False)],
-                         ids=["band_round", "tall", "band_round_tall", "surface_tiles_off"])
-def test_tilings_give_the_same_result(layout, exact):
+@pytest.mark.parametrize("layout,exact,ny", [({"WaveBandRound": 1}, True, 150), ({"WaveTallInterior": 1}, True, 150),
+                                             ({"WaveBandRound": 1, "WaveTallInterior": 1}, True, 150),
+                                             ({"WaveSurfaceTiles": 0}, False, 150),
+                                             ({"WaveInteriorKernel": 0}, True, 150),
+                                             ({"WaveInteriorKernel": 2}, True, 150),
+                                             ({"WaveBandKernel": 1}, True, 150),
+                                             ({"WaveBandKernel": 1, "useCerjan": 1}, True, 150)],
+                         ids=["band_round", "tall", "band_round_tall", "surface_tiles_off", "general_interior",
+                              "interior_64x6", "pipelined_band", "pipelined_band_cerjan"])
+def test_tilings_give_the_same_result(layout, exact, ny):
     """The tiling options only move points between the interior, band and
-    free-surface launches; a point's arithmetic is the same in each (beta = 1
-    and no damping outside the absorbing layers).  10 steps of a
-    slip-weakening rupture on a curved grid with PML and free surface:
-    - moves between the interior and band kernels give the default tiling's
-      result bit for bit;
+    free-surface launches, or between interior kernels of other tile shapes;
+    a point's arithmetic is the same in each (beta = 1 and no damping outside
+    the absorbing layers).  10 steps of a slip-weakening rupture on a curved
+    grid with PML and free surface:
+    - moves between the interior and band kernels, the general and 64 x 6
+      interior kernels, and the pipelined kernel on the absorbing-band tiles
+      (with ADE-PML, and with PML and the Cerjan sponge together), give the
+      default tiling's result bit for bit;
     - WaveSurfaceTiles 0 moves rows between the free-surface kernel and the
       others, whose code the compiler contracts differently (multiply-adds
       fused or not), so the result agrees to FP32 rounding: 1e-5 relative
       per field after 10 steps (both stay 4e-6 from the FP64 oracle)."""
-    case = cases.sw_case(curved=True, NY=150)
+    case = cases.sw_case(curved=True, NY=ny, **{k: v for k, v in layout.items() if k == "useCerjan"})
     a = dyn.Solver(case["params"])
     b = dyn.Solver(dict(case["params"], **layout))
     for s in (a, b):
