@@ -49,7 +49,7 @@ struct Params {
   int StageStreams = 0;  // 1: interior and free-surface launches of a stage on side streams
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
   int WaveInteriorKernel = 1;  // interior tiles: 0 general kernel; 1 pipelined kernel (64 x 8); 2 pipelined, 64 x 6
-  int WaveBandKernel = 0;  // absorbing-band tiles: 0 general kernel; 1 pipelined kernel
+  int WaveBandKernel = 1;  // absorbing-band tiles: 0 general kernel; 1 pipelined kernel
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
