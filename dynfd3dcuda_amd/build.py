@@ -12,8 +12,11 @@ ROOT = os.path.dirname(HERE)
 
 SOURCES = ["params.cpp", "setup.cpp", "netcdf_classic.cpp", "kernels.hip", "wave_stage.hip", "mlp_train.hip", "stream_probe.hip", "field_check.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-# packed FP32 instructions are disabled for all device code
-DEVFLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# packed FP32 instructions are disabled for all device code; multiply-adds
+# are fused only within one source expression (-ffp-contract=on), so kernels
+# that share an operator's code round it the same way whatever the register
+# allocation and block structure around it
+DEVFLAGS = ["--offload-arch=gfx950", "-ffp-contract=on", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 
 

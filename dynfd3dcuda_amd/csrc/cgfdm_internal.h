@@ -48,7 +48,8 @@ struct Params {
   int WaveIChunk = 0;   // planes per block of the stage kernel (0: automatic)
   int StageStreams = 0;  // 1: interior and free-surface launches of a stage on side streams
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
-  int WaveInteriorKernel = 1;  // interior tiles: 0 general kernel; 1 pipelined kernel (64 x 8); 2 pipelined, 64 x 6
+  int WaveInteriorKernel = -1;  // interior tiles: -1 automatic; 0 general kernel; pipelined kernel: 1 64 x 8 with
+                                // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
   int WaveBandKernel = 1;  // absorbing-band tiles: 0 general kernel; 1 pipelined kernel
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
@@ -192,6 +193,7 @@ struct Ctx {
   int wave_nwork = 0;                  // all work items
   bool wave_tall = false;           // interior tiles 64 x 16 (WaveTallInterior)
   bool wave_pipe = false;           // pipelined interior kernel (WaveInteriorKernel)
+  bool wave_ring = false;           // its LDS plane ring (WaveInteriorKernel 1)
   bool wave_band_pipe = false;      // pipelined kernel on the absorbing-band tiles (WaveBandKernel)
   int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)

@@ -41,6 +41,7 @@ constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j 
 constexpr int NTHRN = 256;  // threads per block of the narrow tiles
 constexpr int TKN = NTHRN / TJN;
 constexpr int TK6 = 6;     // rows of the 64 x 6 interior tiles (WaveInteriorKernel 2)
+constexpr int TK4 = 4;     // rows of the 64 x 4 interior tiles (WaveInteriorKernel 3)
 
 
 // Volume arrays are plane-interleaved with padded rows: word (field c,
@@ -557,7 +558,7 @@ __device__ __forceinline__ float met_raw_get(const uint32_t (&w)[met_raw_n<MT>()
 // two bands of an axis do not overlap), and added after the three axis sums
 // in the general kernel's order; 1/beta and the sponge factor of plane i+1
 // are fetched during plane i.
-template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB, bool BAND = false>
+template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB, bool BAND = false, bool RING = true>
 __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a) {
   constexpr int TJ = TJ_, TK = NT_ / TJ_;
   constexpr int LJ = TJ + 4, LK = TK + 4;
@@ -566,13 +567,23 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   // written nor read (the y and z operators reach along one axis only), so
   // each k-halo row costs the cache lines of the tile's own columns alone
   constexpr int NH = 4 * TJ + 4 * TK;
-  static_assert(NH <= NT_, "one halo cell per thread");
+  constexpr int NHC = (NH + NT_ - 1) / NT_;  // halo cells per thread
+  static_assert(NHC <= 2, "at most two halo cells per thread");
   constexpr int HJL = FY > 0 ? 1 : 3;
   constexpr int HKL = FZ > 0 ? 1 : 3;
   constexpr int QLO = FX > 0 ? -1 : -3;
   constexpr int CTR = -QLO;
   constexpr int NMW = met_raw_n<MT>();
-  __shared__ float lds[2][9][LK][LJ];
+  // RING: an LDS ring of the planes i .. i + LEAD: plane i + LEAD enters the
+  // x-queue during step i, and its y/z halo is loaded in the same step, so
+  // that a tile reads its neighbours' edge lines when they load those lines
+  // as their own queue plane (still in L2), not LEAD planes later.
+  // Otherwise a double buffer of planes i and i + 1 (LEAD 1), whose own
+  // values come from the queue: a third of the LDS, for blocks that share a CU.
+  constexpr int QLEAD = QLO + 5;  // the queue's newest plane is i + QLEAD
+  constexpr int LEAD = RING ? QLEAD : 1;
+  constexpr int NS = LEAD + 1;
+  __shared__ float lds[NS][9][LK][LJ];
   // BAND: the absorbing layers' uniform descriptors, staged once per block
   __shared__ const float* s_auxY[6];
   __shared__ const float* s_pBx;
@@ -597,23 +608,47 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   const float* __restrict__ Win = a.Win;
   const int irk = a.irk;
 
-  int hlk = 0, hlj = 0;
-  uint32_t hcol = col;
-  if (tid < NH) {
-    if (tid < 4 * TJ) {
-      const int r = tid / TJ;
-      hlk = r < HKL ? r : r + TK;
-      hlj = HJL + tid % TJ;
-    } else {
-      const int h2 = tid - 4 * TJ;
-      hlk = HKL + h2 / 4;
-      const int cc = h2 % 4;
-      hlj = cc < HJL ? cc : cc + TJ;
+  // this thread's halo cells tid and tid + NT_: the 4 k-halo rows, then the
+  // 4 j-halo cells of each row
+  int hlk[NHC], hlj[NHC];
+  uint32_t hcol4[NHC];
+#pragma unroll
+  for (int e = 0; e < NHC; e++) {
+    const int h = tid + e * NT_;
+    hlk[e] = 0; hlj[e] = 0;
+    uint32_t hcol = col;
+    if (h < NH) {
+      if (h < 4 * TJ) {
+        const int r = h / TJ;
+        hlk[e] = r < HKL ? r : r + TK;
+        hlj[e] = HJL + h % TJ;
+      } else {
+        const int h2 = h - 4 * TJ;
+        hlk[e] = HKL + h2 / 4;
+        const int cc = h2 % 4;
+        hlj[e] = cc < HJL ? cc : cc + TJ;
+      }
+      const int gj = min(max(j0 + hlj[e] - HJL, -3), nj + 2), gk = min(max(k0 + hlk[e] - HKL, -3), nk + 2);
+      hcol = jpad + (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * py;
     }
-    const int gj = min(max(j0 + hlj - HJL, -3), nj + 2), gk = min(max(k0 + hlk - HKL, -3), nk + 2);
-    hcol = jpad + (uint32_t)(gj + 3) + (uint32_t)(gk + 3) * py;
+    hcol4[e] = hcol << 2;
   }
-  const uint32_t hcol4 = hcol << 2;
+  auto load_halo = [&](Rsrc r, float (&v)[NHC][9]) {
+#pragma unroll
+    for (int e = 0; e < NHC; e++)
+      if (e == 0 || tid + e * NT_ < NH) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) v[e][c] = bld(r, hcol4[e], c * pb);
+      }
+  };
+  auto store_halo = [&](float (*L)[LK][LJ], const float (&v)[NHC][9]) {
+#pragma unroll
+    for (int e = 0; e < NHC; e++)
+      if (tid + e * NT_ < NH) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) L[c][hlk[e]][hlj[e]] = v[e][c];
+      }
+  };
 
   // BAND: per lane, the axes whose absorbing band (one side) holds the
   // point's column (j, k), their 1/beta, and the sponge factor of the column
@@ -650,39 +685,40 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
 #pragma unroll
     for (int c = 0; c < 9; c++) q[m][c] = bld(r, col4, c * pb);
   }
-  float hv[9];
-  {
-    const Rsrc r = plane_rsrc(Win, ibeg + 3, 9, nyz);
+  // planes ibeg .. ibeg + LEAD - 1 of the ring: own values from the queue,
+  // halos loaded here
+  float hv[NHC][9];
 #pragma unroll
-    for (int c = 0; c < 9; c++) hv[c] = bld(r, hcol4, c * pb);
+  for (int m = 0; m < LEAD; m++) {
+    if (ibeg + m < iend) {  // uniform per block
+      load_halo(plane_rsrc(Win, ibeg + m + 3, 9, nyz), hv);
+#pragma unroll
+      for (int c = 0; c < 9; c++) lds[m][c][tk + HKL][tj + HJL] = q[CTR + m][c];
+      store_halo(lds[m], hv);
+    }
   }
   uint32_t mw[NMW];
   met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw);
   float rb1 = 1.0f, cjx = 1.0f;
   if constexpr (BAND) { rb1 = rb_x(ibeg); cjx = cj_x(ibeg); }
-#pragma unroll
-  for (int c = 0; c < 9; c++) lds[0][c][tk + HKL][tj + HJL] = q[CTR][c];
-  if (tid < NH) {
-#pragma unroll
-    for (int c = 0; c < 9; c++) lds[0][c][hlk][hlj] = hv[c];
-  }
   __syncthreads();
 
+  int b = 0;  // ring slot of plane i; plane i + LEAD goes to the slot before it
   for (int i = ibeg; i < iend; ++i) {
-    const int b = (i - ibeg) & 1;
-    const bool more = i + 1 < iend;  // uniform per block
-    // plane i+1: halo, queue plane, metric (clamped in-bounds past the chunk)
-    {
-      const Rsrc r = plane_rsrc(Win, min(i + 4, nxp - 1), 9, nyz);
-#pragma unroll
-      for (int c = 0; c < 9; c++) hv[c] = bld(r, hcol4, c * pb);
-    }
+    const bool more = i + 1 < iend;       // uniform per block
+    const bool ahead = i + LEAD < iend;   // uniform per block
+    const int bw = b == 0 ? NS - 1 : b - 1;
+    // plane i + QLEAD: queue plane, and with RING its halo (clamped
+    // in-bounds past the chunk, where the queue plane goes unused); plane
+    // i + 1: the halo without RING, and the metric
     float nq[9];
     {
-      const Rsrc r = plane_rsrc(Win, min(i + QLO + 8, nxp - 1), 9, nyz);
+      const Rsrc r = plane_rsrc(Win, min(i + QLEAD + 3, nxp - 1), 9, nyz);
 #pragma unroll
       for (int c = 0; c < 9; c++) nq[c] = bld(r, col4, c * pb);
+      if (RING && ahead) load_halo(r, hv);
     }
+    if (!RING && ahead) load_halo(plane_rsrc(Win, i + 4, 9, nyz), hv);
     uint32_t mwn[NMW];
     met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn);
     float rb1n = 1.0f, cjxn = 1.0f;
@@ -804,14 +840,12 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
       }
     }
 
-    if (more) {
+    if (ahead) {
 #pragma unroll
-      for (int c = 0; c < 9; c++) lds[b ^ 1][c][tk + HKL][tj + HJL] = q[CTR + 1][c];
-      if (tid < NH) {
-#pragma unroll
-        for (int c = 0; c < 9; c++) lds[b ^ 1][c][hlk][hlj] = hv[c];
-      }
+      for (int c = 0; c < 9; c++) lds[bw][c][tk + HKL][tj + HJL] = RING ? nq[c] : q[CTR + 1][c];
+      store_halo(lds[bw], hv);
     }
+    b = b == NS - 1 ? 0 : b + 1;
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
@@ -965,9 +999,16 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     a.work = w; a.nwork = n[0];
     pr.begin(si, PK_STAGE_INTERIOR);
     if (c.wave_tki == TK6)
-      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK6, 2>), dim3(a.nwork), dim3(TJ, TK6, 1), 0, si, a);
-    else if (c.wave_pipe)
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK6, 2, false, false>), dim3(a.nwork), dim3(TJ, TK6, 1),
+                         0, si, a);
+    else if (c.wave_tki == TK4)  // at least 3 waves per SIMD: three blocks per CU
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK4, 3, false, false>), dim3(a.nwork), dim3(TJ, TK4, 1),
+                         0, si, a);
+    else if (c.wave_pipe && c.wave_ring)
       hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, si, a);
+    else if (c.wave_pipe)
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2, false, false>), dim3(a.nwork), dim3(TJ, TK, 1), 0,
+                         si, a);
     else if (c.wave_tall)
       hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 0, TJ, 2 * NTHR>), dim3(a.nwork), dim3(TJ, 2 * TK, 1), 0, si, a);
     else
@@ -1167,9 +1208,27 @@ int upload_wave_cold(Ctx& c) {
   const bool tall = p.WaveTallInterior != 0;
   c.wave_tall = tall;
   // WaveInteriorKernel: 0 the general kernel on the interior tiles, 1 the
-  // pipelined interior kernel (64 x 8 tiles), 2 the pipelined kernel on
-  // 64 x 6 tiles (384 threads, two blocks per CU)
-  c.wave_pipe = !tall && p.WaveInteriorKernel >= 1;
+  // pipelined interior kernel (64 x 8 tiles, LDS plane ring, one block per
+  // CU), 2 the pipelined kernel on 64 x 6 tiles (384 threads, two blocks per
+  // CU), 3 on 64 x 4 tiles (256 threads, three blocks per CU), 4 on 64 x 8
+  // tiles with an LDS double buffer instead of the ring
+  // WaveInteriorKernel -1 (default) picks 3 when the interior launch holds
+  // at least 16 rounds of resident 64 x 4 blocks (three per CU; fewer rounds
+  // leave the last ones ragged: 1600 x 1000 x 500, 88 k blocks: 90-95 ms of
+  // interior per step against 96-101 for 1; 400 x 300 x 200, 2 k blocks:
+  // 2.67 against 2.49 ms), else 1
+  int wik = p.WaveInteriorKernel;
+  if (wik < 0) {
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    const long ej = std::max(0, n3[1] - lo[1] - hi[1]), ek = std::max(0, n3[2] - lo[2] - hi[2]);
+    const long span = std::max(0, n3[0] - lo[0] - hi[0]);
+    const long blocks4 = (ej / TJ) * (ek / TK4) * ((span + 31) / 32);
+    wik = blocks4 >= 16L * 3 * cus ? 3 : 1;
+  }
+  c.wave_pipe = !tall && wik >= 1;
+  c.wave_ring = wik == 1;
   // WaveBandKernel 1: the pipelined kernel on the band tiles too, which adds
   // the aux terms of one face per axis: used when the two ADE-PML bands of
   // every axis are disjoint (2 N <= the axis length)
@@ -1178,7 +1237,7 @@ int upload_wave_cold(Ctx& c) {
     if (c.pml.on[2 * ax] && c.pml.on[2 * ax + 1] && 2 * c.pml.N > (ax == 0 ? c.d.ni : ax == 1 ? c.d.nj : c.d.nk))
       disjoint = false;
   c.wave_band_pipe = p.WaveBandKernel >= 1 && disjoint;
-  const int tki = tall ? 2 * TK : (c.wave_pipe && p.WaveInteriorKernel == 2) ? TK6 : TK;  // interior tile rows
+  const int tki = tall ? 2 * TK : (c.wave_pipe && wik == 2) ? TK6 : (c.wave_pipe && wik == 3) ? TK4 : TK;  // interior tile rows
   c.wave_tki = tki;
   for (int ax = 1; ax < 3; ax++) {
     const int t = ax == 1 ? TJ : tki, e = n3[ax] - lo[ax] - hi[ax];

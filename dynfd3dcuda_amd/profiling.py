@@ -13,9 +13,9 @@
       same configuration)
 
 Kernel classes: stage_interior / stage_band / stage_surface (the
-instantiations of the wave stage kernel, stage_interior also the pipelined
-interior kernel; stage_surface includes the two free-surface precompute
-passes), pml_aux, fault_pre, fault_rs, fault_deriv,
+instantiations of the wave stage kernel and of the pipelined stage kernel,
+whose BAND instantiations are band tiles; stage_surface includes the two
+free-surface precompute passes), pml_aux, fault_pre, fault_rs, fault_deriv,
 fault_post, rup_sensor, thermpress, setup_metric (the setup's metric
 pass); everything else is listed under its own name.
 """
@@ -39,8 +39,12 @@ def kernel_class(name):
     m = re.search(r"wave_stage_kernelI(?:Li-?\d+E|Lin1E){3}[ft]Li([012])E", name)  # mangled form
     if m:
         return STAGE_CLASSES[int(m.group(1))]
+    m = re.search(r"wave_interior_kernel<([^>]*)>", name)
+    if m:  # the pipelined kernel: FX, FY, FZ, metric type, tile width, threads, waves per SIMD[, BAND, RING]
+        args = [x.strip() for x in m.group(1).split(",")]
+        return "stage_band" if len(args) > 7 and args[7] == "true" else "stage_interior"
     if "wave_interior_kernel" in name:
-        return "stage_interior"  # the pipelined interior kernel
+        return "stage_interior"
     if "traction_top_kernel" in name or "image_top_kernel" in name:
         return "stage_surface"  # the free-surface precompute passes
     for k in ("pml_aux_kernel", "fault_pre_kernel", "fault_rs_kernel", "fault_deriv_kernel", "fault_post_kernel"):
