@@ -772,9 +772,7 @@ int cgfdm_open_outputs(cgfdm_ctx* h) {
   mkdir(c.p.OUT.c_str(), 0755);
   NcOut* o = new NcOut();
   std::string err;
-  const int nj = c.d.nj, nk = c.d.nk, ni = c.d.ni, nx = c.d.nx, ny = c.d.ny, nz = c.d.nz;
-  const long nxyz = c.d.nxyz, nyz = c.d.nyz;
-  auto cpos = [&](int i, int j, int k) { return (long)j + (long)k * ny + (long)i * nyz; };
+  const int nj = c.d.nj, nk = c.d.nk, ni = c.d.ni;
   if (c.p.Rupture) {
     NcFile& f = o->fault;
     if (!f.create(c.p.OUT + "/fault_mpi000000.nc", err)) { delete o; return fail(CGFDM_ENETCDF, err); }
@@ -791,12 +789,7 @@ int cgfdm_open_outputs(cgfdm_ctx* h) {
     if (!f.end_def(err)) { delete o; return fail(CGFDM_ENETCDF, err); }
     std::vector<float> buf((size_t)nj * nk);
     std::vector<int> ib((size_t)nj * nk);
-    const double* C = c.hostC.data();
-    for (int comp = 0; comp < 3; comp++) {
-      for (int k = 0; k < nk; k++)
-        for (int j = 0; j < nj; j++) buf[j + (long)k * nj] = (float)C[comp * nxyz + cpos(c.d.i0, j + 3, k + 3)];
-      f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, buf.data());
-    }
+    for (int comp = 0; comp < 3; comp++) f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, &c.out_coord[0][comp * (size_t)nj * nk]);
     auto put2 = [&](const char* fld, int vid) -> int {
       FieldRef r = field(c, fld);
       if (hipMemcpy(buf.data(), r.ptr, buf.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return fail(CGFDM_EHIP, "copy failed");
@@ -818,19 +811,9 @@ int cgfdm_open_outputs(cgfdm_ctx* h) {
     int vx = f.def_var("x", NC_FLOAT_, d2), vy = f.def_var("y", NC_FLOAT_, d2), vz = f.def_var("z", NC_FLOAT_, d2);
     vids[0] = f.def_var("Vx", NC_FLOAT_, d3); vids[1] = f.def_var("Vy", NC_FLOAT_, d3); vids[2] = f.def_var("Vz", NC_FLOAT_, d3);
     if (!f.end_def(err)) return false;
-    std::vector<float> buf((size_t)d1n * d2n);
-    const double* C = c.hostC.data();
-    for (int comp = 0; comp < 3; comp++) {
-      for (int a = 0; a < d1n; a++)
-        for (int b = 0; b < d2n; b++) {
-          long p;
-          if (axis == 2) p = cpos(a + 3, b + 3, idx + 3);       // (nx, ny) at k
-          else if (axis == 1) p = cpos(a + 3, idx + 3, b + 3);  // (nx, nz) at j
-          else p = cpos(idx + 3, b + 3, a + 3);                 // (nz, ny) at i
-          buf[(long)a * d2n + b] = (float)C[comp * nxyz + p];
-        }
-      f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, buf.data());
-    }
+    // the slice's coordinates, gathered at setup: xy (nx, ny) at k, xz (nx, nz) at j, yz (nz, ny) at i
+    const std::vector<float>& C = c.out_coord[axis == 2 ? 1 : axis == 1 ? 2 : 3];
+    for (int comp = 0; comp < 3; comp++) f.put_var(comp == 0 ? vx : comp == 1 ? vy : vz, &C[comp * (size_t)d1n * d2n]);
     return true;
   };
   int kz = std::min(std::max(c.p.EXPORT_WAVE_SLICE_Z, 0), nk - 1);
@@ -842,7 +825,6 @@ int cgfdm_open_outputs(cgfdm_ctx* h) {
     delete o;
     return fail(CGFDM_ENETCDF, err);
   }
-  (void)nx; (void)nz;
   c.out = o;
   return 0;
 }

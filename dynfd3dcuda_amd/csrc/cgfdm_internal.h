@@ -211,7 +211,8 @@ struct Ctx {
   unsigned long long* check_counts = nullptr;  // counters of cgfdm_check_field (3 words)
   std::vector<void*> allocations;
   // host copies kept from setup
-  std::vector<double> hostC;    // coordinates [3][nxyz] (used for outputs)
+  std::vector<float> out_coord[4];  // coordinates x, y, z of the output files' grids: fault plane [3][nk][nj],
+                                    // wave slices xy [3][ni][nj], xz [3][ni][nk], yz [3][nk][nj]
   double hmin = 0, vpmax = 0;
   // outputs
   struct NcOut* out = nullptr;

@@ -23,7 +23,7 @@ namespace {
 
 const double PI_ = 3.141592653589793238463;
 
-void inv3(double m[3][3]) {
+__host__ __device__ inline void inv3(double m[3][3]) {
   double v[3][3];
   v[0][0] = m[1][1] * m[2][2] - m[2][1] * m[1][2];
   v[0][1] = m[2][1] * m[0][2] - m[0][1] * m[2][2];
@@ -38,32 +38,25 @@ void inv3(double m[3][3]) {
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++) m[a][b] = v[a][b] / det;
 }
-void mul3(const double A[3][3], const double B[3][3], double C[3][3]) {
+__host__ __device__ inline void mul3(const double A[3][3], const double B[3][3], double C[3][3]) {
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++) C[a][b] = A[a][0] * B[0][b] + A[a][1] * B[1][b] + A[a][2] * B[2][b];
 }
-inline void cross(const double* a, const double* b, double* c) {
+__host__ __device__ inline void cross(const double* a, const double* b, double* c) {
   c[0] = a[1] * b[2] - a[2] * b[1];
   c[1] = a[2] * b[0] - a[0] * b[2];
   c[2] = a[0] * b[1] - a[1] * b[0];
 }
-inline double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-inline double nrm(const double* a) { return std::sqrt(dot(a, a)); }
+__host__ __device__ inline double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__host__ __device__ inline double nrm(const double* a) { return sqrt(dot(a, a)); }
 
 // Distance from x0 to the plane through x1 x2 x3 (mathFuncs.cu:73-98).
-double dist_point_plane(const double* x0, const double* x1, const double* x2, const double* x3) {
+__host__ __device__ inline double dist_point_plane(const double* x0, const double* x1, const double* x2, const double* x3) {
   double u[3] = {x2[0] - x1[0], x2[1] - x1[1], x2[2] - x1[2]};
   double v[3] = {x3[0] - x1[0], x3[1] - x1[1], x3[2] - x1[2]};
   double p[3];
   cross(u, v, p);
-  return std::fabs(dot(p, x0) - dot(p, x1)) / nrm(p);
-}
-
-uint16_t to_bf16(float f) {  // round to nearest even
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  uint32_t r = u + 0x7fff + ((u >> 16) & 1);
-  return (uint16_t)(r >> 16);
+  return fabs(dot(p, x0) - dot(p, x1)) / nrm(p);
 }
 
 // Metric and media of one interior point (cal_metric, metric.cu:74-109;
@@ -179,6 +172,322 @@ __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, 
   if ((threadIdx.x & 63) == 0 && vp > 0.0) atomicMax(vpmax_bits, (unsigned long long)__double_as_longlong(vp));
 }
 
+// ---- setup passes on the device, FP64 (one thread per point or node)
+
+// Curvilinear coordinates of the interior points (construct_coord,
+// coord.cu:153-328): the x line (grid refinement normal to the fault, built
+// on the host), the fault-surface geometry of INPORT_GRID_TYPE 1 and the
+// topography stretched over the depth.  Ghost points follow in
+// coord_ghost_kernel.
+struct CoordIn {
+  const double* xline;  // NX + 6 padded x positions (NormalRefine)
+  const float* gx;      // [nk][nj] fault geometry (INPORT_GRID_TYPE 1)
+  const float* gy;
+  const float* gz;
+  const float* topo;    // [ni][nj] surface height, or null
+  int nx, ny, nz, nj, NX, NY, NZ;
+  long nyz;
+  double DH;
+  int normal_refine, grid_type;
+};
+__global__ void __launch_bounds__(256) coord_kernel(CoordIn c, double* X, double* Y, double* Z) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const int mj = c.ny - 6, mk = c.nz - 6;
+  if (idx >= (long)(c.nx - 6) * mk * mj) return;
+  const int j = 3 + (int)(idx % mj);
+  const long r = idx / mj;
+  const int k = 3 + (int)(r % mk), i = 3 + (int)(r / mk);
+  const int gi = i - 3, gj = j - 3, gk = k - 3;
+  const double x0 = (-c.NX / 2) * c.DH, y0 = (-c.NY / 2) * c.DH, z0 = (1 - c.NZ) * c.DH;
+  double x = gi * c.DH + x0, y = gj * c.DH + y0, z = gk * c.DH + z0;
+  if (c.normal_refine) x = c.xline[i];
+  if (c.grid_type == 1) {
+    const long f = (j - 3) + (long)(k - 3) * c.nj;
+    x = (c.normal_refine ? c.xline[i] : gi * c.DH + x0) + c.gx[f];
+    y = c.gy[f];
+    z = c.gz[f];
+  }
+  if (c.topo) z += c.topo[(long)gi * c.nj + gj] * (double)gk / (double)(c.NZ - 1);
+  const long q = j + (long)k * c.ny + (long)i * c.nyz;
+  X[q] = x; Y[q] = y; Z[q] = z;
+}
+
+// Linear extrapolation into the three ghost layers along one axis of every
+// coordinate array (metric.cu:248-313): x for all (j, k), then y for all
+// (i, k), then z for all (i, j), each pass on the previous one's output.
+// blockIdx.y is the coordinate component.
+__global__ void __launch_bounds__(256) coord_ghost_kernel(double* C, long nxyz, int nx, int ny, int nz, long nyz, int axis) {
+  const long line = (long)blockIdx.x * 256 + threadIdx.x;
+  double* A = C + (long)blockIdx.y * nxyz;
+  const int n = axis == 0 ? nx : axis == 1 ? ny : nz;
+  long base, stride;
+  if (axis == 0) {  // line over (j, k)
+    if (line >= (long)ny * nz) return;
+    base = line; stride = nyz;
+  } else if (axis == 1) {  // line over (i, k)
+    if (line >= (long)nx * nz) return;
+    base = (line / nz) * nyz + (line % nz) * ny; stride = 1;
+  } else {  // line over (i, j)
+    if (line >= (long)nx * ny) return;
+    base = (line / ny) * nyz + line % ny; stride = ny;
+  }
+  for (int m = 1; m <= 3; m++) {
+    A[base + (3 - m) * stride] = 2.0 * A[base + 3 * stride] - A[base + (3 + m) * stride];
+    A[base + (n - 4 + m) * stride] = 2.0 * A[base + (n - 4) * stride] - A[base + (n - 4 - m) * stride];
+  }
+}
+
+// Keys that order doubles of either sign like unsigned integers, for
+// atomic min / max.
+__device__ inline unsigned long long order_key(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+inline double from_order_key(unsigned long long k) {
+  const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double v;
+  std::memcpy(&v, &b, sizeof v);
+  return v;
+}
+
+// Smallest distance from an interior point to the planes through its axis
+// neighbours (cal_range_steph, coord.cu:330-391), for the CFL check.
+__global__ void __launch_bounds__(256) hmin_kernel(const double* X, const double* Y, const double* Z, int nx, int ny, int nz,
+                                                   long nyz, unsigned long long* out) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const int mj = ny - 6, mk = nz - 6;
+  double h = 1e38;
+  if (idx < (long)(nx - 6) * mk * mj) {
+    const int j = 3 + (int)(idx % mj);
+    const long r = idx / mj;
+    const int k = 3 + (int)(r % mk), i = 3 + (int)(r / mk);
+    const long q = j + (long)k * ny + (long)i * nyz;
+    const double x0[3] = {X[q], Y[q], Z[q]};
+    for (int ii = -1; ii <= 1; ii += 2)
+      for (int jj = -1; jj <= 1; jj += 2)
+        for (int kk = -1; kk <= 1; kk += 2) {
+          const long q1 = q + ii * nyz, q2 = q + jj, q3 = q + (long)kk * ny;
+          const double x1[3] = {X[q1], Y[q1], Z[q1]}, x2[3] = {X[q2], Y[q2], Z[q2]}, x3[3] = {X[q3], Y[q3], Z[q3]};
+          h = fmin(h, dist_point_plane(x0, x1, x2, x3));
+        }
+  }
+  for (int o = 32; o > 0; o >>= 1) h = fmin(h, __shfl_xor(h, o));
+  if ((threadIdx.x & 63) == 0) atomicMin(out, order_key(h));
+}
+
+// Gathers na x nb doubles src[base + a * sa + b * sb] of each of the three
+// coordinate arrays into dst[comp][a][b] (the output files' coordinate slices).
+__global__ void __launch_bounds__(256) gather_kernel(const double* C, long nxyz, long base, long sa, long sb, int na, int nb,
+                                                     float* dst) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)na * nb) return;
+  const long a = idx / nb, b = idx % nb;
+  for (int comp = 0; comp < 3; comp++) dst[comp * (long)na * nb + idx] = (float)C[comp * nxyz + base + a * sa + b * sb];
+}
+
+// Per-field range of the metric words over a sample of the stored points
+// (every 4th index along each axis and the last, ghosts mirrored): the
+// reference value of each field is its midrange (MetricBF16 = 2).
+// out[2 q] min key, out[2 q + 1] max key of field q.
+__global__ void __launch_bounds__(256) metric_range_kernel(MetricIn m, const int* ai, const int* aj, const int* ak, int ni_,
+                                                           int nj_, int nk_, unsigned long long* out) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const bool ok = idx < (long)ni_ * nj_ * nk_;
+  double v[13];
+  for (int q = 0; q < 13; q++) v[q] = 0.0;
+  if (ok) {
+    const int j = aj[idx % nj_];
+    const long r = idx / nj_;
+    metric_point(m, ai[r / nk_], j, ak[r % nk_], v);
+  }
+  for (int q = 0; q < 10; q++) {
+    unsigned long long lo = ok ? order_key(v[q]) : ~0ull, hi = ok ? order_key(v[q]) : 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+      lo = a < lo ? a : lo;
+      hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicMin(&out[2 * q], lo);
+      atomicMax(&out[2 * q + 1], hi);
+    }
+  }
+}
+
+// Split-node coefficients of one node (j, k) of the padded fault plane
+// (fault_coef.cu:26-1302): the unit normal and strike/dip vectors, the
+// scaled Jacobian, the media of both sides, the traction matrices D_ab of
+// both sides and the maps T1 -> Vx, Vy -> Vx, Vz -> Vx, and on the row
+// under the free surface the split-node velocity maps Vx, Vy -> Vz and the
+// side-to-side maps P2M / M2P (fault_coef.cu:855-1010).
+struct FaultCoefOut {
+  float* rho_f;   // [2][nyz]
+  float* Dm;      // [2 * 9][nyz][9]
+  float* T1x;     // [2][nyz][9]
+  float* Vyx;
+  float* Vzx;
+  float* Vx2Vzf;  // [2][ny][9]
+  float* Vy2Vzf;
+  float* P2M;     // [3][ny][9]
+  float* M2P;
+  float* vn;      // [nyz][3]
+  float* vs1;
+  float* vs2;
+  float* jacvec;  // [nyz]
+  float* jac0;
+  double* nd;     // [nyz][9]: n, s1, s2 in FP64 (the initial stress projection)
+};
+__global__ void __launch_bounds__(256) fault_coef_kernel(MetricIn md, int i0, FaultCoefOut o) {
+  const long pf = (long)blockIdx.x * 256 + threadIdx.x;
+  const int ny = md.ny, nz = md.nz, nx = md.nx;
+  const long nyz = md.nyz;
+  if (pf >= nyz) return;
+  const int j = (int)(pf % ny), k = (int)(pf / ny);
+  double v0[13], vm[13], vp[13];
+  metric_point(md, mirror_idx(i0, nx), mirror_idx(j, ny), mirror_idx(k, nz), v0);
+  metric_point(md, mirror_idx(i0 - 1, nx), mirror_idx(j, ny), mirror_idx(k, nz), vm);
+  metric_point(md, mirror_idx(i0 + 1, nx), mirror_idx(j, ny), mirror_idx(k, nz), vp);
+  double e[3][3];
+  for (int a = 0; a < 9; a++) e[a / 3][a % 3] = v0[a];
+  const double jac = v0[9];
+  double mt[3][3];
+  for (int a = 0; a < 9; a++) mt[a / 3][a % 3] = e[a / 3][a % 3];
+  inv3(mt);
+  const double x_et[3] = {mt[0][1], mt[1][1], mt[2][1]};
+  double s1[3] = {x_et[0], x_et[1], x_et[2]}, n[3] = {e[0][0], e[0][1], e[0][2]}, s2[3];
+  const double ns1 = nrm(s1), nn = nrm(n);
+  for (int a = 0; a < 3; a++) { s1[a] /= ns1; n[a] /= nn; }
+  cross(n, s1, s2);
+  for (int a = 0; a < 3; a++) {
+    o.vn[pf * 3 + a] = (float)n[a]; o.vs1[pf * 3 + a] = (float)s1[a]; o.vs2[pf * 3 + a] = (float)s2[a];
+    o.nd[pf * 9 + a] = n[a]; o.nd[pf * 9 + 3 + a] = s1[a]; o.nd[pf * 9 + 6 + a] = s2[a];
+  }
+  o.jacvec[pf] = (float)(jac * nn);
+  o.jac0[pf] = (float)jac;
+  const double lamf[2] = {vm[10], vp[10]}, muf[2] = {vm[11], vp[11]};
+  o.rho_f[pf] = (float)vm[12];
+  o.rho_f[nyz + pf] = (float)vp[12];
+  const bool surf = k == nz - 4;
+  double V2Z[2][3][3], VY2Z[2][3][3], Dfb[2][3][3][3];  // surface row: the D_1b of both sides
+  for (int m = 0; m < 2; m++) {
+    double D1[3][3][3];  // D_1b, b = 0..2
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        const double eab = dot(e[a], e[b]);
+        double Dab[3][3];
+        for (int r = 0; r < 3; r++)
+          for (int cc = 0; cc < 3; cc++) {
+            const double v = jac * (lamf[m] * e[a][r] * e[b][cc] + muf[m] * e[a][cc] * e[b][r] + (r == cc ? muf[m] * eab : 0.0));
+            Dab[r][cc] = v;
+            o.Dm[(((long)(m * 9 + a * 3 + b)) * nyz + pf) * 9 + r * 3 + cc] = (float)v;
+          }
+        if (a == 0)
+          for (int r = 0; r < 9; r++) D1[b][r / 3][r % 3] = Dab[r / 3][r % 3];
+        if (surf && a == 2 && b == 2) {
+          double A[3][3];
+          for (int r = 0; r < 9; r++) A[r / 3][r % 3] = Dab[r / 3][r % 3];
+          inv3(A);
+          for (int r = 0; r < 9; r++) V2Z[m][r / 3][r % 3] = A[r / 3][r % 3];  // (D_33)^-1 for now
+        }
+        if (surf && a == 2 && b == 0)
+          for (int r = 0; r < 9; r++) VY2Z[m][r / 3][r % 3] = -Dab[r / 3][r % 3];  // -D_31 for now
+        if (surf && a == 2 && b == 1)
+          for (int r = 0; r < 9; r++) Dfb[m][0][r / 3][r % 3] = -Dab[r / 3][r % 3];  // -D_32 for now
+      }
+    double D11[3][3], oo[3][3];
+    for (int r = 0; r < 9; r++) D11[r / 3][r % 3] = D1[0][r / 3][r % 3];
+    inv3(D11);
+    for (int a = 0; a < 9; a++) o.T1x[(m * nyz + pf) * 9 + a] = (float)D11[a / 3][a % 3];
+    mul3(D11, D1[1], oo);
+    for (int a = 0; a < 9; a++) o.Vyx[(m * nyz + pf) * 9 + a] = (float)oo[a / 3][a % 3];
+    mul3(D11, D1[2], oo);
+    for (int a = 0; a < 9; a++) o.Vzx[(m * nyz + pf) * 9 + a] = (float)oo[a / 3][a % 3];
+    if (surf) {
+      // Vx -> Vz = D_33^-1 (-D_31), Vy -> Vz = D_33^-1 (-D_32)
+      double Ainv[3][3], B[3][3], Cc[3][3];
+      for (int r = 0; r < 9; r++) {
+        Ainv[r / 3][r % 3] = V2Z[m][r / 3][r % 3];
+        B[r / 3][r % 3] = VY2Z[m][r / 3][r % 3];
+        Cc[r / 3][r % 3] = Dfb[m][0][r / 3][r % 3];
+      }
+      mul3(Ainv, B, V2Z[m]);
+      mul3(Ainv, Cc, VY2Z[m]);
+      for (int a = 0; a < 9; a++) {
+        o.Vx2Vzf[((long)m * ny + j) * 9 + a] = (float)V2Z[m][a / 3][a % 3];
+        o.Vy2Vzf[((long)m * ny + j) * 9 + a] = (float)VY2Z[m][a / 3][a % 3];
+      }
+      for (int b = 0; b < 3; b++)
+        for (int r = 0; r < 9; r++) Dfb[m][b][r / 3][r % 3] = D1[b][r / 3][r % 3];
+    }
+  }
+  if (!surf) return;
+  double v1[3] = {e[0][0], e[0][1], e[0][2]}, v3[3];
+  const double v1n = nrm(v1);
+  cross(v1, x_et, v3);
+  const double h[3] = {dot(v3, e[0]) / v1n, dot(v3, e[1]) / v1n, dot(v3, e[2]) / v1n};
+  const double miu = v0[11];
+  double Vxf[2][3][3], Vyf[2][3][3];
+  for (int m = 0; m < 2; m++) {
+    double Df[3][3][3];
+    for (int b = 0; b < 3; b++)
+      for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++) Df[b][r][cc] = Dfb[m][b][r][cc] + miu * h[b] * v3[r] * v1[cc];
+    double m1[3][3], m2[3][3];
+    mul3(Df[2], V2Z[m], m1);
+    mul3(Df[2], VY2Z[m], m2);
+    for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 3; cc++) { Vxf[m][r][cc] = Df[0][r][cc] + m1[r][cc]; Vyf[m][r][cc] = Df[1][r][cc] + m2[r][cc]; }
+  }
+  double Ti[3][3], oo[3][3];
+  for (int r = 0; r < 9; r++) Ti[r / 3][r % 3] = Vxf[0][r / 3][r % 3];
+  inv3(Ti);
+  mul3(Ti, Vxf[1], oo); for (int a = 0; a < 9; a++) o.P2M[(0L * ny + j) * 9 + a] = (float)oo[a / 3][a % 3];
+  mul3(Ti, Vyf[1], oo); for (int a = 0; a < 9; a++) o.P2M[(1L * ny + j) * 9 + a] = (float)oo[a / 3][a % 3];
+  mul3(Ti, Vyf[0], oo); for (int a = 0; a < 9; a++) o.P2M[(2L * ny + j) * 9 + a] = (float)oo[a / 3][a % 3];
+  for (int r = 0; r < 9; r++) Ti[r / 3][r % 3] = Vxf[1][r / 3][r % 3];
+  inv3(Ti);
+  mul3(Ti, Vxf[0], oo); for (int a = 0; a < 9; a++) o.M2P[(0L * ny + j) * 9 + a] = (float)oo[a / 3][a % 3];
+  mul3(Ti, Vyf[0], oo); for (int a = 0; a < 9; a++) o.M2P[(1L * ny + j) * 9 + a] = (float)oo[a / 3][a % 3];
+  mul3(Ti, Vyf[1], oo); for (int a = 0; a < 9; a++) o.M2P[(2L * ny + j) * 9 + a] = (float)oo[a / 3][a % 3];
+}
+
+// Free-surface velocity maps of the top row (init_wave_free_cu,
+// fault_coef.cu:1320-1486): with K_b the traction on the zeta surface from
+// the derivatives along axis b, scaled by the PML 1/beta of that axis,
+// DzV = K_3^-1 (-K_1) DxV + K_3^-1 (-K_2) DyV.  One thread per (i, j).
+__global__ void __launch_bounds__(256) wave_free_kernel(MetricIn md, const double* B0, const double* B1, const double* B2,
+                                                        float* mVx, float* mVy) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const int nx = md.nx, ny = md.ny, nz = md.nz;
+  if (idx >= (long)(nx - 6) * (ny - 6)) return;
+  const int j = 3 + (int)(idx % (ny - 6)), i = 3 + (int)(idx / (ny - 6)), k = nz - 4;
+  const double rb[3] = {1.0 / B0[i - 3], 1.0 / B1[j - 3], 1.0 / B2[k - 3]};
+  double v[13];
+  metric_point(md, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(k, nz), v);
+  double e[3][3];
+  for (int a = 0; a < 9; a++) e[a / 3][a % 3] = v[a];
+  const double lam = v[10], mu = v[11];
+  double Ainv[3][3], Bm[3][3], Cm[3][3];
+  for (int b = 0; b < 3; b++) {
+    const double eb3 = dot(e[2], e[b]);
+    for (int r = 0; r < 3; r++)
+      for (int cc = 0; cc < 3; cc++) {
+        const double K = rb[b] * (lam * e[2][r] * e[b][cc] + mu * e[2][cc] * e[b][r] + (r == cc ? mu * eb3 : 0.0));
+        if (b == 2) Ainv[r][cc] = K;
+        else if (b == 0) Bm[r][cc] = -K;
+        else Cm[r][cc] = -K;
+      }
+  }
+  inv3(Ainv);
+  double AB[3][3], AC[3][3];
+  mul3(Ainv, Bm, AB);
+  mul3(Ainv, Cm, AC);
+  for (int a = 0; a < 9; a++) {
+    mVx[(j + (long)i * ny) * 9 + a] = (float)AB[a / 3][a % 3];
+    mVy[(j + (long)i * ny) * 9 + a] = (float)AC[a / 3][a % 3];
+  }
+}
+
 template <class T>
 T* dmalloc(Ctx& c, size_t n) {
   void* p = nullptr;
@@ -230,7 +539,6 @@ int host_setup(Ctx& c, std::string& err) {
   if (p.MetricBF16 < 0 || p.MetricBF16 > 2) { err = "MetricBF16 must be 0, 1 or 2"; return 1; }
   const int nx = d.nx, ny = d.ny, nz = d.nz, ni = d.ni, nj = d.nj, nk = d.nk;
   const long nxyz = d.nxyz, nyz = d.nyz;
-  auto P = [&](int i, int j, int k) { return j + (long)k * ny + (long)i * nyz; };
   const int freenode = p.FreeSurface ? 1 : 0;
   int on[6] = {1, 1, 1, 1, 1, freenode ? 0 : 1};  // single rank: every face absorbs except a free top
 
@@ -271,13 +579,34 @@ int host_setup(Ctx& c, std::string& err) {
     }
   }
 
-  // ---- coordinates (coord.cu:153-328) and ghost extrapolation (metric.cu:248-313)
-  c.hostC.assign(3 * nxyz, 0.0);
-  double* X = &c.hostC[0];
-  double* Y = &c.hostC[nxyz];
-  double* Z = &c.hostC[2 * nxyz];
-  std::vector<double> xline(p.NX + 6);
+  // Device copies of the setup inputs and the FP64 coordinates, freed at the
+  // end of the setup (the run needs neither).
+  struct DevTemps {
+    std::vector<void*> v;
+    ~DevTemps() { for (void* q : v) (void)hipFree(q); }
+  } tmp;
+  auto dput = [&](const void* src, size_t bytes) -> void* {
+    void* q = nullptr;
+    if (bytes == 0) return nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
+    tmp.v.push_back(q);
+    if (src && hipMemcpy(q, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return q;
+  };
+  auto blocks_for = [](long n) { return (unsigned)((n + 255) / 256); };
+
+  // ---- coordinates (coord.cu:153-328) and ghost extrapolation (metric.cu:248-313), on the device
+  if (p.INPORT_GRID_TYPE == 1 && c.coord3d.empty() &&
+      (c.gx.size() < (size_t)nj * nk || c.gy.size() < (size_t)nj * nk || c.gz.size() < (size_t)nj * nk)) {
+    err = "fault geometry smaller than NY*NZ";
+    return 4;
+  }
+  if (!c.topo.empty() && c.topo.size() < (size_t)ni * nj) { err = "topography smaller than NX*NY"; return 4; }
+  if (!c.coord3d.empty() && c.coord3d.size() < 3 * (size_t)nxyz) { err = "3-D coordinates smaller than 3 x the padded grid"; return 4; }
+  double* dC = (double*)dput(nullptr, 3 * nxyz * sizeof(double));
+  if (!dC) { err = "hipMalloc failed for the coordinates"; return 6; }
   {
+    std::vector<double> xline(p.NX + 6);
     int ic = p.NX / 2 + 3;
     xline[ic] = 0;
     auto spacing = [&](int i, int ref) {
@@ -287,109 +616,98 @@ int host_setup(Ctx& c, std::string& err) {
     };
     for (int i = ic + 1; i < p.NX + 6; i++) xline[i] = xline[i - 1] + spacing(i, ic + 10);
     for (int i = ic - 1; i >= 0; i--) xline[i] = xline[i + 1] - spacing(i, ic - 10);
-  }
-#pragma omp parallel for
-  for (int i = 3; i < nx - 3; i++)
-    for (int k = 3; k < nz - 3; k++)
-      for (int j = 3; j < ny - 3; j++) {
-        long q = P(i, j, k);
-        if (!c.coord3d.empty()) { X[q] = c.coord3d[q]; Y[q] = c.coord3d[nxyz + q]; Z[q] = c.coord3d[2 * nxyz + q]; continue; }
-        int gi = i - 3, gj = j - 3, gk = k - 3;
-        double x0 = (-p.NX / 2) * p.DH, y0 = (-p.NY / 2) * p.DH, z0 = (1 - p.NZ) * p.DH;
-        double x = gi * p.DH + x0, y = gj * p.DH + y0, z = gk * p.DH + z0;
-        if (p.NormalRefine) x = xline[i];
-        if (p.INPORT_GRID_TYPE == 1) {
-          long f = (j - 3) + (long)(k - 3) * nj;
-          x = (p.NormalRefine ? xline[i] : gi * p.DH + x0) + c.gx[f];
-          y = c.gy[f];
-          z = c.gz[f];
-        }
-        if (!c.topo.empty()) z += c.topo[(long)gi * nj + gj] * (double)gk / (double)(p.NZ - 1);
-        X[q] = x; Y[q] = y; Z[q] = z;
+    bool ok = true;
+    if (!c.coord3d.empty()) {  // interior points as given; the ghosts follow below
+      ok = hipMemcpy(dC, c.coord3d.data(), 3 * nxyz * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+    } else {
+      CoordIn ci{};
+      ci.xline = (const double*)dput(xline.data(), xline.size() * sizeof(double));
+      if (p.INPORT_GRID_TYPE == 1) {
+        ci.gx = (const float*)dput(c.gx.data(), c.gx.size() * sizeof(float));
+        ci.gy = (const float*)dput(c.gy.data(), c.gy.size() * sizeof(float));
+        ci.gz = (const float*)dput(c.gz.data(), c.gz.size() * sizeof(float));
+        ok = ci.gx && ci.gy && ci.gz;
       }
-  for (int l = 0; l < 3; l++) {
-    double* A = &c.hostC[l * nxyz];
-    for (int n = 1; n <= 3; n++)
-      for (int k = 0; k < nz; k++)
-        for (int j = 0; j < ny; j++) {
-          A[P(3 - n, j, k)] = 2.0 * A[P(3, j, k)] - A[P(3 + n, j, k)];
-          A[P(nx - 4 + n, j, k)] = 2.0 * A[P(nx - 4, j, k)] - A[P(nx - 4 - n, j, k)];
-        }
-    for (int i = 0; i < nx; i++)
-      for (int k = 0; k < nz; k++)
-        for (int n = 1; n <= 3; n++) {
-          A[P(i, 3 - n, k)] = 2.0 * A[P(i, 3, k)] - A[P(i, 3 + n, k)];
-          A[P(i, ny - 4 + n, k)] = 2.0 * A[P(i, ny - 4, k)] - A[P(i, ny - 4 - n, k)];
-        }
-    for (int i = 0; i < nx; i++)
-      for (int n = 1; n <= 3; n++)
-        for (int j = 0; j < ny; j++) {
-          A[P(i, j, 3 - n)] = 2.0 * A[P(i, j, 3)] - A[P(i, j, 3 + n)];
-          A[P(i, j, nz - 4 + n)] = 2.0 * A[P(i, j, nz - 4)] - A[P(i, j, nz - 4 - n)];
-        }
+      if (!c.topo.empty()) { ci.topo = (const float*)dput(c.topo.data(), c.topo.size() * sizeof(float)); ok = ok && ci.topo; }
+      ci.nx = nx; ci.ny = ny; ci.nz = nz; ci.nj = nj; ci.NX = p.NX; ci.NY = p.NY; ci.NZ = p.NZ;
+      ci.nyz = nyz; ci.DH = p.DH; ci.normal_refine = p.NormalRefine; ci.grid_type = p.INPORT_GRID_TYPE;
+      ok = ok && ci.xline;
+      if (ok) hipLaunchKernelGGL(coord_kernel, dim3(blocks_for((long)(nx - 6) * (ny - 6) * (nz - 6))), dim3(256), 0, 0, ci, dC,
+                                 dC + nxyz, dC + 2 * nxyz);
+    }
+    if (ok) {
+      const long lines[3] = {(long)ny * nz, (long)nx * nz, (long)nx * ny};
+      for (int ax = 0; ax < 3; ax++)
+        hipLaunchKernelGGL(coord_ghost_kernel, dim3(blocks_for(lines[ax]), 3), dim3(256), 0, 0, dC, nxyz, nx, ny, nz, nyz, ax);
+    }
+    if (!ok || hipGetLastError() != hipSuccess) { err = "coordinates on the device failed"; return 6; }
   }
+  const double* X = dC;
+  const double* Y = dC + nxyz;
+  const double* Z = dC + 2 * nxyz;
 
   // ---- grid-step range for the CFL check (cal_range_steph, coord.cu:330-391)
-  double hmin = 1e38;
-#pragma omp parallel for reduction(min : hmin)
-  for (int i = 3; i < nx - 3; i++)
-    for (int j = 3; j < ny - 3; j++)
-      for (int k = 3; k < nz - 3; k++) {
-        long q = P(i, j, k);
-        double x0[3] = {X[q], Y[q], Z[q]};
-        for (int ii = -1; ii <= 1; ii += 2)
-          for (int jj = -1; jj <= 1; jj += 2)
-            for (int kk = -1; kk <= 1; kk += 2) {
-              long q1 = P(i + ii, j, k), q2 = P(i, j + jj, k), q3 = P(i, j, k + kk);
-              double x1[3] = {X[q1], Y[q1], Z[q1]}, x2[3] = {X[q2], Y[q2], Z[q2]}, x3[3] = {X[q3], Y[q3], Z[q3]};
-              hmin = std::min(hmin, dist_point_plane(x0, x1, x2, x3));
-            }
-      }
-  c.hmin = hmin;
+  {
+    unsigned long long* dh = (unsigned long long*)dput(nullptr, sizeof(unsigned long long));
+    const unsigned long long init = ~0ull;
+    unsigned long long bits = 0;
+    bool ok = dh && hipMemcpy(dh, &init, sizeof init, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) hipLaunchKernelGGL(hmin_kernel, dim3(blocks_for((long)(nx - 6) * (ny - 6) * (nz - 6))), dim3(256), 0, 0, X, Y, Z, nx, ny,
+                               nz, nyz, dh);
+    ok = ok && hipGetLastError() == hipSuccess && hipMemcpy(&bits, dh, sizeof bits, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) { err = "grid-step range on the device failed"; return 6; }
+    c.hmin = from_order_key(bits);
+  }
+  const double hmin = c.hmin;
+
+  // coordinates of the output files: the fault plane and the three wave
+  // slices (io_fault.cu:37-67, io_wave_*.cu)
+  {
+    const int kz = std::min(std::max(p.EXPORT_WAVE_SLICE_Z, 0), nk - 1);
+    const int jy = std::min(std::max(p.EXPORT_WAVE_SLICE_Y, 0), nj - 1);
+    const int ix = std::min(std::max(p.EXPORT_WAVE_SLICE_X, 0), ni - 1);
+    auto P3 = [&](int i, int j, int k) { return (long)j + (long)k * ny + (long)i * nyz; };
+    // (base, stride a, stride b, na, nb) of the fault plane (nk, nj), xy (ni, nj), xz (ni, nk), yz (nk, nj)
+    const long base[4] = {P3(d.i0, 3, 3), P3(3, 3, kz + 3), P3(3, jy + 3, 3), P3(ix + 3, 3, 3)};
+    const long sa[4] = {(long)ny, nyz, nyz, (long)ny}, sb[4] = {1, 1, (long)ny, 1};
+    const int na[4] = {nk, ni, ni, nk}, nb[4] = {nj, nj, nk, nj};
+    for (int s = 0; s < 4; s++) {
+      const long n = (long)na[s] * nb[s];
+      float* dst = (float*)dput(nullptr, 3 * n * sizeof(float));
+      c.out_coord[s].resize(3 * n);
+      bool ok = dst != nullptr;
+      if (ok) hipLaunchKernelGGL(gather_kernel, dim3(blocks_for(n)), dim3(256), 0, 0, X, nxyz, base[s], sa[s], sb[s], na[s], nb[s], dst);
+      ok = ok && hipGetLastError() == hipSuccess &&
+           hipMemcpy(c.out_coord[s].data(), dst, 3 * n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess;
+      if (!ok) { err = "coordinate slices failed"; return 6; }
+    }
+  }
 
   // ---- metric + media (metric.cu:15-171, extend_Symm_array metric.cu:173-246,
   // media.cu:63-204): on the GPU, FP64 from the FP64 coordinates, rounded
-  // once into the device layout (FP32 or BF16); the host keeps no metric
-  // volume.  The host evaluates the same point function on the planes its
-  // own coefficients read: i0-3..i0+3 around the fault and the row k = nz-4
-  // under the free surface.
+  // once into the device layout (FP32 or BF16).  The fault coefficients and
+  // the free-surface maps below evaluate the same point function on the
+  // device for the planes they read.
   const long pyz = d.pyz;
-  MetricIn mh{};
-  mh.X = X; mh.Y = Y; mh.Z = Z;
-  mh.nyz = nyz; mh.ny = ny; mh.nz = nz; mh.nx = nx; mh.nj = nj; mh.nk = nk; mh.NX = p.NX;
-  mh.rDH = 1.0 / p.DH;
-  mh.lam1 = p.bi_rho1 * (p.bi_vp1 * p.bi_vp1 - 2 * p.bi_vs1 * p.bi_vs1); mh.miu1 = p.bi_rho1 * p.bi_vs1 * p.bi_vs1;
-  mh.lam2 = p.bi_rho2 * (p.bi_vp2 * p.bi_vp2 - 2 * p.bi_vs2 * p.bi_vs2); mh.miu2 = p.bi_rho2 * p.bi_vs2 * p.bi_vs2;
-  mh.rho1 = p.bi_rho1; mh.rho2 = p.bi_rho2;
-  mh.nl = (int)c.m1d_depth.size();
-  mh.d1 = c.m1d_depth.data(); mh.r1 = c.m1d_rho.data(); mh.p1 = c.m1d_vp.data(); mh.s1 = c.m1d_vs.data();
-  if (!c.m3d_vp.empty()) { mh.vp3 = c.m3d_vp.data(); mh.vs3 = c.m3d_vs.data(); mh.rho3 = c.m3d_rho.data(); }
+  MetricIn md{};
+  md.X = X; md.Y = Y; md.Z = Z;
+  md.nyz = nyz; md.ny = ny; md.nz = nz; md.nx = nx; md.nj = nj; md.nk = nk; md.NX = p.NX;
+  md.rDH = 1.0 / p.DH;
+  md.lam1 = p.bi_rho1 * (p.bi_vp1 * p.bi_vp1 - 2 * p.bi_vs1 * p.bi_vs1); md.miu1 = p.bi_rho1 * p.bi_vs1 * p.bi_vs1;
+  md.lam2 = p.bi_rho2 * (p.bi_vp2 * p.bi_vp2 - 2 * p.bi_vs2 * p.bi_vs2); md.miu2 = p.bi_rho2 * p.bi_vs2 * p.bi_vs2;
+  md.rho1 = p.bi_rho1; md.rho2 = p.bi_rho2;
+  md.nl = (int)c.m1d_depth.size();
   {
-    // device copies of the inputs, freed when the metric is written
-    std::vector<void*> tmp;
-    auto dput = [&](const void* src, size_t bytes) -> void* {
-      void* q = nullptr;
-      if (bytes == 0) return nullptr;
-      if (hipMalloc(&q, bytes) != hipSuccess) return nullptr;
-      tmp.push_back(q);
-      if (hipMemcpy(q, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-      return q;
-    };
-    MetricIn md = mh;
-    double* dC = (double*)dput(c.hostC.data(), c.hostC.size() * sizeof(double));
-    md.X = dC; md.Y = dC ? dC + nxyz : nullptr; md.Z = dC ? dC + 2 * nxyz : nullptr;
     const size_t n1 = c.m1d_depth.size() * sizeof(float);
     md.d1 = (const float*)dput(c.m1d_depth.data(), n1); md.r1 = (const float*)dput(c.m1d_rho.data(), n1);
     md.p1 = (const float*)dput(c.m1d_vp.data(), n1); md.s1 = (const float*)dput(c.m1d_vs.data(), n1);
-    if (mh.vp3) {
+    if (!c.m3d_vp.empty()) {
       const size_t n3 = c.m3d_vp.size() * sizeof(float);
       md.vp3 = (const float*)dput(c.m3d_vp.data(), n3); md.vs3 = (const float*)dput(c.m3d_vs.data(), n3);
       md.rho3 = (const float*)dput(c.m3d_rho.data(), n3);
     }
-    unsigned long long* dvp = nullptr;
-    void* vq = nullptr;
-    if (hipMalloc(&vq, sizeof(unsigned long long)) == hipSuccess) { tmp.push_back(vq); dvp = (unsigned long long*)vq; }
-    bool ok = dC && dvp && (mh.nl == 0 || (md.d1 && md.r1 && md.p1 && md.s1)) && (!mh.vp3 || (md.vp3 && md.vs3 && md.rho3));
+    unsigned long long* dvp = (unsigned long long*)dput(nullptr, sizeof(unsigned long long));
+    bool ok = dvp && (md.nl == 0 || (md.d1 && md.r1 && md.p1 && md.s1)) && (c.m3d_vp.empty() || (md.vp3 && md.vs3 && md.rho3));
     const long nmet = (long)nx * (p.MetricBF16 == 2 ? 16 : 13) * pyz;
     void* met = nullptr;
     if (ok) met = p.MetricBF16 ? (void*)dmalloc<uint16_t>(c, nmet) : (void*)dmalloc<float>(c, nmet);
@@ -401,36 +719,30 @@ int host_setup(Ctx& c, std::string& err) {
       // device stores them, rounded to FP32 (the kernels add it in FP32)
       auto axis = [](int n) {
         std::vector<int> v;
-        for (int a = 0; a < n; a += 4) v.push_back(mirror_idx(a, n));
+        for (int q = 0; q < n; q += 4) v.push_back(mirror_idx(q, n));
         v.push_back(mirror_idx(n - 1, n));
         return v;
       };
       const std::vector<int> ai = axis(nx), aj = axis(ny), ak = axis(nz);
-      double lo[10], hi[10];
-      for (int q = 0; q < 10; q++) { lo[q] = 1e300; hi[q] = -1e300; }
-#pragma omp parallel
-      {
-        double l[10], h[10], v[13];
-        for (int q = 0; q < 10; q++) { l[q] = 1e300; h[q] = -1e300; }
-#pragma omp for nowait
-        for (long t = 0; t < (long)ai.size() * (long)ak.size(); t++) {
-          const int i = ai[t / (long)ak.size()], k = ak[t % (long)ak.size()];
-          for (int j : aj) {
-            metric_point(mh, i, j, k, v);
-            for (int q = 0; q < 10; q++) { l[q] = std::min(l[q], v[q]); h[q] = std::max(h[q], v[q]); }
-          }
-        }
-#pragma omp critical
-        for (int q = 0; q < 10; q++) { lo[q] = std::min(lo[q], l[q]); hi[q] = std::max(hi[q], h[q]); }
-      }
-      for (int q = 0; q < 10; q++) {
-        c.met_ref[q] = (float)(0.5 * (lo[q] + hi[q]));
+      std::vector<unsigned long long> keys(20);
+      for (int q = 0; q < 10; q++) { keys[2 * q] = ~0ull; keys[2 * q + 1] = 0ull; }
+      const int* dai = (const int*)dput(ai.data(), ai.size() * sizeof(int));
+      const int* daj = (const int*)dput(aj.data(), aj.size() * sizeof(int));
+      const int* dak = (const int*)dput(ak.data(), ak.size() * sizeof(int));
+      unsigned long long* dk = (unsigned long long*)dput(keys.data(), keys.size() * sizeof(unsigned long long));
+      ok = dai && daj && dak && dk;
+      const long ns = (long)ai.size() * aj.size() * ak.size();
+      if (ok) hipLaunchKernelGGL(metric_range_kernel, dim3(blocks_for(ns)), dim3(256), 0, 0, md, dai, daj, dak, (int)ai.size(),
+                                 (int)aj.size(), (int)ak.size(), dk);
+      ok = ok && hipGetLastError() == hipSuccess &&
+           hipMemcpy(keys.data(), dk, keys.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost) == hipSuccess;
+      for (int q = 0; ok && q < 10; q++) {
+        c.met_ref[q] = (float)(0.5 * (from_order_key(keys[2 * q]) + from_order_key(keys[2 * q + 1])));
         ref.r[q] = c.met_ref[q];
       }
     }
     if (ok) {
-      const long total = (long)nx * nz * ny;
-      const unsigned blocks = (unsigned)((total + 255) / 256);
+      const unsigned blocks = blocks_for((long)nx * nz * ny);
       if (p.MetricBF16 == 2)
         hipLaunchKernelGGL((metric_media_kernel<uint16_t, 16>), dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
       else if (p.MetricBF16)
@@ -444,9 +756,8 @@ int host_setup(Ctx& c, std::string& err) {
         c.met_fault = dmalloc<float>(c, (size_t)fnp * 13 * pyz);
         if (c.met_fault) {
           c.met_fault_base = c.met_fault - (long)fb * 13 * pyz;
-          const unsigned fblocks = (unsigned)(((long)fnp * nz * ny + 255) / 256);
-          hipLaunchKernelGGL(metric_media_kernel<float>, dim3(fblocks), dim3(256), 0, 0, md, c.met_fault, pyz, d.py, d.jpad,
-                             dvp, ref, fb, fnp);
+          hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks_for((long)fnp * nz * ny)), dim3(256), 0, 0, md, c.met_fault,
+                             pyz, d.py, d.jpad, dvp, ref, fb, fnp);
         }
         ok = c.met_fault != nullptr;
       }
@@ -456,34 +767,9 @@ int host_setup(Ctx& c, std::string& err) {
       std::memcpy(&vpm, &bits, sizeof vpm);
       c.vpmax = vpm;
     }
-    for (void* q : tmp) (void)hipFree(q);
     if (!ok) { err = "metric computation on the device failed"; return 6; }
     c.met = met;
   }
-  const int ip0 = std::max(d.i0 - 3, 0), ip1 = std::min(d.i0 + 3, nx - 1);
-  std::vector<double> mdx((long)(ip1 - ip0 + 1) * nyz * 13), mdz((long)nx * ny * 13);
-  if (p.Rupture) {
-#pragma omp parallel for
-    for (int i = ip0; i <= ip1; i++)
-      for (int k = 0; k < nz; k++)
-        for (int j = 0; j < ny; j++)
-          metric_point(mh, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(k, nz), &mdx[(((long)(i - ip0) * nz + k) * ny + j) * 13]);
-  }
-  if (freenode) {
-#pragma omp parallel for
-    for (int i = 0; i < nx; i++)
-      for (int j = 0; j < ny; j++)
-        metric_point(mh, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(nz - 4, nz), &mdz[((long)i * ny + j) * 13]);
-  }
-  // metric word m of point q = P(i, j, k), from the plane caches
-  auto MD = [&](int m, long q) -> double {
-    const int i = (int)(q / nyz), k = (int)((q % nyz) / ny), j = (int)(q % ny);
-    if (p.Rupture && i >= ip0 && i <= ip1) return mdx[(((long)(i - ip0) * nz + k) * ny + j) * 13 + m];
-    if (freenode && k == nz - 4) return mdz[((long)i * ny + j) * 13 + m];
-    double v[13];
-    metric_point(mh, mirror_idx(i, nx), mirror_idx(j, ny), mirror_idx(k, nz), v);
-    return v[m];
-  };
   // CFL (main.cu:302-345)
   const double vpmax = c.vpmax;
   float dtmax = (float)(1.3 * hmin / vpmax);
@@ -520,38 +806,21 @@ int host_setup(Ctx& c, std::string& err) {
     }
   }
 
-  // ---- free-surface velocity maps (init_wave_free_cu, fault_coef.cu:1320-1486)
-  std::vector<float> mVx(nx * (long)ny * 9, 0.f), mVy(nx * (long)ny * 9, 0.f);
+  // ---- free-surface velocity maps (init_wave_free_cu, fault_coef.cu:1320-1486), on the device
+  c.matVx2Vz = dmalloc<float>(c, nx * (long)ny * 9);
+  c.matVy2Vz = dmalloc<float>(c, nx * (long)ny * 9);
+  if (!c.matVx2Vz || !c.matVy2Vz || hipMemset(c.matVx2Vz, 0, nx * (long)ny * 9 * sizeof(float)) != hipSuccess ||
+      hipMemset(c.matVy2Vz, 0, nx * (long)ny * 9 * sizeof(float)) != hipSuccess) {
+    err = "hipMalloc failed for the free-surface maps";
+    return 6;
+  }
   if (freenode) {
-    int k = nz - 4;
-#pragma omp parallel for
-    for (int i = 3; i < nx - 3; i++)
-      for (int j = 3; j < ny - 3; j++) {
-        double rb[3] = {1.0 / Bp[0][i - 3], 1.0 / Bp[1][j - 3], 1.0 / Bp[2][k - 3]};
-        long q = P(i, j, k);
-        double e[3][3];
-        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = MD(a, q);
-        double lam = MD(10, q), mu = MD(11, q);
-        // traction on the zeta surface produced by derivatives along axis b:
-        // K_b[r][c] = lam e3[r] eb[c] + mu e3[c] eb[r] + mu (e3.eb) delta_rc
-        double K[3][3][3];
-        for (int b = 0; b < 3; b++) {
-          double eb3 = dot(e[2], e[b]);
-          for (int r = 0; r < 3; r++)
-            for (int cc = 0; cc < 3; cc++)
-              K[b][r][cc] = rb[b] * (lam * e[2][r] * e[b][cc] + mu * e[2][cc] * e[b][r] + (r == cc ? mu * eb3 : 0.0));
-        }
-        double Ainv[3][3], AB[3][3], AC[3][3], Bm[3][3], Cm[3][3];
-        for (int r = 0; r < 3; r++)
-          for (int cc = 0; cc < 3; cc++) { Ainv[r][cc] = K[2][r][cc]; Bm[r][cc] = -K[0][r][cc]; Cm[r][cc] = -K[1][r][cc]; }
-        inv3(Ainv);
-        mul3(Ainv, Bm, AB);
-        mul3(Ainv, Cm, AC);
-        for (int a = 0; a < 9; a++) {
-          mVx[(j + (long)i * ny) * 9 + a] = (float)AB[a / 3][a % 3];
-          mVy[(j + (long)i * ny) * 9 + a] = (float)AC[a / 3][a % 3];
-        }
-      }
+    const double* dB[3];
+    for (int a = 0; a < 3; a++) dB[a] = (const double*)dput(Bp[a].data(), Bp[a].size() * sizeof(double));
+    bool ok = dB[0] && dB[1] && dB[2];
+    if (ok) hipLaunchKernelGGL(wave_free_kernel, dim3(blocks_for((long)(nx - 6) * (ny - 6))), dim3(256), 0, 0, md, dB[0], dB[1], dB[2],
+                               c.matVx2Vz, c.matVy2Vz);
+    if (!ok || hipGetLastError() != hipSuccess) { err = "free-surface maps on the device failed"; return 6; }
   }
 
   // ---- device allocations: 4 wavefield buffers (step start S, stage A/B, accumulator T)
@@ -562,10 +831,6 @@ int host_setup(Ctx& c, std::string& err) {
   c.Tacc = dmalloc<float>(c, nvol);
   if (!c.S || !c.A || !c.B || !c.Tacc) { err = "hipMalloc failed for the wavefield"; return 6; }
   c.cur = c.S;
-  c.matVx2Vz = dmalloc<float>(c, mVx.size());
-  c.matVy2Vz = dmalloc<float>(c, mVy.size());
-  upload(c.matVx2Vz, mVx);
-  upload(c.matVy2Vz, mVy);
 
   // PML
   c.pml.N = p.PML_N;
@@ -609,99 +874,40 @@ int host_setup(Ctx& c, std::string& err) {
     FaultDev& F = c.f;
     const int i0 = d.i0;
     long n2 = (long)nj * nk;
-    std::vector<float> rho_f(2 * nyz), Dm(2L * 9 * nyz * 9), T1x(2 * nyz * 9), Vyx(2 * nyz * 9), Vzx(2 * nyz * 9);
-    std::vector<float> Vx2Vzf(2L * ny * 9, 0.f), Vy2Vzf(2L * ny * 9, 0.f), P2M(3L * ny * 9, 0.f), M2P(3L * ny * 9, 0.f);
-    std::vector<float> vn(nyz * 3), vs1(nyz * 3), vs2(nyz * 3), jacvec(nyz), jac0(nyz);
+    // split-node coefficients (fault_coef.cu:26-1302) on the device, straight
+    // into the fault's device arrays; the FP64 unit vectors come back for the
+    // initial stress projection below
+    FaultCoefOut fo{};
+    fo.rho_f = F.rho_f = dmalloc<float>(c, 2 * nyz);
+    fo.Dm = F.Dmat = dmalloc<float>(c, 2L * 9 * nyz * 9);
+    fo.T1x = F.matT1toVx = dmalloc<float>(c, 2 * nyz * 9);
+    fo.Vyx = F.matVytoVx = dmalloc<float>(c, 2 * nyz * 9);
+    fo.Vzx = F.matVztoVx = dmalloc<float>(c, 2 * nyz * 9);
+    fo.Vx2Vzf = F.matVx2Vz_f = dmalloc<float>(c, 2L * ny * 9);
+    fo.Vy2Vzf = F.matVy2Vz_f = dmalloc<float>(c, 2L * ny * 9);
+    fo.P2M = F.matP2Mf = dmalloc<float>(c, 3L * ny * 9);
+    fo.M2P = F.matM2Pf = dmalloc<float>(c, 3L * ny * 9);
+    fo.vn = F.vec_n = dmalloc<float>(c, nyz * 3);
+    fo.vs1 = F.vec_s1 = dmalloc<float>(c, nyz * 3);
+    fo.vs2 = F.vec_s2 = dmalloc<float>(c, nyz * 3);
+    fo.jacvec = F.jacvec = dmalloc<float>(c, nyz);
+    fo.jac0 = F.jac_i0 = dmalloc<float>(c, nyz);
+    fo.nd = (double*)dput(nullptr, nyz * 9 * sizeof(double));
+    std::vector<double> nd(nyz * 9);
+    {
+      bool ok = fo.rho_f && fo.Dm && fo.T1x && fo.Vyx && fo.Vzx && fo.Vx2Vzf && fo.Vy2Vzf && fo.P2M && fo.M2P && fo.vn && fo.vs1 &&
+                fo.vs2 && fo.jacvec && fo.jac0 && fo.nd;
+      // the surface maps exist for the row under the free surface only
+      for (float* z : {fo.Vx2Vzf, fo.Vy2Vzf}) ok = ok && hipMemset(z, 0, 2L * ny * 9 * sizeof(float)) == hipSuccess;
+      for (float* z : {fo.P2M, fo.M2P}) ok = ok && hipMemset(z, 0, 3L * ny * 9 * sizeof(float)) == hipSuccess;
+      if (ok) hipLaunchKernelGGL(fault_coef_kernel, dim3(blocks_for(nyz)), dim3(256), 0, 0, md, i0, fo);
+      ok = ok && hipGetLastError() == hipSuccess &&
+           hipMemcpy(nd.data(), fo.nd, nd.size() * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess;
+      if (!ok) { err = "fault coefficients on the device failed"; return 6; }
+    }
     std::vector<double> vnd(nyz * 3), vs1d(nyz * 3), vs2d(nyz * 3);
-#pragma omp parallel for
-    for (int k = 0; k < nz; k++)
-      for (int j = 0; j < ny; j++) {
-        long q = P(i0, j, k), pf = j + (long)k * ny;
-        double e[3][3];
-        for (int a = 0; a < 9; a++) e[a / 3][a % 3] = MD(a, q);
-        double jac = MD(9, q);
-        double mt[3][3];
-        std::memcpy(mt, e, sizeof mt);
-        inv3(mt);
-        double x_et[3] = {mt[0][1], mt[1][1], mt[2][1]};
-        double s1[3] = {x_et[0], x_et[1], x_et[2]}, n[3] = {e[0][0], e[0][1], e[0][2]}, s2[3];
-        double ns1 = nrm(s1), nn = nrm(n);
-        for (int a = 0; a < 3; a++) { s1[a] /= ns1; n[a] /= nn; }
-        cross(n, s1, s2);
-        for (int a = 0; a < 3; a++) {
-          vn[pf * 3 + a] = (float)n[a]; vs1[pf * 3 + a] = (float)s1[a]; vs2[pf * 3 + a] = (float)s2[a];
-          vnd[pf * 3 + a] = n[a]; vs1d[pf * 3 + a] = s1[a]; vs2d[pf * 3 + a] = s2[a];
-        }
-        jacvec[pf] = (float)(jac * nn);
-        jac0[pf] = (float)jac;
-        double lamf[2] = {MD(10, q - nyz), MD(10, q + nyz)};
-        double muf[2] = {MD(11, q - nyz), MD(11, q + nyz)};
-        rho_f[pf] = (float)MD(12, q - nyz);
-        rho_f[nyz + pf] = (float)MD(12, q + nyz);
-        double Dl[2][3][3][3][3];
-        for (int m = 0; m < 2; m++) {
-          for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) {
-              double eab = dot(e[a], e[b]);
-              for (int r = 0; r < 3; r++)
-                for (int cc = 0; cc < 3; cc++) {
-                  double v = jac * (lamf[m] * e[a][r] * e[b][cc] + muf[m] * e[a][cc] * e[b][r] + (r == cc ? muf[m] * eab : 0.0));
-                  Dl[m][a][b][r][cc] = v;
-                  Dm[(((long)(m * 9 + a * 3 + b)) * nyz + pf) * 9 + r * 3 + cc] = (float)v;
-                }
-            }
-          double D11[3][3], D12[3][3], D13[3][3], o[3][3];
-          std::memcpy(D11, Dl[m][0][0], sizeof D11);
-          std::memcpy(D12, Dl[m][0][1], sizeof D12);
-          std::memcpy(D13, Dl[m][0][2], sizeof D13);
-          inv3(D11);
-          for (int a = 0; a < 9; a++) T1x[(m * nyz + pf) * 9 + a] = (float)D11[a / 3][a % 3];
-          mul3(D11, D12, o);
-          for (int a = 0; a < 9; a++) Vyx[(m * nyz + pf) * 9 + a] = (float)o[a / 3][a % 3];
-          mul3(D11, D13, o);
-          for (int a = 0; a < 9; a++) Vzx[(m * nyz + pf) * 9 + a] = (float)o[a / 3][a % 3];
-        }
-        if (k == nz - 4) {
-          double V2Z[2][3][3], VY2Z[2][3][3];
-          for (int m = 0; m < 2; m++) {
-            double A[3][3], B[3][3], Cc[3][3];
-            for (int r = 0; r < 3; r++)
-              for (int cc = 0; cc < 3; cc++) { A[r][cc] = Dl[m][2][2][r][cc]; B[r][cc] = -Dl[m][2][0][r][cc]; Cc[r][cc] = -Dl[m][2][1][r][cc]; }
-            inv3(A);
-            mul3(A, B, V2Z[m]);
-            mul3(A, Cc, VY2Z[m]);
-            for (int a = 0; a < 9; a++) { Vx2Vzf[(m * ny + j) * 9 + a] = (float)V2Z[m][a / 3][a % 3]; Vy2Vzf[(m * ny + j) * 9 + a] = (float)VY2Z[m][a / 3][a % 3]; }
-          }
-          double v1[3] = {e[0][0], e[0][1], e[0][2]}, v3[3];
-          double v1n = nrm(v1);
-          cross(v1, x_et, v3);
-          double h[3] = {dot(v3, e[0]) / v1n, dot(v3, e[1]) / v1n, dot(v3, e[2]) / v1n};
-          double miu = MD(11, q);
-          double Vxf[2][3][3], Vyf[2][3][3];
-          for (int m = 0; m < 2; m++) {
-            double Df[3][3][3];
-            for (int b = 0; b < 3; b++)
-              for (int r = 0; r < 3; r++)
-                for (int cc = 0; cc < 3; cc++) Df[b][r][cc] = Dl[m][0][b][r][cc] + miu * h[b] * v3[r] * v1[cc];
-            double m1[3][3], m2[3][3];
-            mul3(Df[2], V2Z[m], m1);
-            mul3(Df[2], VY2Z[m], m2);
-            for (int r = 0; r < 3; r++)
-              for (int cc = 0; cc < 3; cc++) { Vxf[m][r][cc] = Df[0][r][cc] + m1[r][cc]; Vyf[m][r][cc] = Df[1][r][cc] + m2[r][cc]; }
-          }
-          double Ti[3][3], o[3][3];
-          std::memcpy(Ti, Vxf[0], sizeof Ti);
-          inv3(Ti);
-          mul3(Ti, Vxf[1], o); for (int a = 0; a < 9; a++) P2M[(0 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
-          mul3(Ti, Vyf[1], o); for (int a = 0; a < 9; a++) P2M[(1 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
-          mul3(Ti, Vyf[0], o); for (int a = 0; a < 9; a++) P2M[(2 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
-          std::memcpy(Ti, Vxf[1], sizeof Ti);
-          inv3(Ti);
-          mul3(Ti, Vxf[0], o); for (int a = 0; a < 9; a++) M2P[(0 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
-          mul3(Ti, Vyf[0], o); for (int a = 0; a < 9; a++) M2P[(1 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
-          mul3(Ti, Vyf[1], o); for (int a = 0; a < 9; a++) M2P[(2 * ny + j) * 9 + a] = (float)o[a / 3][a % 3];
-        }
-      }
+    for (long q = 0; q < nyz; q++)
+      for (int a = 0; a < 3; a++) { vnd[q * 3 + a] = nd[q * 9 + a]; vs1d[q * 3 + a] = nd[q * 9 + 3 + a]; vs2d[q * 3 + a] = nd[q * 9 + 6 + a]; }
     // initial fault state (init_fault_cu, init.cu:160-849)
     std::vector<float> Tn(n2), Ts1(n2), Ts2(n2), six(n2), siy(n2), siz(n2), peak(n2), initt0(n2, -9999.9f);
     std::vector<int> riy(n2), riz(n2), uni(n2), fg(n2);
@@ -754,9 +960,6 @@ int host_setup(Ctx& c, std::string& err) {
     F.T11 = dmalloc<float>(c, 7 * nyz); F.T12 = dmalloc<float>(c, 7 * nyz); F.T13 = dmalloc<float>(c, 7 * nyz);
     F.mT11 = dmalloc<float>(c, nyz); F.mT12 = dmalloc<float>(c, nyz); F.mT13 = dmalloc<float>(c, nyz);
     F.hT11 = dmalloc<float>(c, nyz); F.hT12 = dmalloc<float>(c, nyz); F.hT13 = dmalloc<float>(c, nyz);
-    UPF(rho_f, rho_f); UPF(Dmat, Dm); UPF(matT1toVx, T1x); UPF(matVytoVx, Vyx); UPF(matVztoVx, Vzx);
-    UPF(matVx2Vz_f, Vx2Vzf); UPF(matVy2Vz_f, Vy2Vzf); UPF(matP2Mf, P2M); UPF(matM2Pf, M2P);
-    UPF(vec_n, vn); UPF(vec_s1, vs1); UPF(vec_s2, vs2); UPF(jacvec, jacvec); UPF(jac_i0, jac0);
     UPF(Tn, Tn); UPF(Ts1, Ts1); UPF(Ts2, Ts2);
     F.tTn = dmalloc<float>(c, n2); F.tTs1 = dmalloc<float>(c, n2); F.tTs2 = dmalloc<float>(c, n2);
     UPF(str_init_x, six); UPF(str_init_y, siy); UPF(str_init_z, siz);
