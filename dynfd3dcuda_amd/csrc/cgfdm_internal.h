@@ -50,7 +50,7 @@ struct Params {
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
   int WaveInteriorKernel = -1;  // interior tiles: -1 automatic; 0 general kernel; pipelined kernel: 1 64 x 8 with
                                 // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
-  int WaveBandKernel = 1;  // absorbing-band tiles: 0 general kernel; 1 pipelined kernel
+  int WaveBandKernel = 2;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
@@ -195,6 +195,8 @@ struct Ctx {
   bool wave_pipe = false;           // pipelined interior kernel (WaveInteriorKernel)
   bool wave_ring = false;           // its LDS plane ring (WaveInteriorKernel 1)
   bool wave_band_pipe = false;      // pipelined kernel on the absorbing-band tiles (WaveBandKernel)
+  int wave_tjn = 32;                // columns of the narrow band tiles (20 or 32)
+  int wave_band_tk = 8;             // rows of the 64-wide band tiles (WaveBandKernel 2: 4)
   int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the

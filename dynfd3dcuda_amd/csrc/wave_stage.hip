@@ -40,6 +40,8 @@ constexpr int TK = NTHR / TJ;
 constexpr int TJN = 32;    // narrow tile for the absorbing bands along j (32 j x 8 k)
 constexpr int NTHRN = 256;  // threads per block of the narrow tiles
 constexpr int TKN = NTHRN / TJN;
+constexpr int TJB = 20;    // exact-width tiles of the absorbing bands along j when they are at most 20
+constexpr int TKB = 12;    // columns wide (20 x 12, 240 threads; the 20-layer PML and sponge)
 constexpr int TK6 = 6;     // rows of the 64 x 6 interior tiles (WaveInteriorKernel 2)
 constexpr int TK4 = 4;     // rows of the 64 x 4 interior tiles (WaveInteriorKernel 3)
 
@@ -1030,14 +1032,20 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     pr.begin(s, PK_STAGE_BAND);
     if (n[1] > 0) {
       a.work = w + 2 * o1; a.nwork = n[1];
-      if (c.wave_band_pipe)
+      if (c.wave_band_pipe && c.wave_band_tk == TK4)
+        hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK4, 2, true, false>), dim3(a.nwork), dim3(TJ, TK4, 1),
+                           0, s, a);
+      else if (c.wave_band_pipe)
         hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2, true>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
       else
         hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 1>), dim3(a.nwork), dim3(TJ, TK, 1), 0, s, a);
     }
     if (n[2] > 0) {
       a.work = w + 2 * o2; a.nwork = n[2];
-      if (c.wave_band_pipe)
+      if (c.wave_band_pipe && c.wave_tjn == TJB)
+        hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJB, TJB * TKB, 2, true>), dim3(a.nwork), dim3(TJB, TKB, 1), 0,
+                           s, a);
+      else if (c.wave_band_pipe)
         hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJN, NTHRN, 2, true>), dim3(a.nwork), dim3(TJN, TKN, 1), 0,
                            s, a);
       else
@@ -1237,6 +1245,7 @@ int upload_wave_cold(Ctx& c) {
     if (c.pml.on[2 * ax] && c.pml.on[2 * ax + 1] && 2 * c.pml.N > (ax == 0 ? c.d.ni : ax == 1 ? c.d.nj : c.d.nk))
       disjoint = false;
   c.wave_band_pipe = p.WaveBandKernel >= 1 && disjoint;
+  c.wave_band_tk = c.wave_band_pipe && p.WaveBandKernel == 2 ? TK4 : TK;  // 2: 64 x 4 band tiles, two blocks per CU
   const int tki = tall ? 2 * TK : (c.wave_pipe && wik == 2) ? TK6 : (c.wave_pipe && wik == 3) ? TK4 : TK;  // interior tile rows
   c.wave_tki = tki;
   for (int ax = 1; ax < 3; ax++) {
@@ -1299,31 +1308,45 @@ int upload_wave_cold(Ctx& c) {
     c.wave_points[list == 0 ? 0 : list == 3 ? 2 : 1] += (long)(a.n - a.o) * (b.n - b.o) * (i1 - i0);
   };
   const auto narrow = [&](const Seg& a) { return !a.in && a.n - a.o <= TJN; };
-  // the same active columns in a TJN-wide tile: anchored at the interior
+  // bands of at most TJB columns get tiles exactly TJB wide (with the
+  // pipelined band kernel), so that a 20-column band keeps every lane busy
+  // (a 32-wide tile idles 12 of its 32; 1600 x 1000 x 500: 2.8 ms per stage
+  // for the 32 M points of the two j bands against 2.4 ms for the 48 M
+  // points of the 64-wide band tiles); wider narrow bands get TJN x TKN tiles
+  int wmax = 0;
+  for (const Seg& a : sj)
+    if (narrow(a)) wmax = std::max(wmax, a.n - a.o);
+  const bool exact = c.wave_band_pipe && wmax > 0 && wmax <= TJB;
+  const int tjn = exact ? TJB : TJN, tkn = exact ? TKB : TKN;
+  c.wave_tjn = tjn;
+  // the same active columns in a tjn-wide tile: anchored at the interior
   // edge on the low side, at the band start on the high side
   const auto narrowed = [&](const Seg& a) {
     const int w = a.n - a.o;
-    return a.o > 0 ? Seg{a.s + a.n - TJN, TJN, false, TJN - w} : Seg{a.s, w, false, 0};
+    return a.o > 0 ? Seg{a.s + a.n - tjn, tjn, false, tjn - w} : Seg{a.s, w, false, 0};
   };
   int ktop = nk;  // first row of the free-surface tiles
   for (const Seg& b : sk)
     if (is_top(b)) ktop = std::min(ktop, b.s);
   for (int i0 = ia; i0 < ib; i0 += ich_in)
     for (const Seg& b : ski) for (const Seg& a : sji) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
+  // band tiles of TK rows, or of TK4 rows with WaveBandKernel 2 (the rows
+  // from ktop up are the free-surface tiles')
+  const std::vector<Seg> skb = c.wave_band_tk == TK4 ? axis_segments(n3[2], lo[2], hi[2], TK4) : sk;
   for (int i0 = 0; i0 < ni; i0 += ich_band)
-    for (const Seg& b : sk)
+    for (const Seg& b : skb)
       for (const Seg& a : sj) {
         const int i1 = std::min(ni, i0 + ich_band);
-        if (is_top(b) || narrow(a)) continue;
+        if (b.s >= ktop || narrow(a)) continue;
         if (!(a.in && b.in)) { push(1, a, b, i0, i1); continue; }
         // interior tile: only the parts of [i0, i1) outside [ia, ib)
         if (i0 < ia) push(1, a, b, i0, std::min(i1, ia));
         if (i1 > ib) push(1, a, b, std::max(i0, ib), i1);
       }
   for (int i0 = 0; i0 < ni; i0 += ich_band)
-    for (int k0 = 0; k0 < ktop; k0 += TKN)
+    for (int k0 = 0; k0 < ktop; k0 += tkn)
       for (const Seg& a : sj)
-        if (narrow(a)) push(2, narrowed(a), Seg{k0, std::min(TKN, ktop - k0), false, 0}, i0, std::min(ni, i0 + ich_band));
+        if (narrow(a)) push(2, narrowed(a), Seg{k0, std::min(tkn, ktop - k0), false, 0}, i0, std::min(ni, i0 + ich_band));
   for (int i0 = 0; i0 < ni; i0 += ich_top)
     for (const Seg& b : sk)
       if (is_top(b))

@@ -17,7 +17,8 @@ instantiations of the wave stage kernel and of the pipelined stage kernel,
 whose BAND instantiations are band tiles; stage_surface includes the two
 free-surface precompute passes), pml_aux, fault_pre, fault_rs, fault_deriv,
 fault_post, rup_sensor, thermpress, setup_metric (the setup's metric
-pass); everything else is listed under its own name.
+pass), setup_other (its other passes); everything else is listed under its
+own name.
 """
 import argparse
 import collections
@@ -56,6 +57,10 @@ def kernel_class(name):
         return "thermpress"
     if "metric_media_kernel" in name:
         return "setup_metric"
+    for k in ("coord_kernel", "coord_ghost_kernel", "hmin_kernel", "gather_kernel", "metric_range_kernel",
+              "fault_coef_kernel", "wave_free_kernel"):
+        if k in name:
+            return "setup_other"  # the other setup passes (coordinates, CFL range, coefficients)
     return name.split("(")[0].strip()
 
 

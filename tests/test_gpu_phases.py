@@ -154,10 +154,11 @@ def test_stage_streams_same_result(extra):
                                              ({"WaveInteriorKernel": 3}, True, 150),
                                              ({"WaveInteriorKernel": 4}, True, 150),
                                              ({"WaveBandKernel": 0}, True, 150),
-                                             ({"WaveBandKernel": 0, "useCerjan": 1}, True, 150)],
+                                             ({"WaveBandKernel": 0, "useCerjan": 1}, True, 150),
+                                             ({"WaveBandKernel": 1, "useCerjan": 1}, True, 150)],
                          ids=["band_round", "tall", "band_round_tall", "surface_tiles_off", "general_interior",
                               "interior_64x6", "interior_64x4", "interior_64x8_double_buffer", "general_band",
-                              "general_band_cerjan"])
+                              "general_band_cerjan", "band_64x8_cerjan"])
 def test_tilings_give_the_same_result(layout, exact, ny):
     """The tiling options only move points between the interior, band and
     free-surface launches, or between interior kernels of other tile shapes;
@@ -165,8 +166,8 @@ def test_tilings_give_the_same_result(layout, exact, ny):
     the absorbing layers).  10 steps of a slip-weakening rupture on a curved
     grid with PML and free surface:
     - moves between the interior and band kernels, the general, 64 x 6,
-      64 x 4 and double-buffered 64 x 8 interior kernels, and the general
-      kernel on the absorbing-band tiles
+      64 x 4 and double-buffered 64 x 8 interior kernels, the general
+      kernel and 64 x 8 tiles on the absorbing bands
       (with ADE-PML, and with PML and the Cerjan sponge together), give the
       default tiling's result bit for bit;
     - WaveSurfaceTiles 0 moves rows between the free-surface kernel and the
