@@ -51,6 +51,7 @@ struct Params {
   int WaveInteriorKernel = -1;  // interior tiles: -1 automatic; 0 general kernel; pipelined kernel: 1 64 x 8 with
                                 // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
   int WaveBandKernel = 2;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles
+  int PmlAuxFused = 1;     // 1: the pipelined band tiles advance their points' ADE-PML auxiliary fields
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
@@ -197,6 +198,8 @@ struct Ctx {
   bool wave_band_pipe = false;      // pipelined kernel on the absorbing-band tiles (WaveBandKernel)
   int wave_tjn = 32;                // columns of the narrow band tiles (20 or 32)
   int wave_band_tk = 8;             // rows of the 64-wide band tiles (WaveBandKernel 2: 4)
+  bool pml_fused = false;           // band tiles advance their ADE-PML auxiliary fields (PmlAuxFused)
+  int wave_ktop = 0;                // first row of the free-surface tiles (nk without them)
   int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the

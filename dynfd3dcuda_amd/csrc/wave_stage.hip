@@ -159,6 +159,7 @@ struct WaveArgs {
   float rDH, rka, rkb;
   float mref[10];   // metric reference values (bfloat16-deviation form)
   int irk, freenode, cerjan, fault, rhs_only, pml;
+  int aux_fused;  // BAND tiles advance their points' ADE-PML auxiliary fields themselves
 };
 
 // Free surface, first pass: the conservative traction vectors
@@ -587,7 +588,12 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   constexpr int NS = LEAD + 1;
   __shared__ float lds[NS][9][LK][LJ];
   // BAND: the absorbing layers' uniform descriptors, staged once per block
-  __shared__ const float* s_auxY[6];
+  __shared__ float* s_auxY[6];
+  __shared__ float* s_auxS[6];
+  __shared__ float* s_auxT[6];
+  __shared__ const float* s_pA[3];
+  __shared__ const float* s_pB[3];
+  __shared__ const float* s_pD[3];
   __shared__ const float* s_pBx;
   __shared__ const float* s_cjx;
   __shared__ int s_on[6];
@@ -662,7 +668,14 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   if constexpr (BAND) {
     if (threadIdx.y == 0 && threadIdx.x < 6) {
       s_auxY[threadIdx.x] = a.cold->auxY[threadIdx.x];
+      s_auxS[threadIdx.x] = a.cold->auxS[threadIdx.x];
+      s_auxT[threadIdx.x] = a.cold->auxT[threadIdx.x];
       s_on[threadIdx.x] = a.cold->on[threadIdx.x];
+      if (threadIdx.x < 3) {
+        s_pA[threadIdx.x] = a.cold->pA[threadIdx.x];
+        s_pB[threadIdx.x] = a.cold->pB[threadIdx.x];
+        s_pD[threadIdx.x] = a.cold->pD[threadIdx.x];
+      }
       if (threadIdx.x == 0) { s_N = a.cold->N; s_pBx = a.cold->pB[0]; s_cjx = a.cold->cj[0]; }
     }
     if (act && pml) { rb2 = 1.0f / a.cold->pB[1][j]; rb3 = 1.0f / a.cold->pB[2][k]; }
@@ -741,34 +754,55 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
         for (int c = 0; c < 9; c++) sv[c] = bld<kStream>(rs, col4, c * pb);
       }
     }
-    // BAND: the aux words of plane i of the face along each axis whose band
-    // holds the point (pmlx.cu:126-210 and the y/z variants)
-    float ya[3][9];
+    // BAND: the face along each axis whose band holds the point, and the aux
+    // words of plane i of the first such face (pmlx.cu:126-210 and the y/z
+    // variants): its value, and with the fused auxiliary update its
+    // step-start (S) and accumulator (T) words; those of a second or third
+    // face (at the edges of the absorbing box) are read where they are used
+    float ya[9], sa[9], ta[9];
     bool inx = false;
+    int fa = -1;                        // first face axis of the point
+    int fo[3] = {0, 0, 0};              // face of each axis
+    uint32_t pa4s[3] = {0, 0, 0};       // aux byte offset of the point in the face of each axis
+    auto face_ps = [&](int ax) { return (size_t)(ax == 0 ? N : ni) * (ax == 1 ? N : nj) * (ax == 2 ? N : nk); };
     if constexpr (BAND) {
 #pragma unroll
-      for (int ax = 0; ax < 3; ax++)
-#pragma unroll
-        for (int c = 0; c < 9; c++) ya[ax][c] = 0.0f;
+      for (int c = 0; c < 9; c++) { ya[c] = 0.0f; sa[c] = 0.0f; ta[c] = 0.0f; }
       if (pml) {
         uint32_t offi = 0;
         if (s_on[0] && i < N) { inx = true; offi = 0; }
         else if (s_on[1] && i >= ni - N) { inx = true; offi = ni - N; }
+        const bool fuse = a.aux_fused && !a.rhs_only;
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
           const bool in = ax == 0 ? inx : ax == 1 ? inj : ink;
           if (!act || !in) continue;
-          const int f = 2 * ax + (ax == 0 ? offi != 0 : ax == 1 ? offj != 0 : offk != 0);
+          const uint32_t off = ax == 0 ? offi : ax == 1 ? offj : offk;
+          fo[ax] = 2 * ax + (off != 0);
           const uint32_t pi = ax == 0 ? i - offi : i, pj = ax == 1 ? j - offj : j, pk = ax == 2 ? k - offk : k;
-          const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk, pni = ax == 0 ? N : ni;
-          const size_t ps = (size_t)pni * pnj * pnk;
-          const uint32_t pa4 = (pj + pk * pnj + pi * pnj * pnk) << 2;
-          const float* Y = s_auxY[f];
+          const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk;
+          pa4s[ax] = (pj + pk * pnj + pi * pnj * pnk) << 2;
+          if (fa < 0) fa = ax;
+        }
+        if (fa >= 0) {
+          const int f = fo[fa];
+          const size_t ps = face_ps(fa);
+          const uint32_t pa4 = pa4s[fa];
 #pragma unroll
-          for (int c = 0; c < 9; c++) ya[ax][c] = ldo(Y + c * ps, pa4);
+          for (int c = 0; c < 9; c++) ya[c] = ldo(s_auxY[f] + c * ps, pa4);
+          if (fuse && irk != 0) {
+#pragma unroll
+            for (int c = 0; c < 9; c++) ta[c] = ldo(s_auxT[f] + c * ps, pa4);
+          }
+          if (fuse && (irk == 1 || irk == 2)) {
+#pragma unroll
+            for (int c = 0; c < 9; c++) sa[c] = ldo(s_auxS[f] + c * ps, pa4);
+          }
         }
       }
     }
+    // aux word c of the face along axis ax (kept in registers for the first face)
+    auto aux_y = [&](int ax, int c) { return ax == fa ? ya[c] : ldo(s_auxY[fo[ax]] + c * face_ps(ax), pa4s[ax]); };
 
     if (act) {
       auto MET = [&](int c) { return met_raw_get<MT>(mw, c, a.mref); };
@@ -808,7 +842,50 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
             const float rbb = ax == 0 ? rb1 : ax == 1 ? rb2 : rb3;  // 1 / beta at the point
             if (in) {
 #pragma unroll
-              for (int c = 0; c < 9; c++) h[c] += -ya[ax][c] * rbb;
+              for (int c = 0; c < 9; c++) h[c] += -aux_y(ax, c) * rbb;
+            }
+          }
+          // the fused auxiliary update (what pml_aux_kernel does for the
+          // point, with the same arithmetic): aux' = -(alpha + d) aux +
+          // d * RHS_axis, RHS_axis from the plain derivative along the face's
+          // axis, advanced with the RK4 stage combine
+          if (a.aux_fused && !a.rhs_only) {
+            const float rka = a.rka, rkb = a.rkb;
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+              const bool in = ax == 0 ? inx : ax == 1 ? inj : ink;
+              if (!in) continue;
+              float dd[9];
+#pragma unroll
+              for (int c = 0; c < 9; c++) dd[c] = rDH * (ax == 0 ? Dx(c) : ax == 1 ? Dy(c) : Dz(c));
+              const float e0 = ax == 0 ? xix : ax == 1 ? etx : ztx;
+              const float e1 = ax == 0 ? xiy : ax == 1 ? ety : zty;
+              const float e2 = ax == 0 ? xiz : ax == 1 ? etz : ztz;
+              const int li = ax == 0 ? i : ax == 1 ? j : k;
+              const float d1 = s_pD[ax][li], ad1 = -(s_pA[ax][li] + d1);
+              const int f = fo[ax];
+              const size_t ps = face_ps(ax);
+              const uint32_t pa4 = pa4s[ax];
+              float* Y = s_auxY[f];
+              float* S = s_auxS[f];
+              float* T = s_auxT[f];
+#pragma unroll
+              for (int c = 0; c < 9; c++) {
+                const float y = aux_y(ax, c);
+                const float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
+                float w;
+                if (irk == 0) {
+                  sto(S + c * ps, pa4, y);
+                  sto(T + c * ps, pa4, y + rkb * ah);
+                  w = y + rka * ah;
+                } else if (irk == 3) {
+                  w = (ax == fa ? ta[c] : ldo(T + c * ps, pa4)) + rkb * ah;
+                } else {
+                  sto(T + c * ps, pa4, (ax == fa ? ta[c] : ldo(T + c * ps, pa4)) + rkb * ah);
+                  w = (ax == fa ? sa[c] : ldo(S + c * ps, pa4)) + rka * ah;
+                }
+                sto(Y + c * ps, pa4, w);
+              }
             }
           }
         }
@@ -887,6 +964,8 @@ struct PmlAuxArgs {
   const float* pD[3];
   const float* mat[2];
   int blk0[7];  // first block of each face (prefix sums; off faces have none)
+  int pk0[6];   // face-local k index range [pk0, pk0 + pkn) of the points updated
+  int pkn[6];
   int flag[3];  // one-sided operator sign along each axis
   int N;
   int ni, nj, nk, ny, py, jpad;
@@ -918,12 +997,13 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
   const int off = (f & 1) ? nn - N : 0;
   const int pnj = ax == 1 ? N : a.nj, pnk = ax == 2 ? N : a.nk, pni = ax == 0 ? N : a.ni;
   const long ps = (long)pni * pnj * pnk;
+  const int pk0 = pick6(f, a.pk0), pkn = pick6(f, a.pkn);
   const long idx = (long)(b - fb0) * 256L + threadIdx.x;
-  if (idx >= ps) return;
+  if (idx >= (long)pni * pnj * pkn) return;
   const int pj = (int)(idx % pnj);
   const long t = idx / pnj;
-  const int pk = ax == 0 ? (int)(t / pni) : (int)(t % pnk);
-  const int pi = ax == 0 ? (int)(t % pni) : (int)(t / pnk);
+  const int pk = pk0 + (ax == 0 ? (int)(t / pni) : (int)(t % pkn));
+  const int pi = ax == 0 ? (int)(t % pni) : (int)(t / pkn);
   const long pa = pj + (long)pk * pnj + (long)pi * pnj * pnk;  // aux layout [c][pi][pk][pj]
   const int i = pi + (ax == 0 ? off : 0), j = pj + (ax == 1 ? off : 0), k = pk + (ax == 2 ? off : 0);
   const int li = ax == 0 ? i : ax == 1 ? j : k;
@@ -1057,7 +1137,10 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     (void)hipEventRecord(c.fence[2], ss);
     (void)hipStreamWaitEvent(s, c.fence[2], 0);
   }
-  if (a.pml && !a.rhs_only) {
+  // the auxiliary fields of the points the pipelined band tiles do not
+  // advance themselves (all of them with the general band kernel; with the
+  // fused update only the free-surface rows k >= ktop)
+  if (a.pml && !a.rhs_only && (!a.aux_fused || c.wave_ktop < a.nk)) {
     PmlAuxArgs<MT> p{};
     p.Win = a.Win; p.met = a.met; p.N = c.pml.N;
     p.ni = a.ni; p.nj = a.nj; p.nk = a.nk; p.ny = a.ny; p.py = a.py; p.jpad = a.jpad; p.pyz = a.nyz;
@@ -1072,7 +1155,13 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
       p.Y[f] = c.pml.Y[f]; p.S[f] = c.pml.S[f]; p.T[f] = c.pml.T[f];
       if (!c.pml.on[f]) continue;
       const int ax = f / 2;
-      const long n = (long)c.pml.N * (ax == 0 ? (long)a.nj * a.nk : ax == 1 ? (long)a.nk * a.ni : (long)a.ni * a.nj);
+      const int N = c.pml.N, pnk = ax == 2 ? N : a.nk;
+      // face-local k of global row kmin: the range starts there (clamped)
+      const int kmin = a.aux_fused ? c.wave_ktop : 0;
+      const int koff = ax == 2 && (f & 1) ? a.nk - N : 0;
+      p.pk0[f] = std::min(std::max(kmin - koff, 0), pnk);
+      p.pkn[f] = pnk - p.pk0[f];
+      const long n = (long)(ax == 0 ? N : a.ni) * (ax == 1 ? N : a.nj) * p.pkn[f];
       blocks += (int)((n + 255) / 256);
     }
     p.blk0[6] = blocks;
@@ -1129,6 +1218,7 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
   a.cerjan = c.p.useCerjan ? 1 : 0;
   a.fault = c.p.Rupture ? 1 : 0;
   a.rhs_only = rhs_only ? 1 : 0;
+  a.aux_fused = c.pml_fused ? 1 : 0;
   return a;
 }
 
@@ -1246,6 +1336,9 @@ int upload_wave_cold(Ctx& c) {
       disjoint = false;
   c.wave_band_pipe = p.WaveBandKernel >= 1 && disjoint;
   c.wave_band_tk = c.wave_band_pipe && p.WaveBandKernel == 2 ? TK4 : TK;  // 2: 64 x 4 band tiles, two blocks per CU
+  // PmlAuxFused 1: the pipelined band tiles advance their own points'
+  // auxiliary fields (every band point outside the free-surface rows)
+  c.pml_fused = c.wave_band_pipe && p.PmlAuxFused != 0;
   const int tki = tall ? 2 * TK : (c.wave_pipe && wik == 2) ? TK6 : (c.wave_pipe && wik == 3) ? TK4 : TK;  // interior tile rows
   c.wave_tki = tki;
   for (int ax = 1; ax < 3; ax++) {
@@ -1328,6 +1421,7 @@ int upload_wave_cold(Ctx& c) {
   int ktop = nk;  // first row of the free-surface tiles
   for (const Seg& b : sk)
     if (is_top(b)) ktop = std::min(ktop, b.s);
+  c.wave_ktop = ktop;
   for (int i0 = ia; i0 < ib; i0 += ich_in)
     for (const Seg& b : ski) for (const Seg& a : sji) if (a.in && b.in) push(0, a, b, i0, std::min(ib, i0 + ich_in));
   // band tiles of TK rows, or of TK4 rows with WaveBandKernel 2 (the rows
