@@ -51,7 +51,8 @@ struct Params {
   int WaveInteriorKernel = -1;  // interior tiles: -1 automatic; 0 general kernel; pipelined kernel: 1 64 x 8 with
                                 // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
   int WaveBandKernel = 2;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles
-  int PmlAuxFused = 1;     // 1: the pipelined band tiles advance their points' ADE-PML auxiliary fields
+  int PmlAuxFused = -1;    // 1: the pipelined band tiles advance their points' ADE-PML auxiliary fields;
+                           // 0: a kernel of its own; -1: 1 when few band points lie on two or more faces
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
