@@ -530,14 +530,18 @@ __device__ __forceinline__ float dxv_off(const FArgs& a, int nn, int c, long pc,
 // parts that are independent per slab plane run in parallel, one per block:
 // 0-5 the velocity rates of planes i0-3..-1, +1..+3,
 // 6-11 the stress rates of the same planes, 12-13 the split-node velocity
-// rates of sides 0 and 1, 14-15 their stress rates.  The 16 parts of one
+// rates of sides 0 and 1, 14-15 their stress rates.  The parts of one
 // 64 x 4 tile of fault nodes are consecutive work items, kept on one XCD:
 // they read the same slab planes around the tile, which then come from that
-// XCD's L2 instead of being fetched once per part.
-constexpr int kDerivParts = 16;
+// XCD's L2 instead of being fetched once per part.  Each group of parts
+// (G 0: 0-5, 1: 6-11, 2: 12-15) is a kernel of its own, so that the
+// registers of the split-node stress part do not set the occupancy of the
+// others.
+constexpr int kDerivGroupBase[3] = {0, 6, 12}, kDerivGroupParts[3] = {6, 6, 4};
+template <int G>
 __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a, int tiles_j, int items) {
   const int item = xcd_remap((int)blockIdx.x, items);
-  const int part = item % kDerivParts, tile = item / kDerivParts;
+  const int part = kDerivGroupBase[G] + item % kDerivGroupParts[G], tile = item / kDerivGroupParts[G];
   int j = (tile % tiles_j) * 64 + threadIdx.x;
   int k = (tile / tiles_j) * 4 + threadIdx.y;
   if (j >= a.nj || k >= a.nk) return;
@@ -555,45 +559,63 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a, int tiles_j, 
   float* slab = a.slabH;
   auto slabx = [&](int i, int c) -> float& { return slab[(long)(i - (i0 - 3)) * 9 * pyz + c * pyz + pc]; };
 
-  if (part < 6) {  // velocity rates at i0-3..-1, +1..+3 (fault_dvelo.cu:153-255)
+  if (G == 0) {  // velocity rates at i0-3..-1, +1..+3 (fault_dvelo.cu:153-255)
     const int i = i0 + (part < 3 ? part - 3 : part - 2);
     {
+      // the three axes one after another, each array indexed with constants
+      // only (a runtime index would put it in scratch memory)
       const int nn = i0 - i;
-      float v1[3][7], v2[3][7], v3[3][7];
-      for (int l = -3; l <= 3; l++) {
-        float o[3];
-        trac(a, W, i + l, pc, 0, o);
-        for (int c = 0; c < 3; c++) v1[c][l + 3] = o[c];
-        trac(a, W, i, pc + l, 1, o);
-        for (int c = 0; c < 3; c++) v2[c][l + 3] = o[c];
-        trac(a, W, i, pc + (long)l * py, 2, o);
-        for (int c = 0; c < 3; c++) v3[c][l + 3] = o[c];
-      }
-      v1[0][nn + 3] = F.T11[3 * nyz + pf];
-      v1[1][nn + 3] = F.T12[3 * nyz + pf];
-      v1[2][nn + 3] = F.T13[3 * nyz + pf];
-      for (int c = 0; c < 3; c++) {
-        float* v = v1[c];
-        if (nn == 2) v[6] = 2.0f * v[5] - v[4];
-        if (nn == 1) { v[5] = 2.0f * v[4] - v[3]; v[6] = 2.0f * v[4] - v[2]; }
-        if (nn == -1) { v[0] = 2.0f * v[2] - v[4]; v[1] = 2.0f * v[2] - v[3]; }
-        if (nn == -2) v[0] = 2.0f * v[1] - v[2];
-      }
-      if (a.freenode && km <= 3)
-        for (int c = 0; c < 3; c++) {
-          v3[c][km + 2] = 0.0f;
-          for (int l = km + 3; l < 7; l++) v3[c][l] = -v3[c][2 * (km + 2) - l];
+      float dxt[3], dyt[3], dzt[3];
+      {  // across the fault: the traction on the fault plane replaces the point at i0
+        float v[3][7];
+        for (int l = -3; l <= 3; l++) {
+          float o[3];
+          trac(a, W, i + l, pc, 0, o);
+          for (int c = 0; c < 3; c++) v[c][l + 3] = o[c];
         }
+        const float tf[3] = {F.T11[3 * nyz + pf], F.T12[3 * nyz + pf], F.T13[3 * nyz + pf]};
+        for (int c = 0; c < 3; c++) {
+#pragma unroll
+          for (int l = 0; l < 7; l++)
+            if (l == nn + 3) v[c][l] = tf[c];
+          if (nn == 2) v[c][6] = 2.0f * v[c][5] - v[c][4];
+          if (nn == 1) { v[c][5] = 2.0f * v[c][4] - v[c][3]; v[c][6] = 2.0f * v[c][4] - v[c][2]; }
+          if (nn == -1) { v[c][0] = 2.0f * v[c][2] - v[c][4]; v[c][1] = 2.0f * v[c][2] - v[c][3]; }
+          if (nn == -2) v[c][0] = 2.0f * v[c][1] - v[c][2];
+          dxt[c] = drp(&v[c][3], 1, FX) * rDH;
+        }
+      }
+      {  // along strike
+        float v[3][7];
+        for (int l = -3; l <= 3; l++) {
+          float o[3];
+          trac(a, W, i, pc + l, 1, o);
+          for (int c = 0; c < 3; c++) v[c][l + 3] = o[c];
+        }
+        for (int c = 0; c < 3; c++) dyt[c] = drp(&v[c][3], 1, FY) * rDH;
+      }
+      {  // along dip, with the odd image above the free surface
+        float v[3][7];
+        for (int l = -3; l <= 3; l++) {
+          float o[3];
+          trac(a, W, i, pc + (long)l * py, 2, o);
+          for (int c = 0; c < 3; c++) v[c][l + 3] = o[c];
+        }
+        if (a.freenode && km <= 3)
+          for (int c = 0; c < 3; c++) {
+            if (km == 1) { v[c][3] = 0.0f; v[c][4] = -v[c][2]; v[c][5] = -v[c][1]; v[c][6] = -v[c][0]; }
+            else if (km == 2) { v[c][4] = 0.0f; v[c][5] = -v[c][3]; v[c][6] = -v[c][2]; }
+            else { v[c][5] = 0.0f; v[c][6] = -v[c][4]; }
+          }
+        for (int c = 0; c < 3; c++) dzt[c] = drp(&v[c][3], 1, FZ) * rDH;
+      }
       const long p = pc + i * mps;
       float rrj = 1.0f / (LMEDIA(a, 12, i, pc) * LMF(a, 9, p));
-      for (int c = 0; c < 3; c++) {
-        float dxt = drp(&v1[c][3], 1, FX) * rDH, dyt = drp(&v2[c][3], 1, FY) * rDH, dzt = drp(&v3[c][3], 1, FZ) * rDH;
-        slabx(i, c) = (dxt + dyt + dzt) * rrj;
-      }
+      for (int c = 0; c < 3; c++) slabx(i, c) = (dxt[c] + dyt[c] + dzt[c]) * rrj;
     }
     return;
   }
-  if (part < 12) {  // off-fault stress rates (fault_dstrs_f.cu:1009-1250)
+  if (G == 1) {  // off-fault stress rates (fault_dstrs_f.cu:1009-1250)
     const int nn = part < 9 ? part - 6 : part - 4;
     {
       const int l = nn < 4 ? nn - 3 : nn - 4;
@@ -628,7 +650,7 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a, int tiles_j, 
     }
     return;
   }
-  if (part < 14) {  // split-node velocity rates of side part-12 (fault_dvelo.cu:260-385)
+  if (part < 14) {  // G 2: split-node velocity rates of side part-12 (fault_dvelo.cu:260-385)
     const int m = part - 12;
     const float* TT[3] = {F.T11, F.T12, F.T13};
     float rrj = 1.0f / (F.rho_f[m * nyz + pf] * LMF(a, 9, pc + i0 * mps));
@@ -1051,8 +1073,14 @@ void launch_fault_deriv(Ctx& c, const float* Win, int fx, int fy, int fz) {
   a.FX = fx; a.FY = fy; a.FZ = fz;
   c.prof.begin(c.stream, PK_FAULT_DERIV);
   const dim3 g = fgrid(c);
-  const int items = (int)(g.x * g.y) * kDerivParts;  // per-plane parts and the split-node parts of each tile
-  hipLaunchKernelGGL(fault_deriv_kernel, dim3(items), dim3(64, 4, 1), 0, c.stream, a, (int)g.x, items);
+  const int tiles = (int)(g.x * g.y);
+  // per-plane velocity parts, per-plane stress parts, split-node parts of each tile
+  hipLaunchKernelGGL(fault_deriv_kernel<0>, dim3(tiles * kDerivGroupParts[0]), dim3(64, 4, 1), 0, c.stream, a, (int)g.x,
+                     tiles * kDerivGroupParts[0]);
+  hipLaunchKernelGGL(fault_deriv_kernel<1>, dim3(tiles * kDerivGroupParts[1]), dim3(64, 4, 1), 0, c.stream, a, (int)g.x,
+                     tiles * kDerivGroupParts[1]);
+  hipLaunchKernelGGL(fault_deriv_kernel<2>, dim3(tiles * kDerivGroupParts[2]), dim3(64, 4, 1), 0, c.stream, a, (int)g.x,
+                     tiles * kDerivGroupParts[2]);
   c.prof.end(c.stream);
 }
 
