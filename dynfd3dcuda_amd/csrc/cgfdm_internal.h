@@ -51,6 +51,7 @@ struct Params {
   int WaveInteriorKernel = -1;  // interior tiles: -1 automatic; 0 general kernel; pipelined kernel: 1 64 x 8 with
                                 // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
   int WaveBandKernel = 2;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles
+  int WaveSurfaceKernel = 1;  // free-surface tiles: 0 general kernel; 1 pipelined kernel
   int PmlAuxFused = -1;    // 1: the pipelined band tiles advance their points' ADE-PML auxiliary fields;
                            // 0: a kernel of its own; -1: 1 when few band points lie on two or more faces
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
@@ -200,6 +201,7 @@ struct Ctx {
   int wave_tjn = 32;                // columns of the narrow band tiles (20 or 32)
   int wave_band_tk = 8;             // rows of the 64-wide band tiles (WaveBandKernel 2: 4)
   bool pml_fused = false;           // band tiles advance their ADE-PML auxiliary fields (PmlAuxFused)
+  bool wave_surf_pipe = false;      // pipelined kernel on the free-surface tiles (WaveSurfaceKernel)
   int wave_ktop = 0;                // first row of the free-surface tiles (nk without them)
   int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)

@@ -561,8 +561,10 @@ __device__ __forceinline__ float met_raw_get(const uint32_t (&w)[met_raw_n<MT>()
 // two bands of an axis do not overlap), and added after the three axis sums
 // in the general kernel's order; 1/beta and the sponge factor of plane i+1
 // are fetched during plane i.
-template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB, bool BAND = false, bool RING = true>
+template <int FX, int FY, int FZ, class MT, int TJ_, int NT_, int MINB, bool BAND = false, bool RING = true,
+          bool TOP = false>
 __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a) {
+  static_assert(!TOP || (BAND && TJ_ == 64), "free-surface tiles: band terms, one k row per wavefront");
   constexpr int TJ = TJ_, TK = NT_ / TJ_;
   constexpr int LJ = TJ + 4, LK = TK + 4;
   // halo cells of one plane: the 4 k-halo rows over the tile's own columns
@@ -598,6 +600,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   __shared__ const float* s_cjx;
   __shared__ int s_on[6];
   __shared__ int s_N;
+  __shared__ const float* s_mV[2];  // TOP: the free-surface velocity maps
 
   const int item = xcd_remap((int)blockIdx.x, a.nwork);
   const int4 wk = a.work[2 * item], we = a.work[2 * item + 1];
@@ -677,6 +680,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
         s_pD[threadIdx.x] = a.cold->pD[threadIdx.x];
       }
       if (threadIdx.x == 0) { s_N = a.cold->N; s_pBx = a.cold->pB[0]; s_cjx = a.cold->cj[0]; }
+      if (TOP && threadIdx.x < 2) s_mV[threadIdx.x] = threadIdx.x ? a.cold->mVy : a.cold->mVx;
     }
     if (act && pml) { rb2 = 1.0f / a.cold->pB[1][j]; rb3 = 1.0f / a.cold->pB[2][k]; }
     if (act && cerjan) cjk = fminf(a.cold->cj[1][j + 3], a.cold->cj[2][k + 3]);
@@ -772,7 +776,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
         uint32_t offi = 0;
         if (s_on[0] && i < N) { inx = true; offi = 0; }
         else if (s_on[1] && i >= ni - N) { inx = true; offi = ni - N; }
-        const bool fuse = a.aux_fused && !a.rhs_only;
+        const bool fuse = !TOP && a.aux_fused && !a.rhs_only;
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {
           const bool in = ax == 0 ? inx : ax == 1 ? inj : ink;
@@ -832,7 +836,44 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
         const float s1 = rb1 * rDH, s2 = rb2 * rDH, s3 = rb3 * rDH;
         accum_axis(h, Dx, xix * s1, xiy * s1, xiz * s1, rrho, lam, l2m, mu, true);
         accum_axis(h, Dy, etx * s2, ety * s2, etz * s2, rrho, lam, l2m, mu, true);
-        accum_axis(h, Dz, ztx * s3, zty * s3, ztz * s3, rrho, lam, l2m, mu, true);
+        accum_axis(h, Dz, ztx * s3, zty * s3, ztz * s3, rrho, lam, l2m, mu, !(TOP && a.freenode && k >= nk - 3));
+      }
+      if constexpr (TOP) {
+        // the top three rows under the free surface (wave.cu:207-245), as
+        // the general kernel computes them: DzV of the stress rates from the
+        // traction-free condition, L22 or L24, and the velocity RHS of the
+        // image (image_top_kernel); one k row per wavefront
+        if (a.freenode && k >= nk - 3) {
+          float dzv[3];
+          if (k == nk - 1) {  // traction-free condition (wave.cu:207-232)
+            const uint32_t ij = ((uint32_t)(j + 3) + (uint32_t)(i + 3) * (uint32_t)a.ny) * 9;
+            const float* mVx = s_mV[0];
+            const float* mVy = s_mV[1];
+            const float dxv[3] = {rDH * Dx(0), rDH * Dx(1), rDH * Dx(2)}, dyv[3] = {rDH * Dy(0), rDH * Dy(1), rDH * Dy(2)};
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+              dzv[r] = mVx[ij + 3 * r] * dxv[0] + mVx[ij + 3 * r + 1] * dxv[1] + mVx[ij + 3 * r + 2] * dxv[2]
+                     + mVy[ij + 3 * r] * dyv[0] + mVy[ij + 3 * r + 1] * dyv[1] + mVy[ij + 3 * r + 2] * dyv[2];
+          } else if (k == nk - 2) {  // L22 (wave.cu:233-238)
+#pragma unroll
+            for (int c = 0; c < 3; c++) dzv[c] = rDH * (FZ > 0 ? L[c][lk + 1][lj] - L[c][lk][lj] : L[c][lk][lj] - L[c][lk - 1][lj]);
+          } else {  // L24 (wave.cu:239-245)
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+              dzv[c] = rDH * (FZ > 0 ? -C24_1 * L[c][lk][lj] + C24_2 * L[c][lk + 1][lj] - C24_3 * L[c][lk + 2][lj]
+                                     : C24_3 * L[c][lk - 2][lj] - C24_2 * L[c][lk - 1][lj] + C24_1 * L[c][lk][lj]);
+          }
+          const float e0 = ztx * rb3, e1 = zty * rb3, e2 = ztz * rb3;
+          h[3] += (l2m * e0 * dzv[0] + lam * e1 * dzv[1] + lam * e2 * dzv[2]);
+          h[4] += (lam * e0 * dzv[0] + l2m * e1 * dzv[1] + lam * e2 * dzv[2]);
+          h[5] += (lam * e0 * dzv[0] + lam * e1 * dzv[1] + l2m * e2 * dzv[2]);
+          h[6] += (e1 * dzv[0] + e0 * dzv[1]) * mu;
+          h[7] += (e2 * dzv[0] + e0 * dzv[2]) * mu;
+          h[8] += (e2 * dzv[1] + e1 * dzv[2]) * mu;
+          const long plane = (long)ni * nj, o = ((long)(k - (nk - 3))) * plane + (long)i * nj + j;
+#pragma unroll
+          for (int c = 0; c < 3; c++) h[c] = a.itop[(long)c * 3 * plane + o];
+        }
       }
       if constexpr (BAND) {
         if (pml) {
@@ -849,7 +890,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
           // point, with the same arithmetic): aux' = -(alpha + d) aux +
           // d * RHS_axis, RHS_axis from the plain derivative along the face's
           // axis, advanced with the RK4 stage combine
-          if (a.aux_fused && !a.rhs_only) {
+          if (!TOP && a.aux_fused && !a.rhs_only) {
             const float rka = a.rka, rkb = a.rkb;
 #pragma unroll
             for (int ax = 0; ax < 3; ax++) {
@@ -1105,7 +1146,11 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
       hipLaunchKernelGGL(traction_top_kernel<MT>, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0, ss, a, c.d.nx);
       hipLaunchKernelGGL((image_top_kernel<FX, FY, FZ, MT>), dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, ss, a);
     }
-    hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, ss, a);
+    if (c.wave_surf_pipe)
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2, true, true, true>), dim3(a.nwork), dim3(TJ, TK, 1),
+                         0, ss, a);
+    else
+      hipLaunchKernelGGL((wave_stage_kernel<FX, FY, FZ, MT, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, ss, a);
     pr.end(ss);
   }
   if (n[1] + n[2] > 0) {
@@ -1336,6 +1381,9 @@ int upload_wave_cold(Ctx& c) {
       disjoint = false;
   c.wave_band_pipe = p.WaveBandKernel >= 1 && disjoint;
   c.wave_band_tk = c.wave_band_pipe && p.WaveBandKernel == 2 ? TK4 : TK;  // 2: 64 x 4 band tiles, two blocks per CU
+  // WaveSurfaceKernel 1: the pipelined kernel on the free-surface tiles too
+  // (with the band terms; their auxiliary fields stay with pml_aux_kernel)
+  c.wave_surf_pipe = disjoint && p.WaveSurfaceKernel >= 1;
   // PmlAuxFused 1: the pipelined band tiles advance their own points'
   // auxiliary fields (every band point outside the free-surface rows).  A
   // point on two or three faces (an edge of the absorbing box) reads the

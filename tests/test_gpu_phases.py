@@ -156,10 +156,13 @@ def test_stage_streams_same_result(extra):
                                              ({"WaveBandKernel": 0}, True, 150),
                                              ({"WaveBandKernel": 0, "useCerjan": 1}, True, 150),
                                              ({"WaveBandKernel": 1, "useCerjan": 1}, True, 150),
-                                             ({"PmlAuxFused": 1}, True, 150)],
+                                             ({"PmlAuxFused": 1}, True, 150),
+                                             ({"WaveSurfaceKernel": 0}, True, 150),
+                                             ({"WaveSurfaceKernel": 0, "useCerjan": 1}, True, 150)],
                          ids=["band_round", "tall", "band_round_tall", "surface_tiles_off", "general_interior",
                               "interior_64x6", "interior_64x4", "interior_64x8_double_buffer", "general_band",
-                              "general_band_cerjan", "band_64x8_cerjan", "aux_update_fused"])
+                              "general_band_cerjan", "band_64x8_cerjan", "aux_update_fused", "general_surface",
+                              "general_surface_cerjan"])
 def test_tilings_give_the_same_result(layout, exact, ny):
     """The tiling options only move points between the interior, band and
     free-surface launches, or between interior kernels of other tile shapes;
@@ -174,8 +177,9 @@ def test_tilings_give_the_same_result(layout, exact, ny):
       kernel (the automatic choice on this grid; NX 30 puts 16 of the 30
       planes in the x bands, so most y- and z-band points are edges of two
       or three faces, whose second and third auxiliary words the band tiles
-      read where they use them), give the default tiling's result bit for
-      bit;
+      read where they use them), and the general kernel instead of the
+      pipelined one on the free-surface tiles, give the default tiling's
+      result bit for bit;
     - WaveSurfaceTiles 0 moves rows between the free-surface kernel and the
       others, whose code the compiler contracts differently (multiply-adds
       fused or not), so the result agrees to FP32 rounding: 1e-5 relative
