@@ -10,7 +10,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-SOURCES = ["params.cpp", "setup.cpp", "netcdf_classic.cpp", "kernels.hip", "wave_stage.hip", "mlp_train.hip", "stream_probe.hip", "field_check.hip", "capi.cpp"]
+SOURCES = ["params.cpp", "setup.cpp", "netcdf_classic.cpp", "kernels.hip", "wave_stage.hip", "wave_stage_f32.hip", "wave_stage_bf16.hip",
+           "wave_stage_bf16dev.hip", "mlp_train.hip", "stream_probe.hip", "field_check.hip", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled for all device code; multiply-adds
 # are fused only within one source expression (-ffp-contract=on), so kernels

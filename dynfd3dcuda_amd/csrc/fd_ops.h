@@ -1,5 +1,5 @@
 // Finite-difference operators and small device helpers shared by the wave
-// stage kernel (wave_stage.hip) and the split-node fault kernels
+// stage kernels (wave_stage_kernels.h) and the split-node fault kernels
 // (kernels.hip).  Everything here is header-only device code.
 #pragma once
 #include "cgfdm_internal.h"

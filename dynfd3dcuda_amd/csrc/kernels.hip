@@ -4,7 +4,7 @@
 //   fault_deriv = fault_dvelo + fault_dstrs_{f,b}
 //   fault_post  = slab RK + fault_rk + fault_rk_stage2 + fault2wave (+Cerjan)
 //   rup_sensor  = rupture-front sensor (two passes)
-// The volume stencil that precedes them in each stage is in wave_stage.hip.
+// The volume stencil that precedes them in each stage is in wave_stage_kernels.h.
 #include "fd_ops.h"
 #include "mlp.h"
 #include <hip/hip_bf16.h>

@@ -41,8 +41,10 @@ def kernel_class(name):
     if m:
         return STAGE_CLASSES[int(m.group(1))]
     m = re.search(r"wave_interior_kernel<([^>]*)>", name)
-    if m:  # the pipelined kernel: FX, FY, FZ, metric type, tile width, threads, waves per SIMD[, BAND, RING]
+    if m:  # the pipelined kernel: FX, FY, FZ, metric type, tile width, threads, waves per SIMD[, BAND, RING, TOP]
         args = [x.strip() for x in m.group(1).split(",")]
+        if len(args) > 9 and args[9] == "true":
+            return "stage_surface"
         return "stage_band" if len(args) > 7 and args[7] == "true" else "stage_interior"
     if "wave_interior_kernel" in name:
         return "stage_interior"
