@@ -252,6 +252,7 @@ int cgfdm_destroy(cgfdm_ctx* h) {
   for (void* p : c.allocations) (void)hipFree(p);
   if (c.hW) (void)hipFree(c.hW);
   if (c.wave_work) (void)hipFree(c.wave_work);
+  if (c.pml_boxes) (void)hipFree(c.pml_boxes);
   if (c.mlp_dev) (void)hipFree(c.mlp_dev);
   delete h;
   return 0;

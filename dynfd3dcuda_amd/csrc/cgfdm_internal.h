@@ -52,8 +52,8 @@ struct Params {
                                 // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
   int WaveBandKernel = 2;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles
   int WaveSurfaceKernel = 1;  // free-surface tiles: 0 general kernel; 1 pipelined kernel
-  int PmlAuxFused = -1;    // 1: the pipelined band tiles advance their points' ADE-PML auxiliary fields;
-                           // 0: a kernel of its own; -1: 1 when few band points lie on two or more faces
+  int PmlAuxFused = 1;     // 1: the pipelined band tiles advance the ADE-PML auxiliary fields of their
+                           // points on one face; 0: a kernel of its own advances every band point's
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
@@ -203,6 +203,8 @@ struct Ctx {
   bool pml_fused = false;           // band tiles advance their ADE-PML auxiliary fields (PmlAuxFused)
   bool wave_surf_pipe = false;      // pipelined kernel on the free-surface tiles (WaveSurfaceKernel)
   int wave_ktop = 0;                // first row of the free-surface tiles (nk without them)
+  int* pml_boxes = nullptr;         // boxes of the auxiliary-update kernel (8 ints each), on the device
+  int pml_nbox = 0, pml_blocks = 0;
   int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the

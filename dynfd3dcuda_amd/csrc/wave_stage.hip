@@ -155,26 +155,13 @@ int upload_wave_cold(Ctx& c) {
   // WaveSurfaceKernel 1: the pipelined kernel on the free-surface tiles too
   // (with the band terms; their auxiliary fields stay with pml_aux_kernel)
   c.wave_surf_pipe = disjoint && p.WaveSurfaceKernel >= 1;
-  // PmlAuxFused 1: the pipelined band tiles advance their own points'
-  // auxiliary fields (every band point outside the free-surface rows).  A
-  // point on two or three faces (an edge of the absorbing box) reads the
-  // second and third faces' words in the middle of the compute, which stalls
-  // its wavefront, so -1 (automatic) fuses only when such points are at most
-  // 6% of the band points (1600 x 1000 x 500: 3.6%, bands + aux 38.2 -> 34.2
-  // ms per step; 400 x 300 x 200: 11.4%, 4.5 -> 5.5 ms)
-  bool fuse = p.PmlAuxFused > 0;
-  if (p.PmlAuxFused < 0) {
-    long nb[3], nn[3] = {c.d.ni, c.d.nj, c.d.nk};
-    for (int ax = 0; ax < 3; ax++)
-      nb[ax] = std::min<long>(nn[ax], (long)c.pml.N * ((c.pml.on[2 * ax] ? 1 : 0) + (c.pml.on[2 * ax + 1] ? 1 : 0)));
-    const long total = nn[0] * nn[1] * nn[2];
-    const long none = (nn[0] - nb[0]) * (nn[1] - nb[1]) * (nn[2] - nb[2]);
-    const long one = nb[0] * (nn[1] - nb[1]) * (nn[2] - nb[2]) + (nn[0] - nb[0]) * nb[1] * (nn[2] - nb[2]) +
-                     (nn[0] - nb[0]) * (nn[1] - nb[1]) * nb[2];
-    const long band = total - none, edges = band - one;
-    fuse = band > 0 && edges <= 0.06 * band;
-  }
-  c.pml_fused = c.wave_band_pipe && fuse;
+  // PmlAuxFused 1: the pipelined band tiles advance the auxiliary fields of
+  // their points on one face only; pml_aux_kernel keeps the edges of the
+  // absorbing box (points on two or three faces, whose words the band tiles
+  // prefetch with the first face's but do not advance) and the free-surface
+  // rows (1600 x 1000 x 500: 138.5 -> 128.6 ms per step; 400 x 300 x 200:
+  // 7.67 -> 6.90 ms; interleaved runs)
+  c.pml_fused = c.wave_band_pipe && p.PmlAuxFused != 0;
   const int tki = tall ? 2 * TK : (c.wave_pipe && wik == 2) ? TK6 : (c.wave_pipe && wik == 3) ? TK4 : TK;  // interior tile rows
   c.wave_tki = tki;
   for (int ax = 1; ax < 3; ax++) {
@@ -281,6 +268,65 @@ int upload_wave_cold(Ctx& c) {
     for (const Seg& b : sk)
       if (is_top(b))
         for (const Seg& a : sjt) push(3, a, b, i0, std::min(ni, i0 + ich_top));
+  // boxes of the auxiliary-update kernel: every absorbing face whole; with
+  // the fused update only the points the band tiles leave to it, the
+  // free-surface rows and the edges of the absorbing box (points on two or
+  // three faces).  In a face's cross-section (u, v) the edges are the
+  // u-band rows for every v, then the v-band columns of the u-middle; the
+  // free-surface rows add the u-middle x v-middle rows from ktop up.
+  {
+    const int N = c.pml.N, nn3[3] = {c.d.ni, c.d.nj, c.d.nk};
+    auto bands = [&](int axis) {
+      std::vector<std::pair<int, int>> v;  // (start, count)
+      if (c.pml.on[2 * axis]) v.push_back({0, N});
+      if (c.pml.on[2 * axis + 1]) v.push_back({nn3[axis] - N, N});
+      return v;
+    };
+    auto middle = [&](int axis) {
+      const int lo = c.pml.on[2 * axis] ? N : 0, hi = c.pml.on[2 * axis + 1] ? nn3[axis] - N : nn3[axis];
+      return std::make_pair(lo, std::max(hi - lo, 0));
+    };
+    std::vector<int> bx;
+    int blocks = 0;
+    // box in face-local (j, k, i) coordinates
+    auto add = [&](int f, int j0, int nj_, int k0, int nk_, int i0, int ni_) {
+      if (nj_ <= 0 || nk_ <= 0 || ni_ <= 0) return;
+      bx.insert(bx.end(), {f, j0, nj_, k0, nk_, i0, ni_, blocks});
+      blocks += (int)(((long)nj_ * nk_ * ni_ + 255) / 256);
+    };
+    for (int f = 0; f < 6; f++) {
+      if (!c.pml.on[f]) continue;
+      const int ax = f / 2;
+      const int pnj = ax == 1 ? N : c.d.nj, pnk = ax == 2 ? N : c.d.nk, pni = ax == 0 ? N : c.d.ni;
+      if (!c.pml_fused) { add(f, 0, pnj, 0, pnk, 0, pni); continue; }
+      if (ax == 0) {  // u = j, v = k
+        const auto mu = middle(1), mv = middle(2);
+        for (auto b : bands(1)) add(f, b.first, b.second, 0, pnk, 0, pni);
+        for (auto b : bands(2)) add(f, mu.first, mu.second, b.first, b.second, 0, pni);
+        const int s0 = std::max(ktop, mv.first);
+        add(f, mu.first, mu.second, s0, mv.first + mv.second - s0, 0, pni);
+      } else if (ax == 1) {  // u = i, v = k
+        const auto mu = middle(0), mv = middle(2);
+        for (auto b : bands(0)) add(f, 0, pnj, 0, pnk, b.first, b.second);
+        for (auto b : bands(2)) add(f, 0, pnj, b.first, b.second, mu.first, mu.second);
+        const int s0 = std::max(ktop, mv.first);
+        add(f, 0, pnj, s0, mv.first + mv.second - s0, mu.first, mu.second);
+      } else {  // u = i, v = j; the face's own k: free-surface rows inside the band
+        const auto mu = middle(0), mv = middle(1);
+        for (auto b : bands(0)) add(f, 0, pnj, 0, pnk, b.first, b.second);
+        for (auto b : bands(1)) add(f, b.first, b.second, 0, pnk, mu.first, mu.second);
+        const int off = (f & 1) ? c.d.nk - N : 0, s0 = std::max(ktop - off, 0);
+        add(f, mv.first, mv.second, s0, N - s0, mu.first, mu.second);
+      }
+    }
+    c.pml_nbox = (int)bx.size() / kPmlBoxInts;
+    c.pml_blocks = blocks;
+    if (c.pml_boxes) { (void)hipFree(c.pml_boxes); c.pml_boxes = nullptr; }
+    if (!bx.empty()) {
+      if (hipMalloc(&c.pml_boxes, bx.size() * sizeof(int)) != hipSuccess) return 1;
+      if (hipMemcpy(c.pml_boxes, bx.data(), bx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return 1;
+    }
+  }
   std::vector<int4> all;
   for (int l = 0; l < 4; l++) {
     all.insert(all.end(), lists[l].begin(), lists[l].end());

@@ -50,6 +50,7 @@ constexpr int TJB = 20;    // exact-width tiles of the absorbing bands along j w
 constexpr int TKB = 12;    // columns wide (20 x 12, 240 threads; the 20-layer PML and sponge)
 constexpr int TK6 = 6;     // rows of the 64 x 6 interior tiles (WaveInteriorKernel 2)
 constexpr int TK4 = 4;     // rows of the 64 x 4 interior tiles (WaveInteriorKernel 3)
+constexpr int kPmlBoxInts = 8;  // ints of one box of the auxiliary update
 
 
 // Volume arrays are plane-interleaved with padded rows: word (field c,
@@ -765,13 +766,16 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
       }
     }
     // BAND: the face along each axis whose band holds the point, and the aux
-    // words of plane i of the first such face (pmlx.cu:126-210 and the y/z
-    // variants): its value, and with the fused auxiliary update its
-    // step-start (S) and accumulator (T) words; those of a second or third
-    // face (at the edges of the absorbing box) are read where they are used
+    // words of plane i (pmlx.cu:126-210 and the y/z variants), all issued
+    // with S and T: the first face's value in ya; for a point on that face
+    // alone and the fused auxiliary update, its step-start (S) and
+    // accumulator (T) words in sa and ta; for a point on two or three faces
+    // (an edge of the absorbing box, whose auxiliary fields pml_aux_kernel
+    // advances) the second and third faces' values in sa and ta instead
     float ya[9], sa[9], ta[9];
     bool inx = false;
-    int fa = -1;                        // first face axis of the point
+    int fa = -1, fb = -1;               // first and second face axes of the point
+    int nfc = 0;                        // faces holding the point
     int fo[3] = {0, 0, 0};              // face of each axis
     uint32_t pa4s[3] = {0, 0, 0};       // aux byte offset of the point in the face of each axis
     auto face_ps = [&](int ax) { return (size_t)(ax == 0 ? N : ni) * (ax == 1 ? N : nj) * (ax == 2 ? N : nk); };
@@ -793,6 +797,8 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
           const uint32_t pnj = ax == 1 ? N : nj, pnk = ax == 2 ? N : nk;
           pa4s[ax] = (pj + pk * pnj + pi * pnj * pnk) << 2;
           if (fa < 0) fa = ax;
+          else if (fb < 0) fb = ax;
+          nfc++;
         }
         if (fa >= 0) {
           const int f = fo[fa];
@@ -800,19 +806,29 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
           const uint32_t pa4 = pa4s[fa];
 #pragma unroll
           for (int c = 0; c < 9; c++) ya[c] = ldo(s_auxY[f] + c * ps, pa4);
-          if (fuse && irk != 0) {
+          if (nfc == 1 && fuse && irk != 0) {
 #pragma unroll
             for (int c = 0; c < 9; c++) ta[c] = ldo(s_auxT[f] + c * ps, pa4);
           }
-          if (fuse && (irk == 1 || irk == 2)) {
+          if (nfc == 1 && fuse && (irk == 1 || irk == 2)) {
 #pragma unroll
             for (int c = 0; c < 9; c++) sa[c] = ldo(s_auxS[f] + c * ps, pa4);
           }
         }
+        if (nfc >= 2) {  // the second face's value
+          const size_t ps = face_ps(fb);
+#pragma unroll
+          for (int c = 0; c < 9; c++) sa[c] = ldo(s_auxY[fo[fb]] + c * ps, pa4s[fb]);
+        }
+        if (nfc == 3) {  // the third face's value (the axes are then 0, 1, 2)
+          const size_t ps = face_ps(2);
+#pragma unroll
+          for (int c = 0; c < 9; c++) ta[c] = ldo(s_auxY[fo[2]] + c * ps, pa4s[2]);
+        }
       }
     }
-    // aux word c of the face along axis ax (kept in registers for the first face)
-    auto aux_y = [&](int ax, int c) { return ax == fa ? ya[c] : ldo(s_auxY[fo[ax]] + c * face_ps(ax), pa4s[ax]); };
+    // aux value c of the face along axis ax (a face holding the point)
+    auto aux_y = [&](int ax, int c) { return ax == fa ? ya[c] : ax == fb ? sa[c] : ta[c]; };
 
     if (act) {
       auto MET = [&](int c) { return met_raw_get<MT>(mw, c, a.mref); };
@@ -896,12 +912,11 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
           // point, with the same arithmetic): aux' = -(alpha + d) aux +
           // d * RHS_axis, RHS_axis from the plain derivative along the face's
           // axis, advanced with the RK4 stage combine
-          if (!TOP && a.aux_fused && !a.rhs_only) {
+          if (!TOP && a.aux_fused && !a.rhs_only && nfc == 1) {  // a point on one face only
             const float rka = a.rka, rkb = a.rkb;
 #pragma unroll
             for (int ax = 0; ax < 3; ax++) {
-              const bool in = ax == 0 ? inx : ax == 1 ? inj : ink;
-              if (!in) continue;
+              if (ax != fa) continue;
               float dd[9];
 #pragma unroll
               for (int c = 0; c < 9; c++) dd[c] = rDH * (ax == 0 ? Dx(c) : ax == 1 ? Dy(c) : Dz(c));
@@ -926,10 +941,10 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
                   sto(T + c * ps, pa4, y + rkb * ah);
                   w = y + rka * ah;
                 } else if (irk == 3) {
-                  w = (ax == fa ? ta[c] : ldo(T + c * ps, pa4)) + rkb * ah;
+                  w = ta[c] + rkb * ah;
                 } else {
-                  sto(T + c * ps, pa4, (ax == fa ? ta[c] : ldo(T + c * ps, pa4)) + rkb * ah);
-                  w = (ax == fa ? sa[c] : ldo(S + c * ps, pa4)) + rka * ah;
+                  sto(T + c * ps, pa4, ta[c] + rkb * ah);
+                  w = sa[c] + rka * ah;
                 }
                 sto(Y + c * ps, pa4, w);
               }
@@ -1010,9 +1025,10 @@ struct PmlAuxArgs {
   const float* pB[3];
   const float* pD[3];
   const float* mat[2];
-  int blk0[7];  // first block of each face (prefix sums; off faces have none)
-  int pk0[6];   // face-local k index range [pk0, pk0 + pkn) of the points updated
-  int pkn[6];
+  // boxes of points to update, kPmlBoxInts ints each: face, then the
+  // face-local (start, count) of j, k and i, then the box's first block
+  const int* __restrict__ boxes;
+  int nbox;
   int flag[3];  // one-sided operator sign along each axis
   int N;
   int ni, nj, nk, ny, py, jpad;
@@ -1029,28 +1045,33 @@ __device__ __forceinline__ T pick6(int f, const T (&v)[6]) {
 template <class T>
 __device__ __forceinline__ T pick3(int x, const T (&v)[3]) { return x == 0 ? v[0] : x == 1 ? v[1] : v[2]; }
 
-// One launch for all absorbing faces: blocks [blk0[f], blk0[f+1]) serve
-// face f, numbered XCD-contiguously so that the blocks sharing stencil
-// planes (neighbouring band planes on x faces, rows on z faces) share one L2.
+// One launch for all absorbing faces: each box of points (a face, or a part
+// of one) is served by a contiguous run of blocks, numbered
+// XCD-contiguously so that the blocks sharing stencil planes (neighbouring
+// band planes on x faces, rows on z faces) share one L2.
 template <class MT>
 __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
   const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-  int f = 0, fb0 = 0;
-#pragma unroll
-  for (int q = 1; q < 6; q++)
-    if (b >= a.blk0[q]) { f = q; fb0 = a.blk0[q]; }
+  // the box of this block (block-uniform loads; a handful of boxes)
+  const int* B = a.boxes;
+  int q = 0;
+  for (int r = 1; r < a.nbox; r++)
+    if (b >= B[r * kPmlBoxInts + 7]) q = r;
+  const int* bx = B + q * kPmlBoxInts;
+  const int f = bx[0], bj0 = bx[1], bnj = bx[2], bk0 = bx[3], bnk = bx[4], bi0 = bx[5], bni = bx[6];
   const int ax = f >> 1, N = a.N, flag = pick3(ax, a.flag);
   const int nn = ax == 0 ? a.ni : ax == 1 ? a.nj : a.nk;
   const int off = (f & 1) ? nn - N : 0;
   const int pnj = ax == 1 ? N : a.nj, pnk = ax == 2 ? N : a.nk, pni = ax == 0 ? N : a.ni;
   const long ps = (long)pni * pnj * pnk;
-  const int pk0 = pick6(f, a.pk0), pkn = pick6(f, a.pkn);
-  const long idx = (long)(b - fb0) * 256L + threadIdx.x;
-  if (idx >= (long)pni * pnj * pkn) return;
-  const int pj = (int)(idx % pnj);
-  const long t = idx / pnj;
-  const int pk = pk0 + (ax == 0 ? (int)(t / pni) : (int)(t % pkn));
-  const int pi = ax == 0 ? (int)(t % pni) : (int)(t / pkn);
+  const long idx = (long)(b - bx[7]) * 256L + threadIdx.x;
+  if (idx >= (long)bnj * bnk * bni) return;
+  const int pj = bj0 + (int)(idx % bnj);
+  const long t = idx / bnj;
+  // j fastest, then the band plane index on x faces (its five x-derivative
+  // planes are read by neighbouring blocks), k on the others
+  const int pk = bk0 + (ax == 0 ? (int)(t / bni) : (int)(t % bnk));
+  const int pi = bi0 + (ax == 0 ? (int)(t % bni) : (int)(t / bnk));
   const long pa = pj + (long)pk * pnj + (long)pi * pnj * pnk;  // aux layout [c][pi][pk][pj]
   const int i = pi + (ax == 0 ? off : 0), j = pj + (ax == 1 ? off : 0), k = pk + (ax == 2 ? off : 0);
   const int li = ax == 0 ? i : ax == 1 ? j : k;
@@ -1190,8 +1211,9 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
   }
   // the auxiliary fields of the points the pipelined band tiles do not
   // advance themselves (all of them with the general band kernel; with the
-  // fused update only the free-surface rows k >= ktop)
-  if (a.pml && !a.rhs_only && (!a.aux_fused || c.wave_ktop < a.nk)) {
+  // fused update those of the free-surface rows k >= ktop and of the edges
+  // of the absorbing box, on two or three faces)
+  if (a.pml && !a.rhs_only) {
     PmlAuxArgs<MT> p{};
     p.Win = a.Win; p.met = a.met; p.N = c.pml.N;
     p.ni = a.ni; p.nj = a.nj; p.nk = a.nk; p.ny = a.ny; p.py = a.py; p.jpad = a.jpad; p.pyz = a.nyz;
@@ -1200,22 +1222,12 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     p.flag[0] = FX; p.flag[1] = FY; p.flag[2] = FZ;
     for (int ax = 0; ax < 3; ax++) { p.pA[ax] = c.pml.A[ax]; p.pB[ax] = c.pml.B[ax]; p.pD[ax] = c.pml.D[ax]; }
     p.mat[0] = c.matVx2Vz; p.mat[1] = c.matVy2Vz;
-    int blocks = 0;
-    for (int f = 0; f < 6; f++) {
-      p.blk0[f] = blocks;
-      p.Y[f] = c.pml.Y[f]; p.S[f] = c.pml.S[f]; p.T[f] = c.pml.T[f];
-      if (!c.pml.on[f]) continue;
-      const int ax = f / 2;
-      const int N = c.pml.N, pnk = ax == 2 ? N : a.nk;
-      // face-local k of global row kmin: the range starts there (clamped)
-      const int kmin = a.aux_fused ? c.wave_ktop : 0;
-      const int koff = ax == 2 && (f & 1) ? a.nk - N : 0;
-      p.pk0[f] = std::min(std::max(kmin - koff, 0), pnk);
-      p.pkn[f] = pnk - p.pk0[f];
-      const long n = (long)(ax == 0 ? N : a.ni) * (ax == 1 ? N : a.nj) * p.pkn[f];
-      blocks += (int)((n + 255) / 256);
-    }
-    p.blk0[6] = blocks;
+    for (int f = 0; f < 6; f++) { p.Y[f] = c.pml.Y[f]; p.S[f] = c.pml.S[f]; p.T[f] = c.pml.T[f]; }
+    // the boxes (upload_wave_cold): every face whole, or with the fused
+    // update the free-surface rows and the edges of the absorbing box
+    p.boxes = c.pml_boxes;
+    p.nbox = c.pml_nbox;
+    const int blocks = c.pml_blocks;
     if (blocks > 0) {
       pr.begin(s, PK_PML_AUX);
       hipLaunchKernelGGL(pml_aux_kernel<MT>, dim3((unsigned)blocks), dim3(256), 0, s, p);

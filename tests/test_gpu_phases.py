@@ -156,12 +156,12 @@ def test_stage_streams_same_result(extra):
                                              ({"WaveBandKernel": 0}, True, 150),
                                              ({"WaveBandKernel": 0, "useCerjan": 1}, True, 150),
                                              ({"WaveBandKernel": 1, "useCerjan": 1}, True, 150),
-                                             ({"PmlAuxFused": 1}, True, 150),
+                                             ({"PmlAuxFused": 0}, True, 150),
                                              ({"WaveSurfaceKernel": 0}, True, 150),
                                              ({"WaveSurfaceKernel": 0, "useCerjan": 1}, True, 150)],
                          ids=["band_round", "tall", "band_round_tall", "surface_tiles_off", "general_interior",
                               "interior_64x6", "interior_64x4", "interior_64x8_double_buffer", "general_band",
-                              "general_band_cerjan", "band_64x8_cerjan", "aux_update_fused", "general_surface",
+                              "general_band_cerjan", "band_64x8_cerjan", "aux_update_separate", "general_surface",
                               "general_surface_cerjan"])
 def test_tilings_give_the_same_result(layout, exact, ny):
     """The tiling options only move points between the interior, band and
@@ -172,12 +172,12 @@ def test_tilings_give_the_same_result(layout, exact, ny):
     - moves between the interior and band kernels, the general, 64 x 6,
       64 x 4 and double-buffered 64 x 8 interior kernels, the general
       kernel and 64 x 8 tiles on the absorbing bands
-      (with ADE-PML, and with PML and the Cerjan sponge together), and the
-      ADE-PML auxiliary update in the band tiles instead of in its own
-      kernel (the automatic choice on this grid; NX 30 puts 16 of the 30
-      planes in the x bands, so most y- and z-band points are edges of two
-      or three faces, whose second and third auxiliary words the band tiles
-      read where they use them), and the general kernel instead of the
+      (with ADE-PML, and with PML and the Cerjan sponge together), the
+      ADE-PML auxiliary update of every band point in its own kernel
+      instead of in the band tiles (which by default advance the points on
+      one face only and leave the edges of the absorbing box to that
+      kernel; NX 30 puts 16 of the 30 planes in the x bands, so most y- and
+      z-band points are such edges), and the general kernel instead of the
       pipelined one on the free-surface tiles, give the default tiling's
       result bit for bit;
     - WaveSurfaceTiles 0 moves rows between the free-surface kernel and the
