@@ -26,7 +26,10 @@ slip/state accumulation.  Prints one JSON line:
                  the deviation-form metric) / average launch time; traffic =
                  HBM bytes per launch from the committed rocprofv3
                  FETCH/WRITE summary of the same workload; stream ceilings =
-                 a hand-written streaming kernel's GB/s for read:write mixes
+                 a hand-written streaming kernel's GB/s for read:write mixes;
+                 frac_with_aux counts the ADE-PML auxiliary words the kernel
+                 reads and advances too (AUX_READ_BYTES, AUX_UPDATE_BYTES
+                 per point and face)
   cpu_baseline   the FP64 oracle (C++/OpenMP) on the host cores, on the
                  100 x 1000 x 500 sub-slab around the fault of the same
                  workload (the full grid needs ~300 GB of host memory in FP64)
@@ -71,6 +74,18 @@ def b_alg(params):
     or 9 bfloat16 metric deviations and 3 exact FP32 media words)."""
     m = params.get("MetricBF16", 0)
     return 144 * 4 + (48 * 4 if not m else 48 * 2 if m == 1 else 4 * (9 * 2 + 3 * 4))
+
+
+# algorithmic bytes of the ADE-PML auxiliary fields per (point, face) pair
+# and RK stage (pmlx.cu:126-210, rk4.cu:154-257): the 9 auxiliary words read
+# for the -aux/beta term of the stage; the ODE step's step-start and
+# accumulator reads (none in stage 0, 18 words in stages 1 and 2, 9 in stage
+# 3: 11.25 on average) and its writes (auxiliary, step-start and
+# accumulator words in stage 0, auxiliary and accumulator in 1 and 2, the
+# auxiliary words in 3: 18 on average), its own read of the 9 auxiliary
+# words being the one above
+AUX_READ_BYTES = 36
+AUX_UPDATE_BYTES = 117
 
 
 def workload_name(name, p):
@@ -266,17 +281,23 @@ def timed_run(case, steps, warmup, workload):
         newton = {"solves": int(cnt[0]), "iters_per_solve": cnt[1] / max(cnt[0], 1), "fallbacks": int(cnt[2]),
                   "learned_start_share": cnt[3] / max(cnt[0], 1), "network_wave_share": cnt[4] / max(cnt[0], 1)}
     finite = bool(np.all(np.isfinite(solver.get("Vs1"))) and np.all(np.isfinite(solver.get("Ts1"))))
+    aux = solver.profile_aux()
     solver.close()
     value = npts * steps / t
     ba = b_alg(p)
     ms, n, pts = prof[dom]
     avg_s = ms / n / 1e3
     achieved = pts * ba / 4.0 / avg_s / 1e9  # one launch = one RK stage of its points
+    aux_bytes = sum(rd * AUX_READ_BYTES + up * AUX_UPDATE_BYTES for rd, up in aux.values())  # per RK stage
+    rd, up = aux.get(dom, (0, 0))
+    achieved_aux = (pts * ba / 4.0 + rd * AUX_READ_BYTES + up * AUX_UPDATE_BYTES) / avg_s / 1e9
     return {"value": value, "ms_per_step": 1e3 * t / steps, "npts": npts, "B_alg": ba, "setup_s": t_setup,
             "step_frac": value * ba / (PEAK_HBM_GBS * 1e9), "dom": dom, "achieved": achieved,
             "avg_launch_ms": avg_s * 1e3, "points_per_launch": pts, "traffic": read_traffic(workload, dom),
             "kernel_ms_per_step": {k: v[0] / PROFILE_STEPS for k, v in prof_all.items() if v[1] > 0},
             "stage_points_per_launch": {k: v[2] for k, v in stage.items()}, "newton": newton,
+            "aux_pairs_per_launch": aux, "achieved_with_aux": achieved_aux,
+            "step_frac_with_aux": (value * ba + 4 * aux_bytes * steps / t) / (PEAK_HBM_GBS * 1e9),
             "fault_outputs_finite": finite}
 
 
@@ -414,9 +435,13 @@ def main():
                    "network": os.path.relpath(p["NewtonMLP"], ROOT) if p.get("NewtonMLP") else None,
                    "B_alg_bytes_per_pt_step": r["B_alg"], "step_hbm_roofline_frac": r["step_frac"],
                    "setup_s": round(r["setup_s"], 1), "stage_points_per_launch": r["stage_points_per_launch"],
+                   "aux_pairs_per_launch": r["aux_pairs_per_launch"],
+                   "aux_bytes_per_pair": {"read": AUX_READ_BYTES, "update": AUX_UPDATE_BYTES},
+                   "step_hbm_roofline_frac_with_aux": r["step_frac_with_aux"],
                    "newton_in_timed_steps": r["newton"], "fault_outputs_finite": r["fault_outputs_finite"]},
         "roofline": {"bound": "hbm", "kernel": r["dom"], "achieved": r["achieved"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": r["achieved"] / PEAK_HBM_GBS, "traffic": r["traffic"],
+                     "frac_with_aux": r["achieved_with_aux"] / PEAK_HBM_GBS,
                      "points_per_launch": r["points_per_launch"], "avg_launch_ms": r["avg_launch_ms"]},
         "kernel_ms_per_step": r["kernel_ms_per_step"],
         "cpu_baseline": None,
@@ -447,8 +472,10 @@ def main():
         out["side"] = {"workload": workload_name(SIDE_WORKLOAD, cs["params"]), "value": rs_["value"],
                        "ms_per_step": rs_["ms_per_step"], "steps": 50, "warmup": 5,
                        "step_hbm_roofline_frac": rs_["step_frac"],
+                       "step_hbm_roofline_frac_with_aux": rs_["step_frac_with_aux"],
                        "roofline": {"kernel": rs_["dom"], "achieved": rs_["achieved"],
                                     "frac": rs_["achieved"] / PEAK_HBM_GBS, "traffic": rs_["traffic"],
+                                    "frac_with_aux": rs_["achieved_with_aux"] / PEAK_HBM_GBS,
                                     "points_per_launch": rs_["points_per_launch"],
                                     "avg_launch_ms": rs_["avg_launch_ms"]},
                        "kernel_ms_per_step": rs_["kernel_ms_per_step"]}

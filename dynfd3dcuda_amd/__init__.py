@@ -30,7 +30,7 @@ EXPORTED = [
     "cgfdm_run", "cgfdm_synchronize", "cgfdm_wave_stage", "cgfdm_wave_rhs", "cgfdm_field_info",
     "cgfdm_get_field", "cgfdm_put_field", "cgfdm_get_dims", "cgfdm_open_outputs",
     "cgfdm_write_outputs", "cgfdm_close_outputs", "cgfdm_newton_stats", "cgfdm_profile",
-    "cgfdm_profile_get", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
+    "cgfdm_profile_get", "cgfdm_profile_aux", "cgfdm_set_newton_mlp", "cgfdm_mlp_forward", "cgfdm_mlp_grad", "cgfdm_mlp_train",
     "cgfdm_stage_flags", "cgfdm_set_stream", "cgfdm_fault_trial", "cgfdm_fault_deriv", "cgfdm_fault_update",
     "cgfdm_rup_sensor", "cgfdm_fault_smooth", "cgfdm_thermpress", "cgfdm_profile_classes",
     "cgfdm_stream_probe", "cgfdm_check_field", "cgfdm_print_params", "cgfdm_mlp_forward_noalloc",
@@ -97,6 +97,7 @@ def load_library(path=None):
     lib.cgfdm_profile.argtypes = [c_p, ctypes.c_int]
     lib.cgfdm_profile_get.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+    lib.cgfdm_profile_aux.argtypes = [c_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
     lib.cgfdm_set_newton_mlp.argtypes = [c_p, c_p, ctypes.c_long]
     lib.cgfdm_mlp_forward.argtypes = [c_p, c_p, ctypes.c_long, c_p, ctypes.c_int]
     lib.cgfdm_mlp_grad.argtypes = [c_p, c_p, c_p, ctypes.c_long, c_p, c_p]
@@ -347,4 +348,16 @@ class Solver:
             self._check(self.lib.cgfdm_profile_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n),
                                                    ctypes.byref(pts)))
             out[name] = (ms.value, n.value, pts.value)
+        return out
+
+    def profile_aux(self):
+        """{kernel class: (ADE-PML (point, face) pairs whose auxiliary value
+        one launch reads for the stage, pairs whose auxiliary ODE step it
+        takes)} for the classes with such work."""
+        out = {}
+        for name in PROFILE_KERNELS:
+            rd, up = ctypes.c_long(), ctypes.c_long()
+            self._check(self.lib.cgfdm_profile_aux(self.h, name.encode(), ctypes.byref(rd), ctypes.byref(up)))
+            if rd.value or up.value:
+                out[name] = (rd.value, up.value)
         return out

@@ -642,6 +642,20 @@ int cgfdm_profile_get(cgfdm_ctx* h, const char* name, double* total_ms, long* la
   return 0;
 }
 
+int cgfdm_profile_aux(cgfdm_ctx* h, const char* name, long* read_pairs, long* update_pairs) {
+  if (!h || !name) return fail(CGFDM_EPARAM, "null argument");
+  Ctx& c = h->c;
+  if (!c.setup_done) return fail(CGFDM_ESTATE, "cgfdm_setup has not run");
+  int k = -1;
+  for (int q = 0; q < PK_COUNT; q++)
+    if (std::strcmp(name, kProfNames[q]) == 0) k = q;
+  if (k < 0) return fail(CGFDM_EPARAM, std::string("unknown kernel class ") + name);
+  const int x = k <= PK_STAGE_SURFACE ? k : k == PK_PML_AUX ? 3 : -1;
+  if (read_pairs) *read_pairs = x >= 0 ? c.aux_read_pairs[x] : 0;
+  if (update_pairs) *update_pairs = x >= 0 ? c.aux_update_pairs[x] : 0;
+  return 0;
+}
+
 int cgfdm_set_newton_mlp(cgfdm_ctx* h, const float* params, long n) {
   if (!h || !params) return fail(CGFDM_EPARAM, "null argument");
   if (n != kMlpParams) return fail(CGFDM_EPARAM, "the Newton network needs 4555 parameters");

@@ -213,6 +213,10 @@ struct Ctx {
   const float* met_fault_base = nullptr;
   int wave_nkind[4] = {0, 0, 0, 0};    // items per list: interior, bands, narrow bands, free-surface tiles
   long wave_points[3] = {0, 0, 0};  // grid points per launch of each stage-kernel kind
+  // ADE-PML (point, face) pairs per launch of each stage-kernel kind and of
+  // pml_aux_kernel (index 3): whose auxiliary value it reads for the -aux/beta
+  // term, and whose auxiliary ODE step it takes
+  long aux_read_pairs[4] = {0, 0, 0, 0}, aux_update_pairs[4] = {0, 0, 0, 0};
   void* mlp_dev = nullptr;          // MlpDev of the learned Newton guess (NewtonMode 2)
   std::vector<float> mlp_blob;      // its parameters, kept for setup
   Profiler prof;

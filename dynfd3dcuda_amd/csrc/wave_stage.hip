@@ -218,10 +218,26 @@ int upload_wave_cold(Ctx& c) {
   // 3 tiles holding free-surface rows
   std::vector<int4> lists[4];
   for (int kd = 0; kd < 3; kd++) c.wave_points[kd] = 0;
+  for (int kd = 0; kd < 3; kd++) c.aux_read_pairs[kd] = c.aux_update_pairs[kd] = 0;
+  // points of [lo, hi) along axis ax inside that axis's absorbing bands
+  auto in_bands = [&](int ax, int lo, int hi) {
+    const int N = c.pml.N, nn = n3[ax];
+    const int a0 = 0, a1 = c.pml.on[2 * ax] ? std::min(N, nn) : 0;
+    const int b0 = c.pml.on[2 * ax + 1] ? std::max(nn - N, 0) : nn, b1 = nn;
+    auto len = [&](int x0, int x1) { return std::max(0, std::min(x1, hi) - std::max(x0, lo)); };
+    return len(a0, a1) + len(b0, b1) - len(std::max(a0, b0), std::min(a1, b1));
+  };
   auto push = [&](int list, const Seg& a, const Seg& b, int i0, int i1) {
     lists[list].push_back(make_int4(a.s, b.s, i0, i1));
     lists[list].push_back(make_int4(a.n, b.n, a.o, b.o));
-    c.wave_points[list == 0 ? 0 : list == 3 ? 2 : 1] += (long)(a.n - a.o) * (b.n - b.o) * (i1 - i0);
+    const int kd = list == 0 ? 0 : list == 3 ? 2 : 1;
+    const long Li = i1 - i0, Lj = a.n - a.o, Lk = b.n - b.o;
+    c.wave_points[kd] += Li * Lj * Lk;
+    const long bi = in_bands(0, i0, i1), bj = in_bands(1, a.s + a.o, a.s + a.n), bk = in_bands(2, b.s + b.o, b.s + b.n);
+    c.aux_read_pairs[kd] += bi * Lj * Lk + Li * bj * Lk + Li * Lj * bk;
+    // the fused update in the band tiles: points on one face only
+    if (kd == 1 && c.pml_fused)
+      c.aux_update_pairs[kd] += bi * (Lj - bj) * (Lk - bk) + (Li - bi) * bj * (Lk - bk) + (Li - bi) * (Lj - bj) * bk;
   };
   const auto narrow = [&](const Seg& a) { return !a.in && a.n - a.o <= TJN; };
   // bands of at most TJB columns get tiles exactly TJB wide (with the
@@ -288,11 +304,13 @@ int upload_wave_cold(Ctx& c) {
     };
     std::vector<int> bx;
     int blocks = 0;
+    c.aux_update_pairs[3] = 0;
     // box in face-local (j, k, i) coordinates
     auto add = [&](int f, int j0, int nj_, int k0, int nk_, int i0, int ni_) {
       if (nj_ <= 0 || nk_ <= 0 || ni_ <= 0) return;
       bx.insert(bx.end(), {f, j0, nj_, k0, nk_, i0, ni_, blocks});
       blocks += (int)(((long)nj_ * nk_ * ni_ + 255) / 256);
+      c.aux_update_pairs[3] += (long)nj_ * nk_ * ni_;
     };
     for (int f = 0; f < 6; f++) {
       if (!c.pml.on[f]) continue;

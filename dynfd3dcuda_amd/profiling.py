@@ -5,12 +5,12 @@
       `rocprofv3 --kernel-trace --output-format csv` run)
   python -m dynfd3dcuda_amd.profiling traffic --fetch D1 --write D2 --cal-fetch D3 --cal-write D4 --bench-json F
       HBM bytes per RK stage of each kernel class (all its dispatches in one
-      stage: the band class has two, the PML class one per absorbing face)
-      from `--pmc FETCH_SIZE` and `--pmc WRITE_SIZE` runs, each scaled by what
-      the same counter reads for a device-to-device copy of known size
-      (`bench.py --calibrate`), next to the algorithmic bytes of the stage
-      kernels (points per stage from the JSON line bench.py printed in the
-      same configuration)
+      stage: the band class has two) from `--pmc FETCH_SIZE` and
+      `--pmc WRITE_SIZE` runs, each scaled by what the same counter reads for
+      a device-to-device copy of known size (`bench.py --calibrate`), next to
+      the algorithmic bytes of the stage kernels (points per stage from the
+      JSON line bench.py printed in the same configuration), and those with
+      the ADE-PML auxiliary words each class reads and advances added
 
 Kernel classes: stage_interior / stage_band / stage_surface (the
 instantiations of the wave stage kernel and of the pipelined stage kernel,
@@ -110,6 +110,8 @@ def traffic(a):
     bj = json.loads(lines[-1])
     pts = bj["config"].get("stage_points_per_launch", {})
     ba = bj["config"]["B_alg_bytes_per_pt_step"]
+    aux = bj["config"].get("aux_pairs_per_launch", {})
+    abyte = bj["config"].get("aux_bytes_per_pair", {})
     out = [f"# calibration kernel: {copy_k}; FETCH_SIZE x {f_fetch:.3f}, WRITE_SIZE x {f_write:.3f} (KiB -> bytes x factor)",
            f"fetch_scale: {f_fetch:.4f}", f"write_scale: {f_write:.4f}", f"B_alg_bytes_per_pt_step: {ba}"]
     # one fault_pre dispatch per RK stage (rupture runs); else the interior stage kernel
@@ -123,10 +125,15 @@ def traffic(a):
         out.append(f"{k}_fetch_bytes_per_stage: {fb:.6g}")
         out.append(f"{k}_write_bytes_per_stage: {wb:.6g}")
         out.append(f"{k}_hbm_bytes_per_stage: {fb + wb:.6g}")
-        if k in pts and pts[k]:
-            alg = pts[k] * ba / 4.0
+        alg = pts.get(k, 0) * ba / 4.0
+        if alg:
             out.append(f"{k}_alg_bytes_per_stage: {alg:.6g}")
             out.append(f"{k}_hbm_over_alg: {(fb + wb) / alg:.3f}")
+        if k in aux and abyte:  # the ADE-PML auxiliary words the class reads and advances
+            rd, up = aux[k]
+            alg_aux = alg + rd * abyte["read"] + up * abyte["update"]
+            out.append(f"{k}_alg_bytes_with_aux_per_stage: {alg_aux:.6g}")
+            out.append(f"{k}_hbm_over_alg_with_aux: {(fb + wb) / alg_aux:.3f}")
     return "\n".join(out)
 
 

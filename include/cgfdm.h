@@ -185,6 +185,13 @@ int cgfdm_profile(cgfdm_ctx* ctx, int enable);
  * "" for all, the default); the other launches then carry no events. */
 int cgfdm_profile_classes(cgfdm_ctx* ctx, const char* names);
 int cgfdm_profile_get(cgfdm_ctx* ctx, const char* name, double* total_ms, long* launches, long* points_per_launch);
+/* ADE-PML work of one launch of kernel class `name` (extension, no
+ * counterpart in the reference; for byte accounting): the (point, face)
+ * pairs whose auxiliary value it reads for the -aux/beta term of the stage
+ * (a point on two or three absorbing faces counts once per face), and those
+ * whose auxiliary ODE step (pmlx.cu:126-210 and the y/z variants) it takes;
+ * 0 for classes without such work.  Needs cgfdm_setup. */
+int cgfdm_profile_aux(cgfdm_ctx* ctx, const char* name, long* read_pairs, long* update_pairs);
 
 /* Device-side check of an FP32 field (extension, no counterpart in the
  * reference): *nonfinite = NaN/Inf words, *ghost_nonzero = nonzero words in

@@ -10,7 +10,9 @@ cgfdm_fault_update, then cgfdm_fault_smooth.
   the split-node rates after the derivatives (FP32 against FP64 after 8
   steps: 1e-4 relative, norm-wise);
 * launches can go on a caller's stream (cgfdm_set_stream) with the same
-  result."""
+  result;
+* the tiling options give the same result, and the ADE-PML work of the
+  launches covers every point of every absorbing face once."""
 import numpy as np
 import pytest
 
@@ -206,3 +208,34 @@ def test_tilings_give_the_same_result(layout, exact, ny):
             den = np.linalg.norm(y)
             e = np.linalg.norm(x - y) / den if den > 0 else np.linalg.norm(x)
             assert e < 1e-5, (name, e)
+
+
+@pytest.mark.parametrize("layout", [{}, {"PmlAuxFused": 0}, {"WaveBandKernel": 0}, {"WaveBandKernel": 1},
+                                    {"WaveSurfaceTiles": 0}, {"FreeSurface": 0}],
+                         ids=["default", "aux_separate", "general_band", "band_64x8", "surface_tiles_off",
+                              "no_free_surface"])
+def test_aux_pairs_cover_every_face_point_once(layout):
+    """The ADE-PML work the launches report (cgfdm_profile_aux) partitions
+    the absorbing faces: every (point, face) pair has its auxiliary value
+    read by exactly one stage-kernel launch and its auxiliary ODE step taken
+    by exactly one launch (the band tiles for points on one face with the
+    fused update, pml_aux_kernel for the edges of the absorbing box, the
+    free-surface rows, or everything).  A box table of pml_aux_kernel that
+    missed or repeated a box shows here as a wrong total."""
+    case = cases.sw_case(curved=True, NY=150, **layout)
+    s = dyn.Solver(case["params"])
+    cases.apply_to_solver(case, s)
+    s.setup()
+    d = s.dims()
+    p = case["params"]
+    N, n = p["PML_N"], (d["ni"], d["nj"], d["nk"])
+    on = [1, 1, 1, 1, 1, 0 if p.get("FreeSurface", 0) else 1]
+    total = sum(N * n[(f // 2 + 1) % 3] * n[(f // 2 + 2) % 3] for f in range(6) if on[f])
+    aux = s.profile_aux()
+    s.close()
+    assert total > 0
+    assert sum(rd for rd, _ in aux.values()) == total, aux
+    assert sum(up for _, up in aux.values()) == total, aux
+    assert aux["pml_aux"][0] == 0
+    fused = layout.get("PmlAuxFused", 1) and layout.get("WaveBandKernel", 2) >= 1
+    assert (aux["stage_band"][1] > 0) == bool(fused), aux
