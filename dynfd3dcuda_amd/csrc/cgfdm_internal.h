@@ -54,6 +54,8 @@ struct Params {
   int WaveSurfaceKernel = 1;  // free-surface tiles: 0 general kernel; 1 pipelined kernel
   int PmlAuxFused = 1;     // 1: the pipelined band tiles advance the ADE-PML auxiliary fields of their
                            // points on one face; 0: a kernel of its own advances every band point's
+  int WaveBufferSkew = 16384;  // words (a multiple of 32) by which buffer q of S, A, B, T starts q times
+                               // further into its allocation
   int WaveBandIChunk = 0;  // planes per block of the band tiles (0: automatic)
   int WaveTopIChunk = 0;  // planes per block of the free-surface tiles (0: automatic)
   int WaveSurfaceTiles = 1;  // free-surface tiles = the top TK rows, own line-aligned j tiles (0: earlier layout)
@@ -77,8 +79,9 @@ struct Dims {
   // Device volume arrays (wavefields, metric, fault slab) are plane-
   // interleaved with a padded row: word (field c, padded plane i, padded
   // row k, padded column j) of an array with nf fields per plane sits at
-  // (i*nf + c)*pyz + k*py + jpad + j.  py is a multiple of 32 words and jpad
-  // puts the first interior stencil tile on a 128-byte line.
+  // (i*nf + c)*pyz + k*py + jpad + j.  py is a multiple of 32 words, not a
+  // power of two, and jpad puts the first interior stencil tile on a
+  // 128-byte line.
   int py, jpad;
   long pyz;
 };

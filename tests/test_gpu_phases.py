@@ -160,11 +160,12 @@ def test_stage_streams_same_result(extra):
                                              ({"WaveBandKernel": 1, "useCerjan": 1}, True, 150),
                                              ({"PmlAuxFused": 0}, True, 150),
                                              ({"WaveSurfaceKernel": 0}, True, 150),
-                                             ({"WaveSurfaceKernel": 0, "useCerjan": 1}, True, 150)],
+                                             ({"WaveSurfaceKernel": 0, "useCerjan": 1}, True, 150),
+                                             ({"WaveBufferSkew": 0}, True, 150)],
                          ids=["band_round", "tall", "band_round_tall", "surface_tiles_off", "general_interior",
                               "interior_64x6", "interior_64x4", "interior_64x8_double_buffer", "general_band",
                               "general_band_cerjan", "band_64x8_cerjan", "aux_update_separate", "general_surface",
-                              "general_surface_cerjan"])
+                              "general_surface_cerjan", "buffers_unskewed"])
 def test_tilings_give_the_same_result(layout, exact, ny):
     """The tiling options only move points between the interior, band and
     free-surface launches, or between interior kernels of other tile shapes;
@@ -180,7 +181,8 @@ def test_tilings_give_the_same_result(layout, exact, ny):
       one face only and leave the edges of the absorbing box to that
       kernel; NX 30 puts 16 of the 30 planes in the x bands, so most y- and
       z-band points are such edges), and the general kernel instead of the
-      pipelined one on the free-surface tiles, give the default tiling's
+      pipelined one on the free-surface tiles, and wavefield buffers at
+      their allocations' bases instead of skewed, give the default tiling's
       result bit for bit;
     - WaveSurfaceTiles 0 moves rows between the free-surface kernel and the
       others, whose code the compiler contracts differently (multiply-adds
