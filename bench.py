@@ -23,7 +23,8 @@ slip/state accumulation.  Prints one JSON line:
                  solver's stream around its own launches over the timed
                  region; achieved = algorithmic bytes per launch
                  (points x B_alg/4; B_alg = 768 B/pt-step FP32, 696 B with
-                 the deviation-form metric) / average launch time; traffic =
+                 the deviation-form metric, 656 B with its media palette) /
+                 average launch time; traffic =
                  HBM bytes per launch from the committed rocprofv3
                  FETCH/WRITE summary of the same workload; stream ceilings =
                  a hand-written streaming kernel's GB/s for read:write mixes;
@@ -68,12 +69,15 @@ CPU_SUB_NX = 100       # fault-normal planes of the CPU baseline's sub-slab of t
 NEWTON_MLP = os.path.join(ROOT, "dynfd3dcuda_amd", "data", "newton_guess_4x64x64x1.f32")
 
 
-def b_alg(params):
+def b_alg(params, palette=False):
     """Algorithmic bytes per grid-point update (one full RK4 step): 144
     wavefield words + 48 coefficient words (12 per stage: FP32; bfloat16;
-    or 9 bfloat16 metric deviations and 3 exact FP32 media words)."""
+    or 9 bfloat16 metric deviations and 3 exact FP32 media words, or with
+    a media palette the deviations and a 16-bit palette entry)."""
     m = params.get("MetricBF16", 0)
-    return 144 * 4 + (48 * 4 if not m else 48 * 2 if m == 1 else 4 * (9 * 2 + 3 * 4))
+    if m == 2:
+        return 144 * 4 + 4 * (9 * 2 + (2 if palette else 3 * 4))
+    return 144 * 4 + (48 * 4 if not m else 48 * 2)
 
 
 # algorithmic bytes of the ADE-PML auxiliary fields per (point, face) pair
@@ -96,7 +100,7 @@ def workload_name(name, p):
             "curved_rs_mlp_400x300x200": "curved grid, rate-and-state aging law, learned Newton guess",
             "large_rs_mlp_bf16_1600x1000x500": "curved grid + topography, bent fault, 3-layer medium, 20-layer "
                                                "ADE-PML, rate-and-state aging law with the learned Newton guess, "
-                                               "metric as BF16 deviations + FP32 media"}
+                                               "metric as BF16 deviations, exact media"}
     return f"{name}: {p['NX']}x{p['NY']}x{p['NZ']} at {p['DH']:g} m, {what.get(name, '')}"
 
 
@@ -282,9 +286,10 @@ def timed_run(case, steps, warmup, workload):
                   "learned_start_share": cnt[3] / max(cnt[0], 1), "network_wave_share": cnt[4] / max(cnt[0], 1)}
     finite = bool(np.all(np.isfinite(solver.get("Vs1"))) and np.all(np.isfinite(solver.get("Ts1"))))
     aux = solver.profile_aux()
+    palette = solver.media_palette_entries() > 0
     solver.close()
     value = npts * steps / t
-    ba = b_alg(p)
+    ba = b_alg(p, palette)
     ms, n, pts = prof[dom]
     avg_s = ms / n / 1e3
     achieved = pts * ba / 4.0 / avg_s / 1e9  # one launch = one RK stage of its points
@@ -296,7 +301,7 @@ def timed_run(case, steps, warmup, workload):
             "avg_launch_ms": avg_s * 1e3, "points_per_launch": pts, "traffic": read_traffic(workload, dom),
             "kernel_ms_per_step": {k: v[0] / PROFILE_STEPS for k, v in prof_all.items() if v[1] > 0},
             "stage_points_per_launch": {k: v[2] for k, v in stage.items()}, "newton": newton,
-            "aux_pairs_per_launch": aux, "achieved_with_aux": achieved_aux,
+            "aux_pairs_per_launch": aux, "achieved_with_aux": achieved_aux, "palette": palette,
             "step_frac_with_aux": (value * ba + 4 * aux_bytes * steps / t) / (PEAK_HBM_GBS * 1e9),
             "fault_outputs_finite": finite}
 
@@ -428,7 +433,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (seeded generators, dynfd3dcuda_amd.configs)",
         "config": {"workload": workload_name(args.workload, p), "grid_points": r["npts"],
-                   "metric_storage": {0: "fp32", 1: "bf16", 2: "bf16_deviation_fp32_media"}[p.get("MetricBF16", 0)],
+                   "metric_storage": {0: "fp32", 1: "bf16", 2: "bf16_deviation_media_palette" if r["palette"]
+                                      else "bf16_deviation_fp32_media"}[p.get("MetricBF16", 0)],
                    "newton": {0: "reference rule", 1: "plain guess, residual rule",
                               2: "learned guess, residual rule"}.get(p.get("NewtonMode", 0), "")
                    if p.get("Friction_type", 0) >= 1 else None,

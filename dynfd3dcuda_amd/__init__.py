@@ -290,6 +290,14 @@ class Solver:
         self._check(self.lib.cgfdm_field_info(self.h, name.encode(), ctypes.byref(n), ctypes.byref(t)))
         return n.value, _DT[t.value]
 
+    def media_palette_entries(self):
+        """Entries of the media palette the stage kernels read lambda, mu,
+        rho from (MetricBF16=2, MetricPalette=1); 0 when they read the FP32
+        media words."""
+        n, t = ctypes.c_size_t(), ctypes.c_int()
+        rc = self.lib.cgfdm_field_info(self.h, b"M_palette", ctypes.byref(n), ctypes.byref(t))
+        return n.value // 4 if rc == 0 else 0
+
     def get(self, name):
         n, dt = self.field_info(name)
         out = np.empty(n, dtype=dt)

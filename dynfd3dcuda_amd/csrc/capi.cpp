@@ -77,12 +77,18 @@ FieldRef field(Ctx& c, const std::string& n) {
   if (n == "W") { f32(c.cur, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "hW") { f32(c.hW, 9 * nxyz); r.nf = 9; r.planes = nx; }
   else if (n == "M") { f32(c.p.MetricBF16 ? nullptr : c.met, 13 * nxyz); r.nf = 13; r.planes = nx; }
-  else if (n == "M_bf16" && c.p.MetricBF16 == 2) {  // 5 words, each two metric deviations from M_ref
-    r.ptr = c.met; r.count = 5 * nxyz; r.dtype = 2; r.nf = 8; r.planes = nx; r.fn = 5;
+  else if (n == "M_bf16" && c.p.MetricBF16 == 2) {  // 5 words: the 9 metric deviations from M_ref and the palette entry
+    r.ptr = c.met; r.count = 5 * nxyz; r.dtype = 2; r.nf = 9; r.planes = nx; r.fn = 5;
+  }
+  else if (n == "M_jac_bf16") {  // MetricBF16 = 2: the Jacobian's deviation from M_ref[9] (low half)
+    r.ptr = c.p.MetricBF16 == 2 ? c.met : nullptr; r.count = nxyz; r.dtype = 2; r.nf = 9; r.planes = nx; r.f0 = 8; r.fn = 1;
+  }
+  else if (n == "M_palette") {  // MetricBF16 = 2: lambda, mu, rho, 0 of each palette entry (null without a palette)
+    f32(c.met_pal, 4 * (size_t)c.met_npal);
   }
   else if (n == "M_bf16") { r.ptr = c.p.MetricBF16 ? c.met : nullptr; r.count = 13 * nxyz; r.dtype = 3; r.nf = 13; r.planes = nx; }
   else if (n == "M_media") {  // MetricBF16 = 2: lambda, mu, rho, the FP32 planes in slots 10-15
-    f32(c.p.MetricBF16 == 2 ? c.met : nullptr, 3 * nxyz); r.nf = 8; r.planes = nx; r.f0 = 5; r.fn = 3;
+    f32(c.p.MetricBF16 == 2 ? c.met : nullptr, 3 * nxyz); r.nf = 9; r.planes = nx; r.f0 = 5; r.fn = 3;
   }
   else if (n == "M_ref") { r.ptr = c.p.MetricBF16 == 2 ? c.met_ref : nullptr; r.count = 10; r.dtype = 0; r.host = true; }
   else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);

@@ -39,6 +39,8 @@ struct Params {
   int usePML = 1, useCerjan = 0, FreeSurface = 1, NormalRefine = 1, Rupture = 1;
   int FaultSmooth = 0;  // Makefile:104: Gaussian smoothing of the fault tractions once per step
   // extensions
+  int MetricPalette = 1;  // MetricBF16 = 2: lambda, mu, rho as entries of a palette of the distinct media (a 1-D layered
+                          // or two-half model of at most 256 media) instead of FP32 planes; 0: always the planes
   int MetricBF16 = 0;   // 1: metric + media stored in bfloat16; 2: metric as bfloat16 deviations from the Cartesian grid, media exact (FP32 compute)
   int NewtonMode = 0;   // 0: reference NRsearch; 1: residual tolerance + plain fallback
   double NewtonTol = 1e-10;
@@ -73,6 +75,8 @@ int parse_params_json(const std::string& text, Params& p, std::string& err);
 // ---------------------------------------------------------------------------
 // Device-side views.  Volume arrays use the reference index
 // pos = j + k*ny + i*ny*nz over padded dims (3 ghost layers per side).
+constexpr int kMaxMediaPalette = 256;  // media palette entries (MetricBF16 = 2) at most
+
 struct Dims {
   int ni, nj, nk, nx, ny, nz, i0;
   long nxyz, nyz;
@@ -210,6 +214,10 @@ struct Ctx {
   int pml_nbox = 0, pml_blocks = 0;
   int wave_tki = 8;                 // rows of an interior tile
   float met_ref[10] = {};           // metric words' reference values (MetricBF16 = 2)
+  // MetricBF16 = 2: the media palette on the device (lambda, mu, rho, 0 per
+  // entry; null when the media have none) and its entries
+  float* met_pal = nullptr;
+  int met_npal = 0;
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the
   // fault kernels read, and that copy's base moved back by its first plane
   float* met_fault = nullptr;

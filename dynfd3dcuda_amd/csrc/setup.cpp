@@ -90,7 +90,10 @@ __host__ __device__ inline double dot3d(const double* a, const double* b) { retu
 // index of the interior point whose values a ghost point mirrors
 // (extend_Symm_array, metric.cu:173-246, applied along x, then y, then z)
 __host__ __device__ inline int mirror_idx(int a, int n) { return a < 3 ? 6 - a : (a > n - 4 ? 2 * (n - 4) - a : a); }
-__host__ __device__ inline void metric_point(const MetricIn& m, int i, int j, int k, double* o) {
+// o: xi(3) eta(3) zeta(3) jac lam mu rho; *entry (if given): the point's
+// media palette entry, the layer of a 1-D model or the half of a two-half
+// one (0 with 3-D media, which have no palette)
+__host__ __device__ inline void metric_point(const MetricIn& m, int i, int j, int k, double* o, int* entry = nullptr) {
   const long q = j + (long)k * m.ny + (long)i * m.nyz;
   const long s[3] = {m.nyz, 1, (long)m.ny};
   double t[3][3];  // t[axis][component] = d(x,y,z)/d(xi,eta,zeta)
@@ -110,19 +113,23 @@ __host__ __device__ inline void metric_point(const MetricIn& m, int i, int j, in
   cross3(t[0], t[1], g);
   for (int a = 0; a < 3; a++) o[6 + a] = g[a] / jac;
   double lam = i < m.NX / 2 ? m.lam1 : m.lam2, miu = i < m.NX / 2 ? m.miu1 : m.miu2, rho = i < m.NX / 2 ? m.rho1 : m.rho2;
+  int e = i < m.NX / 2 ? 0 : 1;
   if (m.nl > 0) {
     const double depth1 = -m.Z[q];
     double r = m.r1[m.nl - 1], va = m.p1[m.nl - 1], vb = m.s1[m.nl - 1];
+    e = m.nl - 1;
     for (int l = m.nl - 1; l >= 0; l--)
-      if (depth1 > m.d1[l]) { r = m.r1[l]; va = m.p1[l]; vb = m.s1[l]; break; }
+      if (depth1 > m.d1[l]) { r = m.r1[l]; va = m.p1[l]; vb = m.s1[l]; e = l; break; }
     lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
   }
   if (m.vp3) {
     const long q3 = (j - 3) + (long)(k - 3) * m.nj + (long)(i - 3) * m.nj * m.nk;
     const double r = m.rho3[q3], va = m.vp3[q3], vb = m.vs3[q3];
     lam = r * (va * va - 2 * vb * vb); miu = r * vb * vb; rho = r;
+    e = 0;
   }
   o[10] = lam; o[11] = miu; o[12] = rho;
+  if (entry) *entry = e;
 }
 
 __device__ inline float metric_store(double v, float*) { return (float)v; }
@@ -133,11 +140,12 @@ __device__ inline uint16_t metric_store(double v, uint16_t*) {  // bf16, round t
 
 // The 13 metric and media words of every stored point (ghosts mirrored), in
 // the device layout, and the largest P-wave speed of the interior as the
-// bits of a positive double.  One thread per point, j fastest.  NF = 16 is
+// bits of a positive double.  One thread per point, j fastest.  NF = 18 is
 // the bfloat16-deviation form: the bfloat16 of each metric word less its
 // field's reference value ref.r[c], two to a 32-bit word (slots c and c+1,
-// field c in the low half), and the slots 10-15 of a plane hold lambda, mu,
-// rho as three FP32 planes.
+// field c in the low half) for the nine coefficients, the media palette
+// entry in slot 9, lambda, mu, rho as three FP32 planes in slots 10-15, and
+// the Jacobian's deviation in slot 16 (slot 17 zero).
 struct MetRef {
   double r[10];
 };
@@ -153,14 +161,16 @@ __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, 
     const long r = idx / m.ny;
     const int k = (int)(r % m.nz), i = ib + (int)(r / m.nz), io = i - ib;
     double v[13];
-    metric_point(m, mirror_idx(i, m.nx), mirror_idx(j, m.ny), mirror_idx(k, m.nz), v);
+    int entry = 0;
+    metric_point(m, mirror_idx(i, m.nx), mirror_idx(j, m.ny), mirror_idx(k, m.nz), v, &entry);
     const long base = (long)k * py + jpad + j;
-    if constexpr (NF == 16) {
-      for (int c = 0; c < 10; c += 2) {  // two deviations per 32-bit word, field c in the low half
-        const uint32_t lo = metric_store(v[c] - ref.r[c], out), hi = metric_store(v[c + 1] - ref.r[c + 1], out);
-        reinterpret_cast<uint32_t*>(out + ((long)io * 16 + c) * pyz)[base] = lo | (hi << 16);
-      }
-      float* media = reinterpret_cast<float*>(out + ((long)io * 16 + 10) * pyz);
+    if constexpr (NF == 18) {
+      auto word = [&](int s) { return reinterpret_cast<uint32_t*>(out + ((long)io * 18 + s) * pyz) + base; };
+      for (int c = 0; c < 8; c += 2)  // two deviations per 32-bit word, field c in the low half
+        *word(c) = metric_store(v[c] - ref.r[c], out) | ((uint32_t)metric_store(v[c + 1] - ref.r[c + 1], out) << 16);
+      *word(8) = metric_store(v[8] - ref.r[8], out) | ((uint32_t)entry << 16);
+      *word(16) = metric_store(v[9] - ref.r[9], out);
+      float* media = reinterpret_cast<float*>(out + ((long)io * 18 + 10) * pyz);
       for (int c = 10; c < 13; c++) media[(long)(c - 10) * pyz + base] = (float)v[c];
     } else {
       for (int c = 0; c < 13; c++) out[((long)io * 13 + c) * pyz + base] = metric_store(v[c], out);
@@ -170,6 +180,36 @@ __global__ void __launch_bounds__(256) metric_media_kernel(MetricIn m, OT* out, 
   }
   for (int o = 32; o > 0; o >>= 1) vp = fmax(vp, __shfl_xor(vp, o));
   if ((threadIdx.x & 63) == 0 && vp > 0.0) atomicMax(vpmax_bits, (unsigned long long)__double_as_longlong(vp));
+}
+
+// The media palette of the bfloat16-deviation form: with check 0 every
+// stored point writes its lambda, mu, rho to its palette entry (the points
+// of one entry hold the same words, so the racing writes agree); with check
+// 1 every point compares its entry with its own words, and *bad counts the
+// points that differ or name no entry.
+__global__ void __launch_bounds__(256) media_palette_kernel(const uint16_t* met, long pyz, int py, int jpad, int nx, int ny,
+                                                            int nz, float4* pal, int npal, int check, unsigned* bad) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)nx * nz * ny) return;
+  const int j = (int)(idx % ny);
+  const long r = idx / ny;
+  const int k = (int)(r % nz), i = (int)(r / nz);
+  const long base = (long)k * py + jpad + j;
+  const uint32_t e = reinterpret_cast<const uint32_t*>(met + ((long)i * 18 + 8) * pyz)[base] >> 16;
+  const float* media = reinterpret_cast<const float*>(met + ((long)i * 18 + 10) * pyz);
+  const float4 v = make_float4(media[base], media[pyz + base], media[2 * pyz + base], 0.0f);
+  if (e >= (uint32_t)npal) {
+    atomicAdd(bad, 1u);
+    return;
+  }
+  if (!check) {
+    pal[e] = v;
+    return;
+  }
+  const float4 w = pal[e];
+  if (__float_as_uint(w.x) != __float_as_uint(v.x) || __float_as_uint(w.y) != __float_as_uint(v.y) ||
+      __float_as_uint(w.z) != __float_as_uint(v.z))
+    atomicAdd(bad, 1u);
 }
 
 // ---- setup passes on the device, FP64 (one thread per point or node)
@@ -713,7 +753,7 @@ int host_setup(Ctx& c, std::string& err) {
     }
     unsigned long long* dvp = (unsigned long long*)dput(nullptr, sizeof(unsigned long long));
     bool ok = dvp && (md.nl == 0 || (md.d1 && md.r1 && md.p1 && md.s1)) && (c.m3d_vp.empty() || (md.vp3 && md.vs3 && md.rho3));
-    const long nmet = (long)nx * (p.MetricBF16 == 2 ? 16 : 13) * pyz;
+    const long nmet = (long)nx * (p.MetricBF16 == 2 ? 18 : 13) * pyz;
     void* met = nullptr;
     if (ok) met = p.MetricBF16 ? (void*)dmalloc<uint16_t>(c, nmet) : (void*)dmalloc<float>(c, nmet);
     ok = ok && met && hipMemset(dvp, 0, sizeof(unsigned long long)) == hipSuccess;
@@ -749,11 +789,30 @@ int host_setup(Ctx& c, std::string& err) {
     if (ok) {
       const unsigned blocks = blocks_for((long)nx * nz * ny);
       if (p.MetricBF16 == 2)
-        hipLaunchKernelGGL((metric_media_kernel<uint16_t, 16>), dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
+        hipLaunchKernelGGL((metric_media_kernel<uint16_t, 18>), dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
       else if (p.MetricBF16)
         hipLaunchKernelGGL(metric_media_kernel<uint16_t>, dim3(blocks), dim3(256), 0, 0, md, (uint16_t*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
       else
         hipLaunchKernelGGL(metric_media_kernel<float>, dim3(blocks), dim3(256), 0, 0, md, (float*)met, pyz, d.py, d.jpad, dvp, ref, 0, nx);
+      // the media palette: a 1-D layered model (one entry per layer) or the
+      // two halves of a bimaterial one, used when every point's FP32 media
+      // words equal its entry's
+      c.met_pal = nullptr;
+      c.met_npal = 0;
+      const int npal = md.nl > 0 ? md.nl : 2;
+      if (ok && p.MetricBF16 == 2 && p.MetricPalette && c.m3d_vp.empty() && npal <= kMaxMediaPalette) {
+        float4* pal = dmalloc<float4>(c, npal);
+        unsigned* bad = (unsigned*)dput(nullptr, sizeof(unsigned));
+        unsigned nbad = 1;
+        if (pal && bad && hipMemset(bad, 0, sizeof(unsigned)) == hipSuccess) {
+          for (int check = 0; check < 2; check++)
+            hipLaunchKernelGGL(media_palette_kernel, dim3(blocks), dim3(256), 0, 0, (const uint16_t*)met, pyz, d.py, d.jpad, nx,
+                               ny, nz, pal, npal, check, bad);
+          if (hipGetLastError() != hipSuccess || hipMemcpy(&nbad, bad, sizeof nbad, hipMemcpyDeviceToHost) != hipSuccess)
+            ok = false;
+        }
+        if (nbad == 0) { c.met_pal = (float*)pal; c.met_npal = npal; }
+      }
       if (p.MetricBF16 && p.Rupture) {
         // FP32 metric and media of the planes the fault kernels read (i0-6..i0+6):
         // the fault normal and traction stay FP32 under bfloat16 storage

@@ -101,30 +101,37 @@ __device__ __forceinline__ float mat_(const float* M, int c, int x, uint32_t jk,
 __device__ __forceinline__ float mat_(const uint16_t* M, int c, int x, uint32_t jk, uint32_t nyz, const float*) {
   return __uint_as_float(((unsigned)M[((size_t)x * 13 + c) * nyz + jk]) << 16);
 }
-// Metric in bfloat16 with the media words exact (16 16-bit slots per
+// Metric in bfloat16 with the media words exact (18 16-bit slots per
 // plane): the bfloat16 of each metric word's deviation from a reference
 // value of its field (ref[c], the midrange of the field over the grid), so
 // the rounding scales with the field's variation rather than its size,
 // packed two to a 32-bit word (slots 2p and 2p+1 form 32-bit plane p, field
-// 2p in the low half); the slots 10-15 hold lambda, mu and rho as three
-// FP32 planes.
+// 2p in the low half).  Slots 0-8 hold the nine metric coefficients, slot 9
+// the point's entry in the media palette (lambda, mu, rho of the few
+// distinct media of a layered or two-half model, see WaveArgs::pal; 0 when
+// the media have no palette), slots 10-15 lambda, mu and rho as three FP32
+// planes, slot 16 the Jacobian (its deviation, field 9 of ref), slot 17 0.
 struct MetBf16Delta {
   uint16_t h;
 };
+constexpr uint32_t kMetDeltaSlots = 18;
+constexpr uint32_t kMetDeltaJac = 16;  // slot of the Jacobian
 template <class MT>
 constexpr uint32_t met_nf() {
-  return std::is_same<MT, MetBf16Delta>::value ? 16u : 13u;
+  return std::is_same<MT, MetBf16Delta>::value ? kMetDeltaSlots : 13u;
 }
 __device__ __forceinline__ float mat_(const MetBf16Delta* Ms, int c, int x, uint32_t jk, uint32_t nyz, const float* ref) {
   const uint16_t* M = &Ms->h;
-  if (c < 10) {
-    const uint32_t u = ((const uint32_t*)(M + ((size_t)x * 16 + (c & ~1)) * nyz))[jk];
-    return __uint_as_float((c & 1) ? (u & 0xffff0000u) : (u << 16)) + ref[c];
+  if (c < 10) {  // field c of the pair at slot c & ~1 (c < 9), or the Jacobian in the low half at its slot
+    const uint32_t s = c == 9 ? kMetDeltaJac : (uint32_t)(c & ~1);
+    const uint32_t u = ((const uint32_t*)(M + ((size_t)x * kMetDeltaSlots + s) * nyz))[jk];
+    return __uint_as_float((c < 9 && (c & 1)) ? (u & 0xffff0000u) : (u << 16)) + ref[c];
   }
-  return ((const float*)(M + ((size_t)x * 16 + 10) * nyz))[(size_t)(c - 10) * nyz + jk];
+  return ((const float*)(M + ((size_t)x * kMetDeltaSlots + 10) * nyz))[(size_t)(c - 10) * nyz + jk];
 }
-// Field c of a metric plane resource r at byte column voff (mpb bytes a
-// field); ref is the field's reference value in the deviation form.
+// Field c (a metric coefficient 0..8 or a media word 10..12) of a metric
+// plane resource r at byte column voff (mpb bytes a field); ref is the
+// field's reference value in the deviation form.
 template <class MT>
 __device__ __forceinline__ float bldmc(Rsrc r, uint32_t voff, int c, uint32_t mpb, float ref) {
   if constexpr (std::is_same<MT, MetBf16Delta>::value) {
@@ -165,6 +172,11 @@ struct WaveArgs {
   int nwork, ni, nj, nk, ny, ii0;
   float rDH, rka, rkb;
   float mref[10];   // metric reference values (bfloat16-deviation form)
+  // bfloat16-deviation form: lambda, mu, rho (and 0) of each media palette
+  // entry, or null when the media have no palette (the pipelined kernels
+  // then read the FP32 media planes)
+  const float4* __restrict__ pal;
+  int npal;
   int irk, freenode, cerjan, fault, rhs_only, pml;
   int aux_fused;  // BAND tiles advance their points' ADE-PML auxiliary fields themselves
 };
@@ -515,18 +527,25 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
 
 
 // Raw metric words of one point (MetricBF16=2: five 32-bit pairs of bfloat16
-// deviations and the FP32 lambda, mu, rho; FP32: the 12 words xix..ztz, lam,
-// mu, rho; bfloat16: the same 12 as 16-bit words), held undecoded while they
-// wait for their plane.
+// deviations, the fifth with the palette entry in its high half, and, when
+// the media have no palette, the FP32 lambda, mu, rho; FP32: the 12 words
+// xix..ztz, lam, mu, rho; bfloat16: the same 12 as 16-bit words), held
+// undecoded while they wait for their plane.
 template <class MT>
 constexpr int met_raw_n() { return std::is_same<MT, MetBf16Delta>::value ? 8 : 12; }
 template <class MT>
-__device__ __forceinline__ void met_raw_load(Rsrc r, uint32_t voff, uint32_t mpb, uint32_t (&w)[met_raw_n<MT>()]) {
+__device__ __forceinline__ void met_raw_load(Rsrc r, uint32_t voff, uint32_t mpb, uint32_t (&w)[met_raw_n<MT>()],
+                                             bool palette) {
   if constexpr (std::is_same<MT, MetBf16Delta>::value) {
 #pragma unroll
     for (int p = 0; p < 5; p++) w[p] = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(2 * p) * mpb));
+    if (palette) {
 #pragma unroll
-    for (int m = 0; m < 3; m++) w[5 + m] = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(10 + 2 * m) * mpb));
+      for (int m = 0; m < 3; m++) w[5 + m] = 0u;
+    } else {
+#pragma unroll
+      for (int m = 0; m < 3; m++) w[5 + m] = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(10 + 2 * m) * mpb));
+    }
   } else if constexpr (sizeof(MT) == 2) {
 #pragma unroll
     for (int q = 0; q < 12; q++)
@@ -536,11 +555,19 @@ __device__ __forceinline__ void met_raw_load(Rsrc r, uint32_t voff, uint32_t mpb
     for (int q = 0; q < 12; q++) w[q] = __float_as_uint(bld<kStream>(r, voff, (uint32_t)(q < 9 ? q : q + 1) * mpb));
   }
 }
-// Field c (0..8 metric, 10..12 media) of raw metric words.
+// Field c (0..8 metric, 10..12 media) of raw metric words; pal is the
+// palette in LDS, or null.
 template <class MT>
-__device__ __forceinline__ float met_raw_get(const uint32_t (&w)[met_raw_n<MT>()], int c, const float* ref) {
+__device__ __forceinline__ float met_raw_get(const uint32_t (&w)[met_raw_n<MT>()], int c, const float* ref,
+                                             const float4* pal) {
   if constexpr (std::is_same<MT, MetBf16Delta>::value) {
-    if (c >= 10) return __uint_as_float(w[5 + c - 10]);
+    if (c >= 10) {
+      if (pal) {
+        const float4 e = pal[w[4] >> 16];
+        return c == 10 ? e.x : c == 11 ? e.y : e.z;
+      }
+      return __uint_as_float(w[5 + c - 10]);
+    }
     const uint32_t u = w[c >> 1];
     return __uint_as_float((c & 1) ? (u & 0xffff0000u) : (u << 16)) + ref[c];
   } else {
@@ -608,6 +635,8 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   __shared__ int s_on[6];
   __shared__ int s_N;
   __shared__ const float* s_mV[2];  // TOP: the free-surface velocity maps
+  // the media palette of the bfloat16-deviation form, staged once per block
+  __shared__ float4 s_pal[std::is_same<MT, MetBf16Delta>::value ? kMaxMediaPalette : 1];
 
   const int item = xcd_remap((int)blockIdx.x, a.nwork);
   const int4 wk = a.work[2 * item], we = a.work[2 * item + 1];
@@ -723,8 +752,11 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
       store_halo(lds[m], hv);
     }
   }
+  const bool palette = std::is_same<MT, MetBf16Delta>::value && a.pal != nullptr;  // uniform
+  if (palette)
+    for (int e = tid; e < a.npal; e += NT_) s_pal[e] = a.pal[e];
   uint32_t mw[NMW];
-  met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw);
+  met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw, palette);
   float rb1 = 1.0f, cjx = 1.0f;
   if constexpr (BAND) { rb1 = rb_x(ibeg); cjx = cj_x(ibeg); }
   __syncthreads();
@@ -746,7 +778,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     }
     if (!RING && ahead) load_halo(plane_rsrc(Win, i + 4, 9, nyz), hv);
     uint32_t mwn[NMW];
-    met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn);
+    met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn, palette);
     float rb1n = 1.0f, cjxn = 1.0f;
     if constexpr (BAND) {
       if (more) { rb1n = rb_x(i + 1); cjxn = cj_x(i + 1); }
@@ -831,7 +863,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     auto aux_y = [&](int ax, int c) { return ax == fa ? ya[c] : ax == fb ? sa[c] : ta[c]; };
 
     if (act) {
-      auto MET = [&](int c) { return met_raw_get<MT>(mw, c, a.mref); };
+      auto MET = [&](int c) { return met_raw_get<MT>(mw, c, a.mref, palette ? s_pal : nullptr); };
       const float xix = MET(0), xiy = MET(1), xiz = MET(2);
       const float etx = MET(3), ety = MET(4), etz = MET(5);
       const float ztx = MET(6), zty = MET(7), ztz = MET(8);

@@ -151,8 +151,12 @@ int cgfdm_fault_smooth(cgfdm_ctx* ctx);
  * wavefield, 9 x nx*ny*nz, float), "hW", "M" (13 fields; MetricBF16 = 0),
  * "M_bf16" (MetricBF16 = 1: 13 bfloat16 fields; = 2: 5 int32 fields, each
  * two bfloat16 metric deviations, field 2p in the low half of field p, word =
- * deviation + "M_ref"[field]), "M_media" (MetricBF16 = 2:
- * lambda, mu, rho, float), "M_ref" (10 floats, read only), "FW" (split nodes),
+ * deviation + "M_ref"[field], the high half of field 4 the point's media
+ * palette entry), "M_jac_bf16" (MetricBF16 = 2: the Jacobian's deviation
+ * from "M_ref"[9] in the low half, int32), "M_media" (MetricBF16 = 2:
+ * lambda, mu, rho, float), "M_palette" (MetricBF16 = 2 with a media palette:
+ * lambda, mu, rho, 0 of each entry, float), "M_ref" (10 floats, read only),
+ * "FW" (split nodes),
  * fault 2-D outputs "Vs1" "Vs2" "Tn" "Ts1" "Ts2" "slip" "State" "init_t0"
  * "rup_sensor" ..., PML aux "PMLx1".."PMLz2".  dtype: 0 float32, 1 float64,
  * 2 int32, 3 bfloat16 (raw 16-bit words). */

@@ -225,13 +225,16 @@ def test_metric_bf16_storage():
 
 def test_metric_bf16_delta_storage():
     """MetricBF16=2 stores the metric words (fields 0-9) as the bfloat16 of
-    their deviation from a reference value per field, two to a 32-bit word
-    (exported as 5 int32 fields, field 2p in the low half; the midrange of the
-    field over every 4th stored point along each axis, and the last, in
-    FP32, is the reference) and lambda, mu, rho as FP32 planes.  Decoded, each metric word lies
-    within one bfloat16 rounding of its deviation (2^-8 |m - ref|, plus FP32
-    rounding) of the FP32-stored word, and the media words equal the
-    FP32-stored ones bit for bit.  A 40-step slip-weakening rupture on a
+    their deviation from a reference value per field (the nine coefficients
+    two to a 32-bit word, exported as 5 int32 fields with field 2p in the
+    low half and the media palette entry in the high half of the fifth, the
+    Jacobian in a word of its own; the midrange of the field over every 4th
+    stored point along each axis, and the last, in FP32, is the reference),
+    lambda, mu, rho as FP32 planes, and a palette of the distinct media.
+    Decoded, each metric word lies within one bfloat16 rounding of its
+    deviation (2^-8 |m - ref|, plus FP32 rounding) of the FP32-stored word,
+    the media words equal the FP32-stored ones bit for bit, and so does
+    each point's palette entry.  A 40-step slip-weakening rupture on a
     curved grid then lies within 2.7e-6 (slip) and 1.7e-7 (Ts1, Tn) of the
     FP64 oracle, against 6.5e-4 and 2.5e-5 with every word in bfloat16;
     the measured errors are printed."""
@@ -241,6 +244,8 @@ def test_metric_bf16_delta_storage():
     shp = (d["nx"], d["nz"], d["ny"])
     pairs = s.get("M_bf16").view(np.uint32).reshape((5,) + shp)  # field 2p low, 2p+1 high
     m2 = np.stack([pairs & 0xffff, pairs >> 16], axis=1).reshape((10,) + shp)
+    entry = m2[9].copy()  # the high half of word 4 holds the media palette entry, the Jacobian has its own word
+    m2[9] = s.get("M_jac_bf16").view(np.uint32).reshape(shp) & 0xffff
     t = dyn.Solver(dict(case["params"], MetricBF16=0))
     cases.apply_to_solver(case, t)
     t.setup()
@@ -258,7 +263,13 @@ def test_metric_bf16_delta_storage():
     plain = (((m0[:10].astype(np.float32).view(np.uint32) + 0x8000) >> 16) << 16).view(np.float32)
     print("max metric error: deviation form", float(err.max()), "plain bfloat16 (about)",
           float(np.abs(plain.astype(np.float64) - m0[:10]).max()))
-    np.testing.assert_array_equal(s.get("M_media").reshape((3,) + shp), m0[10:13].astype(np.float32))
+    media = s.get("M_media").reshape((3,) + shp)
+    np.testing.assert_array_equal(media, m0[10:13].astype(np.float32))
+    # the two halves of the (bimaterial-form) medium: a palette of two
+    # entries, each point's entry holding its own FP32 media words
+    pal = s.get("M_palette").reshape(-1, 4)
+    assert pal.shape[0] == 2 and entry.max() <= 1
+    np.testing.assert_array_equal(np.moveaxis(pal[entry][..., :3], -1, 0), media)
     for it in range(40):
         s.step(it)
         o.step(it)
@@ -436,3 +447,32 @@ def test_thermpress_needs_friction_type_3():
     assert e.value.code == dyn.ESTATE
     with pytest.raises(dyn.CgfdmError):  # no thermal-pressurization state to read
         s.field_info("TP_T")
+
+
+@pytest.mark.parametrize("media", ["two_half", "layered"])
+def test_media_palette_same_result(media):
+    """With MetricBF16=2 the stage kernels read lambda, mu, rho from the
+    media palette (MetricPalette 1) or from the FP32 planes (0): the same
+    words, so the same wavefield and fault fields bit for bit after 10 steps
+    of a slip-weakening rupture on a curved grid (a two-half medium, and
+    the 3-layer 1-D model), with and without the absorbing layers'
+    pipelined band kernel."""
+    from dynfd3dcuda_amd import configs
+    case = cases.sw_case(curved=True, MetricBF16=2)
+    if media == "layered":
+        case["media1d"] = configs.three_layer_media()
+    runs = []
+    for pal in (0, 1):
+        s = dyn.Solver(dict(case["params"], MetricPalette=pal))
+        cases.apply_to_solver(case, s)
+        s.setup()
+        npal = s.field_info("M_palette")[0] if pal else 0
+        for it in range(10):
+            s.step(it)
+        s.synchronize()
+        runs.append({n: s.get(n) for n in ("W", "Vs1", "Ts1", "Tn", "slip")})
+        s.close()
+    assert npal == 4 * (3 if media == "layered" else 2)
+    assert np.max(np.abs(runs[0]["Vs1"])) > 0
+    for n in runs[0]:
+        np.testing.assert_array_equal(runs[0][n], runs[1][n], err_msg=n)
