@@ -143,15 +143,25 @@ __device__ __forceinline__ float bldmc(Rsrc r, uint32_t voff, int c, uint32_t mp
 }
 
 // RK_Syn combine (rk4.cu:20-33) of field c of the planes rs (S) and rt (T).
+// Stage 0 sets the accumulator T = S + (dt/6) h0 beside the stage value
+// W1 = S + (dt/2) h0, so T = S + (W1 - S) / 3: stage 1, which reads S and
+// W1 anyway, derives it (to FP32 rounding) instead of stage 0 writing it and
+// stage 1 reading it, 18 of the 144 wavefield words of a step.  A point
+// whose stage-0 value changes after the combine (Cerjan-damped, or on the
+// fault plane, where fault2wave rewrites it) keeps the stored accumulator:
+// keep_t.  The same point takes the same branch in every stage.
+__device__ __forceinline__ float rk_derived_t(float s, float w1) { return s + (w1 - s) * (1.0f / 3.0f); }
 __device__ __forceinline__ float rk_combine_b(int irk, float rka, float rkb, float w, float h, Rsrc rs, Rsrc rt,
-                                              uint32_t voff, uint32_t soff) {
+                                              uint32_t voff, uint32_t soff, bool keep_t) {
   if (irk == 0) {
-    bst(rt, voff, soff, w + rkb * h);
+    if (keep_t) bst(rt, voff, soff, w + rkb * h);
     return w + rka * h;
   }
   if (irk == 3) return bld<kStream>(rt, voff, soff) + rkb * h;
-  bst(rt, voff, soff, bld<kStream>(rt, voff, soff) + rkb * h);
-  return bld<kStream>(rs, voff, soff) + rka * h;
+  const float s = bld<kStream>(rs, voff, soff);
+  const float t = irk == 1 && !keep_t ? rk_derived_t(s, w) : bld<kStream>(rt, voff, soff);
+  bst(rt, voff, soff, t + rkb * h);
+  return s + rka * h;
 }
 
 template <class MT>
@@ -497,11 +507,12 @@ __global__ void __launch_bounds__(NT_, KIND == 2 ? 2 : 4) wave_stage_kernel(Wave
         for (int c = 0; c < 9; c++) bst(r, col4, c * pb, h[c]);
       } else {
         const float damp = cerjan ? fminf(s_cjx[i + 3], cjk) : 1.0f;
+        const bool keep_t = damp != 1.0f || (a.fault && i == a.ii0);
         const Rsrc rs = plane_rsrc(a.S, i + 3, 9, nyz), rt = plane_rsrc(a.T, i + 3, 9, nyz);
         const Rsrc ro = plane_rsrc(a.Wout, i + 3, 9, nyz);
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-          const float w = rk_combine_b(a.irk, a.rka, a.rkb, q[CTR][c], h[c], rs, rt, col4, c * pb);
+          const float w = rk_combine_b(a.irk, a.rka, a.rkb, q[CTR][c], h[c], rs, rt, col4, c * pb, keep_t);
           bst(ro, col4, c * pb, w * damp);
         }
       }
@@ -783,15 +794,21 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     if constexpr (BAND) {
       if (more) { rb1n = rb_x(i + 1); cjxn = cj_x(i + 1); }
     }
-    // plane i: the step-start (S) and accumulator (T) words of the RK combine
+    // plane i: the step-start (S) and accumulator (T) words of the RK
+    // combine; stage 1 derives T from S and the point's stage-0 value
+    // unless the point keeps it (rk_combine_b)
     const bool slab = a.fault && i >= a.ii0 - 3 && i <= a.ii0 + 3 && i != a.ii0;
+    const float damp = cerjan ? fminf(cjx, cjk) : 1.0f;
+    const bool keep_t = damp != 1.0f || (a.fault && i == a.ii0);
     const Rsrc rs = plane_rsrc(a.S, i + 3, 9, nyz), rt = plane_rsrc(a.T, i + 3, 9, nyz);
     float sv[9], tv[9];
 #pragma unroll
     for (int c = 0; c < 9; c++) { sv[c] = 0.0f; tv[c] = 0.0f; }
     if (!a.rhs_only && !slab && irk != 0) {
+      if (irk != 1 || keep_t) {
 #pragma unroll
-      for (int c = 0; c < 9; c++) tv[c] = bld<kStream>(rt, col4, c * pb);
+        for (int c = 0; c < 9; c++) tv[c] = bld<kStream>(rt, col4, c * pb);
+      }
       if (irk != 3) {
 #pragma unroll
         for (int c = 0; c < 9; c++) sv[c] = bld<kStream>(rs, col4, c * pb);
@@ -995,17 +1012,17 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
         // RK_Syn (rk4.cu:20-33); the damping factor is 1 in the interior
         const Rsrc ro = plane_rsrc(a.Wout, i + 3, 9, nyz);
         const float rka = a.rka, rkb = a.rkb;
-        const float damp = cerjan ? fminf(cjx, cjk) : 1.0f;
 #pragma unroll
         for (int c = 0; c < 9; c++) {
           float w;
           if (irk == 0) {
-            bst(rt, col4, c * pb, q[CTR][c] + rkb * h[c]);
+            if (keep_t) bst(rt, col4, c * pb, q[CTR][c] + rkb * h[c]);
             w = q[CTR][c] + rka * h[c];
           } else if (irk == 3) {
             w = tv[c] + rkb * h[c];
           } else {
-            bst(rt, col4, c * pb, tv[c] + rkb * h[c]);
+            const float t = irk == 1 && !keep_t ? rk_derived_t(sv[c], q[CTR][c]) : tv[c];
+            bst(rt, col4, c * pb, t + rkb * h[c]);
             w = sv[c] + rka * h[c];
           }
           bst(ro, col4, c * pb, w * damp);
