@@ -23,8 +23,11 @@ slip/state accumulation.  Prints one JSON line:
                  solver's stream around its own launches over the timed
                  region; achieved = algorithmic bytes per launch
                  (points x B_alg/4; B_alg = 768 B/pt-step FP32, 696 B with
-                 the deviation-form metric, 656 B with its media palette) /
-                 average launch time; traffic =
+                 the deviation-form metric, 656 B with its media palette:
+                 the reference RK4 structure's 144 wavefield words and the
+                 coefficient words as stored) / average launch time (the
+                 kernels move 18 wavefield words fewer, B_moved, since stage
+                 1 derives stage 0's accumulator); traffic =
                  HBM bytes per launch from the committed rocprofv3
                  FETCH/WRITE summary of the same workload; stream ceilings =
                  a hand-written streaming kernel's GB/s for read:write mixes;
@@ -80,6 +83,12 @@ def b_alg(params, palette=False):
     return 144 * 4 + (48 * 4 if not m else 48 * 2)
 
 
+# wavefield bytes per interior point-step the stage kernels do not move
+# against B_alg's RK4 structure: the accumulator of stage 0 is derived in
+# stage 1 from the step start and the stage-0 value, so stage 0 writes 9
+# words fewer and stage 1 reads 9 fewer
+DERIVED_T_SAVING = 18 * 4
+
 # algorithmic bytes of the ADE-PML auxiliary fields per (point, face) pair
 # and RK stage (pmlx.cu:126-210, rk4.cu:154-257): the 9 auxiliary words read
 # for the -aux/beta term of the stage; the ODE step's step-start and
@@ -87,7 +96,9 @@ def b_alg(params, palette=False):
 # 3: 11.25 on average) and its writes (auxiliary, step-start and
 # accumulator words in stage 0, auxiliary and accumulator in 1 and 2, the
 # auxiliary words in 3: 18 on average), its own read of the 9 auxiliary
-# words being the one above
+# words being the one above; like B_alg these follow the reference's RK4
+# structure (the kernels derive stage 0's accumulator in stage 1 here too,
+# 18 words per pair-step fewer)
 AUX_READ_BYTES = 36
 AUX_UPDATE_BYTES = 117
 
@@ -298,6 +309,7 @@ def timed_run(case, steps, warmup, workload):
     achieved_aux = (pts * ba / 4.0 + rd * AUX_READ_BYTES + up * AUX_UPDATE_BYTES) / avg_s / 1e9
     return {"value": value, "ms_per_step": 1e3 * t / steps, "npts": npts, "B_alg": ba, "setup_s": t_setup,
             "step_frac": value * ba / (PEAK_HBM_GBS * 1e9), "dom": dom, "achieved": achieved,
+            "B_moved": ba - DERIVED_T_SAVING,
             "avg_launch_ms": avg_s * 1e3, "points_per_launch": pts, "traffic": read_traffic(workload, dom),
             "kernel_ms_per_step": {k: v[0] / PROFILE_STEPS for k, v in prof_all.items() if v[1] > 0},
             "stage_points_per_launch": {k: v[2] for k, v in stage.items()}, "newton": newton,
@@ -440,6 +452,8 @@ def main():
                    if p.get("Friction_type", 0) >= 1 else None,
                    "network": os.path.relpath(p["NewtonMLP"], ROOT) if p.get("NewtonMLP") else None,
                    "B_alg_bytes_per_pt_step": r["B_alg"], "step_hbm_roofline_frac": r["step_frac"],
+                   "B_moved_bytes_per_pt_step": r["B_moved"],
+                   "step_hbm_frac_of_bytes_moved": r["value"] * r["B_moved"] / (PEAK_HBM_GBS * 1e9),
                    "setup_s": round(r["setup_s"], 1), "stage_points_per_launch": r["stage_points_per_launch"],
                    "aux_pairs_per_launch": r["aux_pairs_per_launch"],
                    "aux_bytes_per_pair": {"read": AUX_READ_BYTES, "update": AUX_UPDATE_BYTES},

@@ -855,7 +855,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
           const uint32_t pa4 = pa4s[fa];
 #pragma unroll
           for (int c = 0; c < 9; c++) ya[c] = ldo(s_auxY[f] + c * ps, pa4);
-          if (nfc == 1 && fuse && irk != 0) {
+          if (nfc == 1 && fuse && (irk == 2 || irk == 3)) {  // stage 1 derives the accumulator
 #pragma unroll
             for (int c = 0; c < 9; c++) ta[c] = ldo(s_auxT[f] + c * ps, pa4);
           }
@@ -985,14 +985,14 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
                 const float y = aux_y(ax, c);
                 const float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
                 float w;
-                if (irk == 0) {
+                if (irk == 0) {  // the accumulator is derived in stage 1 (rk_derived_t)
                   sto(S + c * ps, pa4, y);
-                  sto(T + c * ps, pa4, y + rkb * ah);
                   w = y + rka * ah;
                 } else if (irk == 3) {
                   w = ta[c] + rkb * ah;
                 } else {
-                  sto(T + c * ps, pa4, ta[c] + rkb * ah);
+                  const float t = irk == 1 ? rk_derived_t(sa[c], y) : ta[c];
+                  sto(T + c * ps, pa4, t + rkb * ah);
                   w = sa[c] + rka * ah;
                 }
                 sto(Y + c * ps, pa4, w);
@@ -1165,8 +1165,22 @@ __global__ void __launch_bounds__(256) pml_aux_kernel(PmlAuxArgs<MT> a) {
     const float y = Y[q];
     float ah = ad1 * y + d1 * axis_rhs_c(c, dd, e0, e1, e2, rrho, lam, l2m, mu);
     if (fsx && c >= 3) ah += pml_fs_term(c, z1, ztx, zty, ztz, lam, l2m, mu, c0);
-    if (a.irk == 0) S[q] = y;
-    Y[q] = rk_combine(a.irk, a.rka, a.rkb, y, ah, S, T, q);
+    // RK_Syn (rk4.cu:20-33) on the step start S and accumulator T of the
+    // auxiliary field, the accumulator of stage 0 derived in stage 1 from S
+    // and the stage-0 value (rk_derived_t), as in the band tiles
+    float w;
+    if (a.irk == 0) {
+      S[q] = y;
+      w = y + a.rka * ah;
+    } else if (a.irk == 3) {
+      w = T[q] + a.rkb * ah;
+    } else {
+      const float s = S[q];
+      const float t = a.irk == 1 ? rk_derived_t(s, y) : T[q];
+      T[q] = t + a.rkb * ah;
+      w = s + a.rka * ah;
+    }
+    Y[q] = w;
   }
 }
 
