@@ -72,14 +72,16 @@ CPU_SUB_NX = 100       # fault-normal planes of the CPU baseline's sub-slab of t
 NEWTON_MLP = os.path.join(ROOT, "dynfd3dcuda_amd", "data", "newton_guess_4x64x64x1.f32")
 
 
-def b_alg(params, palette=False):
+def b_alg(params, palette=False, skip=0):
     """Algorithmic bytes per grid-point update (one full RK4 step): 144
     wavefield words + 48 coefficient words (12 per stage: FP32; bfloat16;
     or 9 bfloat16 metric deviations and 3 exact FP32 media words, or with
-    a media palette the deviations and a 16-bit palette entry)."""
+    a media palette the deviations and a 16-bit palette entry; less the 4
+    bytes of each 32-bit word of deviations that is 0 on the whole grid,
+    which the stage kernels do not load: bits of skip)."""
     m = params.get("MetricBF16", 0)
     if m == 2:
-        return 144 * 4 + 4 * (9 * 2 + (2 if palette else 3 * 4))
+        return 144 * 4 + 4 * (9 * 2 + (2 if palette else 3 * 4) - 4 * bin(skip).count("1"))
     return 144 * 4 + (48 * 4 if not m else 48 * 2)
 
 
@@ -298,9 +300,10 @@ def timed_run(case, steps, warmup, workload):
     finite = bool(np.all(np.isfinite(solver.get("Vs1"))) and np.all(np.isfinite(solver.get("Ts1"))))
     aux = solver.profile_aux()
     palette = solver.media_palette_entries() > 0
+    skip = solver.metric_skip_mask()
     solver.close()
     value = npts * steps / t
-    ba = b_alg(p, palette)
+    ba = b_alg(p, palette, skip)
     ms, n, pts = prof[dom]
     avg_s = ms / n / 1e3
     achieved = pts * ba / 4.0 / avg_s / 1e9  # one launch = one RK stage of its points
@@ -314,6 +317,7 @@ def timed_run(case, steps, warmup, workload):
             "kernel_ms_per_step": {k: v[0] / PROFILE_STEPS for k, v in prof_all.items() if v[1] > 0},
             "stage_points_per_launch": {k: v[2] for k, v in stage.items()}, "newton": newton,
             "aux_pairs_per_launch": aux, "achieved_with_aux": achieved_aux, "palette": palette,
+            "metric_words_skipped": bin(skip).count("1"),
             "step_frac_with_aux": (value * ba + 4 * aux_bytes * steps / t) / (PEAK_HBM_GBS * 1e9),
             "fault_outputs_finite": finite}
 
@@ -453,6 +457,7 @@ def main():
                    "network": os.path.relpath(p["NewtonMLP"], ROOT) if p.get("NewtonMLP") else None,
                    "B_alg_bytes_per_pt_step": r["B_alg"], "step_hbm_roofline_frac": r["step_frac"],
                    "B_moved_bytes_per_pt_step": r["B_moved"],
+                   "metric_words_skipped": r["metric_words_skipped"],
                    "step_hbm_frac_of_bytes_moved": r["value"] * r["B_moved"] / (PEAK_HBM_GBS * 1e9),
                    "setup_s": round(r["setup_s"], 1), "stage_points_per_launch": r["stage_points_per_launch"],
                    "aux_pairs_per_launch": r["aux_pairs_per_launch"],

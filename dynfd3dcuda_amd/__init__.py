@@ -298,6 +298,14 @@ class Solver:
         rc = self.lib.cgfdm_field_info(self.h, b"M_palette", ctypes.byref(n), ctypes.byref(t))
         return n.value // 4 if rc == 0 else 0
 
+    def metric_skip_mask(self):
+        """MetricBF16=2: bit p set when the 32-bit metric word p (coefficient
+        pair p) is 0 at every point and the stage kernels skip loading it."""
+        n, t = ctypes.c_size_t(), ctypes.c_int()
+        if self.lib.cgfdm_field_info(self.h, b"M_skip", ctypes.byref(n), ctypes.byref(t)) != 0:
+            return 0
+        return int(self.get("M_skip")[0])
+
     def get(self, name):
         n, dt = self.field_info(name)
         out = np.empty(n, dtype=dt)

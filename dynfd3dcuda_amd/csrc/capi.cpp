@@ -90,6 +90,9 @@ FieldRef field(Ctx& c, const std::string& n) {
   else if (n == "M_media") {  // MetricBF16 = 2: lambda, mu, rho, the FP32 planes in slots 10-15
     f32(c.p.MetricBF16 == 2 ? c.met : nullptr, 3 * nxyz); r.nf = 9; r.planes = nx; r.f0 = 5; r.fn = 3;
   }
+  else if (n == "M_skip") {  // MetricBF16 = 2: bit p set when 32-bit word p of the metric is +-0 everywhere (not loaded)
+    r.ptr = c.p.MetricBF16 == 2 ? &c.met_skip : nullptr; r.count = 1; r.dtype = 2; r.host = true;
+  }
   else if (n == "M_ref") { r.ptr = c.p.MetricBF16 == 2 ? c.met_ref : nullptr; r.count = 10; r.dtype = 0; r.host = true; }
   else if (n == "matVx2Vz") f32(c.matVx2Vz, (size_t)c.d.nx * c.d.ny * 9);
   else if (n == "matVy2Vz") f32(c.matVy2Vz, (size_t)c.d.nx * c.d.ny * 9);

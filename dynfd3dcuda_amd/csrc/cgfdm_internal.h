@@ -41,6 +41,8 @@ struct Params {
   // extensions
   int MetricPalette = 1;  // MetricBF16 = 2: lambda, mu, rho as entries of a palette of the distinct media (a 1-D layered
                           // or two-half model of at most 256 media) instead of FP32 planes; 0: always the planes
+  int MetricSkipConstant = 1;  // MetricBF16 = 2: coefficient words equal to their reference on the whole grid are not
+                               // loaded by the stage kernels (0: every word loaded)
   int MetricBF16 = 0;   // 1: metric + media stored in bfloat16; 2: metric as bfloat16 deviations from the Cartesian grid, media exact (FP32 compute)
   int NewtonMode = 0;   // 0: reference NRsearch; 1: residual tolerance + plain fallback
   double NewtonTol = 1e-10;
@@ -218,6 +220,9 @@ struct Ctx {
   // entry; null when the media have none) and its entries
   float* met_pal = nullptr;
   int met_npal = 0;
+  // MetricBF16 = 2: bit p set when both halves of 32-bit word p are +-0 at
+  // every stored point (the stage kernels then skip loading it)
+  unsigned met_skip = 0;
   // bfloat16 storage: FP32 metric and media of the planes i0-6..i0+6 that the
   // fault kernels read, and that copy's base moved back by its first plane
   float* met_fault = nullptr;

@@ -138,7 +138,7 @@ int Params::set(const std::string& k, double v) {
   R_(vs1) R_(vp1) R_(bi_vs1) R_(bi_vp1) R_(bi_rho1) R_(bi_vs2) R_(bi_vp2) R_(bi_rho2)
   R_(viscosity) R_(RupThres) I_(TP_n)
   I_(usePML) I_(useCerjan) I_(FreeSurface) I_(NormalRefine) I_(Rupture) I_(FaultSmooth)
-  I_(MetricBF16) I_(MetricPalette) I_(NewtonMode) R_(NewtonTol) R_(NewtonMLPGate) I_(NewtonFallbackIter) I_(WaveIChunk) I_(WaveBandRound) I_(WaveTallInterior) I_(WaveInteriorKernel) I_(WaveBandKernel) I_(WaveSurfaceKernel) I_(PmlAuxFused) I_(WaveBufferSkew) I_(StageStreams) I_(WaveSurfaceTiles) I_(WaveTopIChunk) I_(WaveBandIChunk)
+  I_(MetricBF16) I_(MetricPalette) I_(MetricSkipConstant) I_(NewtonMode) R_(NewtonTol) R_(NewtonMLPGate) I_(NewtonFallbackIter) I_(WaveIChunk) I_(WaveBandRound) I_(WaveTallInterior) I_(WaveInteriorKernel) I_(WaveBandKernel) I_(WaveSurfaceKernel) I_(PmlAuxFused) I_(WaveBufferSkew) I_(StageStreams) I_(WaveSurfaceTiles) I_(WaveTopIChunk) I_(WaveBandIChunk)
   if (k == "RS_V0") { V0 = v; return 0; }
   if (k == "RS_Vini") { Vini = v; return 0; }
   if (k == "RS_f0") { f0 = v; return 0; }
@@ -162,7 +162,7 @@ int Params::get(const std::string& k, double* v) const {
   G_(mu_s) G_(mu_d) G_(mu_0) G_(Dc) G_(C0) G_(smooth_load_T) G_(f0) G_(V0) G_(Vini) G_(fw) G_(L)
   G_(vs1) G_(vp1) G_(bi_vs1) G_(bi_vp1) G_(bi_rho1) G_(bi_vs2) G_(bi_vp2) G_(bi_rho2)
   G_(viscosity) G_(RupThres) G_(TP_n) G_(usePML) G_(useCerjan) G_(FreeSurface) G_(NormalRefine) G_(Rupture) G_(FaultSmooth)
-  G_(MetricBF16) G_(MetricPalette) G_(NewtonMode) G_(NewtonTol) G_(NewtonMLPGate) G_(NewtonFallbackIter) G_(WaveIChunk) G_(WaveBandRound) G_(WaveTallInterior) G_(WaveInteriorKernel) G_(WaveBandKernel) G_(WaveSurfaceKernel) G_(PmlAuxFused) G_(WaveBufferSkew) G_(StageStreams) G_(WaveSurfaceTiles) G_(WaveTopIChunk) G_(WaveBandIChunk)
+  G_(MetricBF16) G_(MetricPalette) G_(MetricSkipConstant) G_(NewtonMode) G_(NewtonTol) G_(NewtonMLPGate) G_(NewtonFallbackIter) G_(WaveIChunk) G_(WaveBandRound) G_(WaveTallInterior) G_(WaveInteriorKernel) G_(WaveBandKernel) G_(WaveSurfaceKernel) G_(PmlAuxFused) G_(WaveBufferSkew) G_(StageStreams) G_(WaveSurfaceTiles) G_(WaveTopIChunk) G_(WaveBandIChunk)
   G_(ni) G_(nj) G_(nk) G_(nx) G_(ny) G_(nz) G_(NT)
 #undef G_
   if (k.size() == 11 && k.compare(0, 10, "Fault_grid") == 0 && k[10] >= '0' && k[10] <= '3') { *v = Fault_grid[k[10] - '0']; return 0; }

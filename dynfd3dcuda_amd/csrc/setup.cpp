@@ -212,6 +212,30 @@ __global__ void __launch_bounds__(256) media_palette_kernel(const uint16_t* met,
     atomicAdd(bad, 1u);
 }
 
+// OR of each of the five 32-bit words (coefficient pairs, the fifth with
+// the palette entry) of the bfloat16-deviation form over every stored
+// point, sign bits left out: a word that is +-0 everywhere (a coefficient
+// equal to its reference value on the whole grid, as the Cartesian y of a
+// grid bent only in x and z gives for four of the nine) decodes to the
+// reference values, as a word that is not loaded (0) does.
+__global__ void __launch_bounds__(256) metric_pair_or_kernel(const uint16_t* met, long pyz, int py, int jpad, int nx, int ny,
+                                                             int nz, unsigned* any) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  unsigned v[5] = {0u, 0u, 0u, 0u, 0u};
+  if (idx < (long)nx * nz * ny) {
+    const int j = (int)(idx % ny);
+    const long r = idx / ny;
+    const int k = (int)(r % nz), i = (int)(r / nz);
+    const long base = (long)k * py + jpad + j;
+    for (int p = 0; p < 5; p++) v[p] = reinterpret_cast<const uint32_t*>(met + ((long)i * 18 + 2 * p) * pyz)[base] & 0x7fff7fffu;
+  }
+  for (int p = 0; p < 5; p++) {
+    unsigned x = v[p];
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0 && x) atomicOr(any + p, x);
+  }
+}
+
 // ---- setup passes on the device, FP64 (one thread per point or node)
 
 // Curvilinear coordinates of the interior points (construct_coord,
@@ -812,6 +836,19 @@ int host_setup(Ctx& c, std::string& err) {
             ok = false;
         }
         if (nbad == 0) { c.met_pal = (float*)pal; c.met_npal = npal; }
+      }
+      // the coefficient words that are 0 on the whole grid (MetricSkipConstant)
+      c.met_skip = 0;
+      if (ok && p.MetricBF16 == 2 && p.MetricSkipConstant) {
+        unsigned* any = (unsigned*)dput(nullptr, 5 * sizeof(unsigned));
+        unsigned h[5] = {1u, 1u, 1u, 1u, 1u};
+        if (any && hipMemset(any, 0, 5 * sizeof(unsigned)) == hipSuccess) {
+          hipLaunchKernelGGL(metric_pair_or_kernel, dim3(blocks), dim3(256), 0, 0, (const uint16_t*)met, pyz, d.py, d.jpad, nx, ny,
+                             nz, any);
+          if (hipGetLastError() != hipSuccess || hipMemcpy(h, any, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+        }
+        for (int q = 0; q < 5; q++)
+          if (h[q] == 0u) c.met_skip |= 1u << q;
       }
       if (p.MetricBF16 && p.Rupture) {
         // FP32 metric and media of the planes the fault kernels read (i0-6..i0+6):

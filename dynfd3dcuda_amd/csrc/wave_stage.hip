@@ -28,6 +28,7 @@ WaveArgs<MT> make_args(Ctx& c, const StageBufs& b, int irk, bool rhs_only, float
   for (int m = 0; m < 10; m++) a.mref[m] = c.met_ref[m];
   a.pal = std::is_same<MT, MetBf16Delta>::value ? (const float4*)c.met_pal : nullptr;
   a.npal = c.met_npal;
+  a.met_skip = std::is_same<MT, MetBf16Delta>::value ? c.met_skip : 0u;
   const double RA[4] = {0.5, 0.5, 1.0, 1e10}, RB[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
   a.rka = (float)(RA[irk] * c.p.DT);
   a.rkb = (float)(RB[irk] * c.p.DT);

@@ -187,6 +187,7 @@ struct WaveArgs {
   // then read the FP32 media planes)
   const float4* __restrict__ pal;
   int npal;
+  unsigned met_skip;  // bfloat16-deviation form: bit p set when 32-bit metric word p is +-0 on the whole grid
   int irk, freenode, cerjan, fault, rhs_only, pml;
   int aux_fused;  // BAND tiles advance their points' ADE-PML auxiliary fields themselves
 };
@@ -546,10 +547,11 @@ template <class MT>
 constexpr int met_raw_n() { return std::is_same<MT, MetBf16Delta>::value ? 8 : 12; }
 template <class MT>
 __device__ __forceinline__ void met_raw_load(Rsrc r, uint32_t voff, uint32_t mpb, uint32_t (&w)[met_raw_n<MT>()],
-                                             bool palette) {
+                                             bool palette, unsigned skip) {
   if constexpr (std::is_same<MT, MetBf16Delta>::value) {
 #pragma unroll
-    for (int p = 0; p < 5; p++) w[p] = __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(2 * p) * mpb));
+    for (int p = 0; p < 5; p++)  // a word that is +-0 on the whole grid (skip bit p) is not loaded
+      w[p] = (skip >> p) & 1u ? 0u : __float_as_uint(bld<kStream>(r, 2 * voff, (uint32_t)(2 * p) * mpb));
     if (palette) {
 #pragma unroll
       for (int m = 0; m < 3; m++) w[5 + m] = 0u;
@@ -767,7 +769,8 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
   if (palette)
     for (int e = tid; e < a.npal; e += NT_) s_pal[e] = a.pal[e];
   uint32_t mw[NMW];
-  met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw, palette);
+  const unsigned met_skip = a.met_skip;  // uniform
+  met_raw_load<MT>(plane_rsrc(a.met, ibeg + 3, met_nf<MT>(), nyz), mcol, mpb, mw, palette, met_skip);
   float rb1 = 1.0f, cjx = 1.0f;
   if constexpr (BAND) { rb1 = rb_x(ibeg); cjx = cj_x(ibeg); }
   __syncthreads();
@@ -789,7 +792,7 @@ __global__ void __launch_bounds__(NT_, MINB) wave_interior_kernel(WaveArgs<MT> a
     }
     if (!RING && ahead) load_halo(plane_rsrc(Win, i + 4, 9, nyz), hv);
     uint32_t mwn[NMW];
-    met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn, palette);
+    met_raw_load<MT>(plane_rsrc(a.met, min(i + 4, nxp - 1), met_nf<MT>(), nyz), mcol, mpb, mwn, palette, met_skip);
     float rb1n = 1.0f, cjxn = 1.0f;
     if constexpr (BAND) {
       if (more) { rb1n = rb_x(i + 1); cjxn = cj_x(i + 1); }

@@ -155,7 +155,10 @@ int cgfdm_fault_smooth(cgfdm_ctx* ctx);
  * palette entry), "M_jac_bf16" (MetricBF16 = 2: the Jacobian's deviation
  * from "M_ref"[9] in the low half, int32), "M_media" (MetricBF16 = 2:
  * lambda, mu, rho, float), "M_palette" (MetricBF16 = 2 with a media palette:
- * lambda, mu, rho, 0 of each entry, float), "M_ref" (10 floats, read only),
+ * lambda, mu, rho, 0 of each entry, float), "M_skip" (MetricBF16 = 2: bit p
+ * set when both halves of 32-bit metric word p are +-0 at every point, so
+ * the stage kernels do not load it; int32, read only), "M_ref" (10 floats,
+ * read only),
  * "FW" (split nodes),
  * fault 2-D outputs "Vs1" "Vs2" "Tn" "Ts1" "Ts2" "slip" "State" "init_t0"
  * "rup_sensor" ..., PML aux "PMLx1".."PMLz2".  dtype: 0 float32, 1 float64,

@@ -476,3 +476,32 @@ def test_media_palette_same_result(media):
     assert np.max(np.abs(runs[0]["Vs1"])) > 0
     for n in runs[0]:
         np.testing.assert_array_equal(runs[0][n], runs[1][n], err_msg=n)
+
+
+def test_metric_constant_words_skipped_same_result():
+    """With MetricBF16=2 the stage kernels skip loading the 32-bit metric
+    words that are +-0 at every point (a coefficient pair equal to its
+    reference values on the whole grid: on this grid, bent in x only with
+    Cartesian y and z, the pairs (eta_y, eta_z) and (zeta_x, zeta_y)); a
+    skipped word decodes to the same reference values, so 10 steps of a slip-weakening rupture on the
+    curved grid give the same fields bit for bit with MetricSkipConstant 0
+    and 1."""
+    case = cases.sw_case(curved=True, MetricBF16=2)
+    runs, masks = [], []
+    for skip in (0, 1):
+        s = dyn.Solver(dict(case["params"], MetricSkipConstant=skip))
+        cases.apply_to_solver(case, s)
+        s.setup()
+        masks.append(s.metric_skip_mask())
+        pairs = s.get("M_bf16").view(np.uint32).reshape(5, -1)
+        for it in range(10):
+            s.step(it)
+        s.synchronize()
+        runs.append({n: s.get(n) for n in ("W", "Vs1", "Ts1", "Tn", "slip")})
+        s.close()
+    assert masks[0] == 0 and masks[1] != 0, masks
+    for p in range(5):  # the mask names exactly the words that are +-0 in both halves everywhere
+        assert bool((masks[1] >> p) & 1) == (not (pairs[p] & 0x7FFF7FFF).any()), (p, masks[1])
+    assert np.max(np.abs(runs[0]["Vs1"])) > 0
+    for n in runs[0]:
+        np.testing.assert_array_equal(runs[0][n], runs[1][n], err_msg=n)
