@@ -218,21 +218,29 @@ __global__ void __launch_bounds__(256) media_palette_kernel(const uint16_t* met,
 // equal to its reference value on the whole grid, as the Cartesian y of a
 // grid bent only in x and z gives for four of the nine) decodes to the
 // reference values, as a word that is not loaded (0) does.
+// A grid-stride loop over the points, then one atomic per block and word
+// (an atomic per wavefront on the same five words would serialize).
 __global__ void __launch_bounds__(256) metric_pair_or_kernel(const uint16_t* met, long pyz, int py, int jpad, int nx, int ny,
                                                              int nz, unsigned* any) {
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  __shared__ unsigned part[4][5];
   unsigned v[5] = {0u, 0u, 0u, 0u, 0u};
-  if (idx < (long)nx * nz * ny) {
+  const long total = (long)nx * nz * ny;
+  for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (long)gridDim.x * 256) {
     const int j = (int)(idx % ny);
     const long r = idx / ny;
     const int k = (int)(r % nz), i = (int)(r / nz);
     const long base = (long)k * py + jpad + j;
-    for (int p = 0; p < 5; p++) v[p] = reinterpret_cast<const uint32_t*>(met + ((long)i * 18 + 2 * p) * pyz)[base] & 0x7fff7fffu;
+    for (int p = 0; p < 5; p++) v[p] |= reinterpret_cast<const uint32_t*>(met + ((long)i * 18 + 2 * p) * pyz)[base] & 0x7fff7fffu;
   }
   for (int p = 0; p < 5; p++) {
     unsigned x = v[p];
     for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
-    if ((threadIdx.x & 63) == 0 && x) atomicOr(any + p, x);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6][p] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const unsigned x = part[0][threadIdx.x] | part[1][threadIdx.x] | part[2][threadIdx.x] | part[3][threadIdx.x];
+    if (x) atomicOr(any + threadIdx.x, x);
   }
 }
 
@@ -843,8 +851,8 @@ int host_setup(Ctx& c, std::string& err) {
         unsigned* any = (unsigned*)dput(nullptr, 5 * sizeof(unsigned));
         unsigned h[5] = {1u, 1u, 1u, 1u, 1u};
         if (any && hipMemset(any, 0, 5 * sizeof(unsigned)) == hipSuccess) {
-          hipLaunchKernelGGL(metric_pair_or_kernel, dim3(blocks), dim3(256), 0, 0, (const uint16_t*)met, pyz, d.py, d.jpad, nx, ny,
-                             nz, any);
+          hipLaunchKernelGGL(metric_pair_or_kernel, dim3(std::min(blocks, 4096u)), dim3(256), 0, 0, (const uint16_t*)met, pyz,
+                             d.py, d.jpad, nx, ny, nz, any);
           if (hipGetLastError() != hipSuccess || hipMemcpy(h, any, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
         }
         for (int q = 0; q < 5; q++)

@@ -60,7 +60,7 @@ def kernel_class(name):
     if "metric_media_kernel" in name:
         return "setup_metric"
     for k in ("coord_kernel", "coord_ghost_kernel", "hmin_kernel", "gather_kernel", "metric_range_kernel",
-              "fault_coef_kernel", "wave_free_kernel", "media_palette_kernel"):
+              "fault_coef_kernel", "wave_free_kernel", "media_palette_kernel", "metric_pair_or_kernel"):
         if k in name:
             return "setup_other"  # the other setup passes (coordinates, CFL range, coefficients)
     return name.split("(")[0].strip()
