@@ -54,7 +54,8 @@ struct Params {
   int WaveTallInterior = 0;  // 1: interior tiles 64 x 16 in blocks of 1024 threads
   int WaveInteriorKernel = -1;  // interior tiles: -1 automatic; 0 general kernel; pipelined kernel: 1 64 x 8 with
                                 // an LDS plane ring, 2 64 x 6, 3 64 x 4, 4 64 x 8 double-buffered
-  int WaveBandKernel = 2;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles
+  int WaveBandKernel = -1;  // absorbing-band tiles: 0 general kernel; pipelined kernel: 1 64 x 8 tiles, 2 64 x 4 tiles;
+                            // -1: 2 with 64 x 4 interior tiles, else 1
   int WaveSurfaceKernel = 1;  // free-surface tiles: 0 general kernel; 1 pipelined kernel
   int PmlAuxFused = 1;     // 1: the pipelined band tiles advance the ADE-PML auxiliary fields of their
                            // points on one face; 0: a kernel of its own advances every band point's

@@ -153,8 +153,12 @@ int upload_wave_cold(Ctx& c) {
   for (int ax = 0; ax < 3; ax++)
     if (c.pml.on[2 * ax] && c.pml.on[2 * ax + 1] && 2 * c.pml.N > (ax == 0 ? c.d.ni : ax == 1 ? c.d.nj : c.d.nk))
       disjoint = false;
-  c.wave_band_pipe = p.WaveBandKernel >= 1 && disjoint;
-  c.wave_band_tk = c.wave_band_pipe && p.WaveBandKernel == 2 ? TK4 : TK;  // 2: 64 x 4 band tiles, two blocks per CU
+  // -1 (automatic): 64 x 4 band tiles beside the 64 x 4 interior tiles of a
+  // large grid, 64 x 8 ones on a grid too small for those (400 x 300 x 200:
+  // bands 3.28 against 3.35 ms per step)
+  const int wbk = p.WaveBandKernel >= 0 ? p.WaveBandKernel : wik == 3 ? 2 : 1;
+  c.wave_band_pipe = wbk >= 1 && disjoint;
+  c.wave_band_tk = c.wave_band_pipe && wbk == 2 ? TK4 : TK;  // 2: 64 x 4 band tiles, two blocks per CU
   // WaveSurfaceKernel 1: the pipelined kernel on the free-surface tiles too
   // (with the band terms; their auxiliary fields stay with pml_aux_kernel)
   c.wave_surf_pipe = disjoint && p.WaveSurfaceKernel >= 1;

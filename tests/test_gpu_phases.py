@@ -157,14 +157,15 @@ def test_stage_streams_same_result(extra):
                                              ({"WaveInteriorKernel": 4}, True, 150),
                                              ({"WaveBandKernel": 0}, True, 150),
                                              ({"WaveBandKernel": 0, "useCerjan": 1}, True, 150),
-                                             ({"WaveBandKernel": 1, "useCerjan": 1}, True, 150),
+                                             ({"WaveBandKernel": 2, "useCerjan": 1}, True, 150),
+                                             ({"WaveBandKernel": 2}, True, 150),
                                              ({"PmlAuxFused": 0}, True, 150),
                                              ({"WaveSurfaceKernel": 0}, True, 150),
                                              ({"WaveSurfaceKernel": 0, "useCerjan": 1}, True, 150),
                                              ({"WaveBufferSkew": 0}, True, 150)],
                          ids=["band_round", "tall", "band_round_tall", "surface_tiles_off", "general_interior",
                               "interior_64x6", "interior_64x4", "interior_64x8_double_buffer", "general_band",
-                              "general_band_cerjan", "band_64x8_cerjan", "aux_update_separate", "general_surface",
+                              "general_band_cerjan", "band_64x4_cerjan", "band_64x4", "aux_update_separate", "general_surface",
                               "general_surface_cerjan", "buffers_unskewed"])
 def test_tilings_give_the_same_result(layout, exact, ny):
     """The tiling options only move points between the interior, band and
@@ -174,7 +175,8 @@ def test_tilings_give_the_same_result(layout, exact, ny):
     grid with PML and free surface:
     - moves between the interior and band kernels, the general, 64 x 6,
       64 x 4 and double-buffered 64 x 8 interior kernels, the general
-      kernel and 64 x 8 tiles on the absorbing bands
+      kernel and 64 x 4 tiles on the absorbing bands (64 x 8 on this grid
+      by default)
       (with ADE-PML, and with PML and the Cerjan sponge together), the
       ADE-PML auxiliary update of every band point in its own kernel
       instead of in the band tiles (which by default advance the points on
@@ -212,9 +214,9 @@ def test_tilings_give_the_same_result(layout, exact, ny):
             assert e < 1e-5, (name, e)
 
 
-@pytest.mark.parametrize("layout", [{}, {"PmlAuxFused": 0}, {"WaveBandKernel": 0}, {"WaveBandKernel": 1},
+@pytest.mark.parametrize("layout", [{}, {"PmlAuxFused": 0}, {"WaveBandKernel": 0}, {"WaveBandKernel": 2},
                                     {"WaveSurfaceTiles": 0}, {"FreeSurface": 0}],
-                         ids=["default", "aux_separate", "general_band", "band_64x8", "surface_tiles_off",
+                         ids=["default", "aux_separate", "general_band", "band_64x4", "surface_tiles_off",
                               "no_free_surface"])
 def test_aux_pairs_cover_every_face_point_once(layout):
     """The ADE-PML work the launches report (cgfdm_profile_aux) partitions
