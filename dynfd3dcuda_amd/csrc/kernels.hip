@@ -712,32 +712,42 @@ __global__ void __launch_bounds__(256) fault_deriv_kernel(FArgs a, int tiles_j, 
     mv3(VX2, DxV[4], o1); mv3(VY2, DyV[4], o2);
     for (int c = 0; c < 3; c++) DzV[4][c] = o1[c] + o2[c];
   } else {
-    for (int c = 0; c < 3; c++)
-      for (int m = ms; m <= ms; m++) {  // off the surface row each side needs only its own
-        const float* f = fv(c, m);
-        if (a.freenode && km == 2) DzV[3 + m][c] = d22(f, ny, FZ) * rDH;
-        else if (a.freenode && km == 3) DzV[3 + m][c] = d24(f, ny, FZ) * rDH;
-        else DzV[3 + m][c] = (F.rup_index_z[q] % 7 ? d22(f, ny, FZ) : drp(f, ny, FZ)) * rDH;
-      }
-    // DxV_T1: D11^-1 (dT1/dt - D12 DyV - D13 DzV) on both sides
+    for (int c = 0; c < 3; c++) {  // off the surface row each side needs only its own
+      const float* f = fv(c, ms);
+      float dz;
+      if (a.freenode && km == 2) dz = d22(f, ny, FZ) * rDH;
+      else if (a.freenode && km == 3) dz = d24(f, ny, FZ) * rDH;
+      else dz = (F.rup_index_z[q] % 7 ? d22(f, ny, FZ) : drp(f, ny, FZ)) * rDH;
+      if (ms == 0) DzV[3][c] = dz;
+      else DzV[4][c] = dz;
+    }
+  }
+  // this side's derivatives, selected with constant indices (a runtime
+  // index into DxV/DyV/DzV would put them in scratch memory)
+  float dx[3], dy[3], dz[3];
+  for (int c = 0; c < 3; c++) {
+    dx[c] = ms == 0 ? DxV[3][c] : DxV[4][c];
+    dy[c] = ms == 0 ? DyV[3][c] : DyV[4][c];
+    dz[c] = ms == 0 ? DzV[3][c] : DzV[4][c];
+  }
+  if (!(a.freenode && km == 1)) {
+    // DxV_T1: D11^-1 (dT1/dt - D12 DyV - D13 DzV) of this side
     float dtT1[3] = {0.f, 0.f, 0.f};
     if (F.faultgrid[q]) { dtT1[0] = F.hT11[pf]; dtT1[1] = F.hT12[pf]; dtT1[2] = F.hT13[pf]; }
-    for (int m = ms; m <= ms; m++) {
-      float o1[3], o2[3], o3[3];
-      mv3(&F.matT1toVx[(m * nyz + pf) * 9], dtT1, o1);
-      mv3(&F.matVytoVx[(m * nyz + pf) * 9], DyV[3 + m], o2);
-      mv3(&F.matVztoVx[(m * nyz + pf) * 9], DzV[3 + m], o3);
-      for (int c = 0; c < 3; c++) DxV[3 + m][c] = o1[c] - o2[c] - o3[c];
-    }
+    float o1[3], o2[3], o3[3];
+    mv3(&F.matT1toVx[(ms * nyz + pf) * 9], dtT1, o1);
+    mv3(&F.matVytoVx[(ms * nyz + pf) * 9], dy, o2);
+    mv3(&F.matVztoVx[(ms * nyz + pf) * 9], dz, o3);
+    for (int c = 0; c < 3; c++) dx[c] = o1[c] - o2[c] - o3[c];
   }
   // split-node hT2*, hT3* (fault_dstrs_f.cu:874-1007)
   {
     const int m = ms;
     for (int row = 1; row <= 2; row++) {
       float o1[3], o2[3], o3[3];
-      mv3(&F.Dmat[((long)(m * 9 + row * 3 + 0) * nyz + pf) * 9], DxV[3 + m], o1);
-      mv3(&F.Dmat[((long)(m * 9 + row * 3 + 1) * nyz + pf) * 9], DyV[3 + m], o2);
-      mv3(&F.Dmat[((long)(m * 9 + row * 3 + 2) * nyz + pf) * 9], DzV[3 + m], o3);
+      mv3(&F.Dmat[((long)(m * 9 + row * 3 + 0) * nyz + pf) * 9], dx, o1);
+      mv3(&F.Dmat[((long)(m * 9 + row * 3 + 1) * nyz + pf) * 9], dy, o2);
+      mv3(&F.Dmat[((long)(m * 9 + row * 3 + 2) * nyz + pf) * 9], dz, o3);
       for (int c = 0; c < 3; c++) F.hW[(3 * row + c) * 2 * nyz + m * nyz + pf] = o1[c] + o2[c] + o3[c];
     }
   }
