@@ -1217,9 +1217,11 @@ void launch_wave_t(WaveArgs<MT> a, const Ctx& c, hipStream_t s) {
     if (c.wave_tki == TK6)
       hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK6, 2, false, false>), dim3(a.nwork), dim3(TJ, TK6, 1),
                          0, si, a);
-    else if (c.wave_tki == TK4)  // at least 3 waves per SIMD: three blocks per CU
-      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK4, 3, false, false>), dim3(a.nwork), dim3(TJ, TK4, 1),
-                         0, si, a);
+    else if (c.wave_tki == TK4)  // at least 3 waves per SIMD: three blocks per CU (two with FP32 metric words,
+      // whose 12 raw words per point do not fit 168 VGPRs: 148 B per lane of
+      // spills, 98-100 against 94-96 ms of interior per step at 1600 x 1000 x 500)
+      hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, TJ * TK4, std::is_same<MT, float>::value ? 2 : 3, false, false>),
+                         dim3(a.nwork), dim3(TJ, TK4, 1), 0, si, a);
     else if (c.wave_pipe && c.wave_ring)
       hipLaunchKernelGGL((wave_interior_kernel<FX, FY, FZ, MT, TJ, NTHR, 2>), dim3(a.nwork), dim3(TJ, TK, 1), 0, si, a);
     else if (c.wave_pipe)
