@@ -40,7 +40,7 @@ struct Params {
   int FaultSmooth = 0;  // Makefile:104: Gaussian smoothing of the fault tractions once per step
   // extensions
   int MetricPalette = 1;  // MetricBF16 = 2: lambda, mu, rho as entries of a palette of the distinct media (a 1-D layered
-                          // or two-half model of at most 256 media) instead of FP32 planes; 0: always the planes
+                          // or two-half model of at most 64 media) instead of FP32 planes; 0: always the planes
   int MetricSkipConstant = 1;  // MetricBF16 = 2: coefficient words equal to their reference on the whole grid are not
                                // loaded by the stage kernels (0: every word loaded)
   int MetricBF16 = 0;   // 1: metric + media stored in bfloat16; 2: metric as bfloat16 deviations from the Cartesian grid, media exact (FP32 compute)
@@ -78,7 +78,10 @@ int parse_params_json(const std::string& text, Params& p, std::string& err);
 // ---------------------------------------------------------------------------
 // Device-side views.  Volume arrays use the reference index
 // pos = j + k*ny + i*ny*nz over padded dims (3 ghost layers per side).
-constexpr int kMaxMediaPalette = 256;  // media palette entries (MetricBF16 = 2) at most
+// media palette entries (MetricBF16 = 2) at most: 1 KB of LDS per block in
+// the stage kernels (4 KB for 256 entries took the 32 x 8 narrow band tiles
+// to one block per CU)
+constexpr int kMaxMediaPalette = 64;
 
 struct Dims {
   int ni, nj, nk, nx, ny, nz, i0;
